@@ -1,0 +1,320 @@
+/*
+ * morpheus_amd.h -- C-ABI of the MI355X-native Morpheus SpMV hot path.
+ *
+ * Every entry point takes plain device pointers, sizes and enums; no C++ or
+ * torch types cross this boundary.  The library never frees caller memory.
+ * All device work is enqueued on the caller's stream (NULL = the legacy
+ * default stream, which is the ordering the reference HIP backend relies on).
+ *
+ * Storage conventions (the reference's HIP/device layout, LayoutLeft):
+ *   indices int32; values FP64, FP32 or BF16 (mph_value_type);
+ *   x / y vectors: double for FP64 matrices, float for FP32 and BF16 ones;
+ *   COO: row_indices[nnz], column_indices[nnz], values[nnz];
+ *   CSR: row_offsets[nrows+1], column_indices[nnz], values[nnz];
+ *   DIA: diagonal_offsets[ndiag] and values column-major with leading
+ *        dimension `pitch` (pad(nrows,32) by default): values(r,d) =
+ *        values[r + pitch*d] = A(r, r + offset[d]);
+ *   ELL: column_indices / values column-major, leading dimension `pitch`
+ *        (= nrows), `width` stored columns (pad(entries_per_row,32));
+ *        slot (r,n) = [r + pitch*n]; padding slots hold index -1.
+ *   HYB: an ELL part plus a COO part; HDC: a DIA part plus a CSR part.
+ *
+ * Status codes mirror the reference exception taxonomy
+ * (core/src/Morpheus_Exceptions.hpp:32-74); the C++ drop-in headers rethrow
+ * them as Morpheus::NotImplementedException, InvalidInputException,
+ * FormatConversionException and RuntimeException.
+ */
+#ifndef MORPHEUS_AMD_H
+#define MORPHEUS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* hipStream_t-compatible opaque stream handle */
+typedef struct ihipStream_t* mph_stream_t;
+
+enum mph_status {
+  MPH_OK = 0,
+  MPH_ERR_NOT_IMPLEMENTED = 1,   /* Morpheus::NotImplementedException   */
+  MPH_ERR_INVALID_INPUT = 2,     /* Morpheus::InvalidInputException     */
+  MPH_ERR_FORMAT_CONVERSION = 3, /* Morpheus::FormatConversionException */
+  MPH_ERR_HIP = 4,               /* HIP runtime failure                 */
+  MPH_ERR_OUT_OF_MEMORY = 5,
+  MPH_ERR_RUNTIME = 6            /* Morpheus::RuntimeException          */
+};
+
+enum mph_value_type { MPH_F64 = 0, MPH_F32 = 1, MPH_BF16 = 2 };
+
+/* Same order as Morpheus::formats_e (core/src/Morpheus_FormatsRegistry.hpp:56-64) */
+enum mph_format {
+  MPH_COO = 0,
+  MPH_CSR = 1,
+  MPH_DIA = 2,
+  MPH_ELL = 3,
+  MPH_HYB = 4,
+  MPH_HDC = 5,
+  MPH_NFORMATS = 6
+};
+
+/* ------------------------------------------------------------ library */
+
+/* Library version string. */
+const char* mph_version(void);
+/* Text for a status code. */
+const char* mph_status_string(int status);
+/* Message of the last failure on the calling thread ("" if none). */
+const char* mph_last_error(void);
+/* Device-side part of Morpheus::initialize (core/src/impl/Morpheus_Core.cpp:172-194):
+ * selects `device` (-1 = keep current) and caches its properties once, where
+ * the reference re-queries them on every launch
+ * (core/src/impl/Morpheus_HIPUtils.hpp:77-90). */
+int mph_initialize(int device);
+/* Morpheus::finalize (core/src/impl/Morpheus_Core.cpp:215-222). */
+int mph_finalize(void);
+
+/* ------------------------------------------------------------ matrices */
+
+/* A matrix handle describes caller-owned device arrays in one of the six
+ * formats and owns the analysis the kernels need (row blocks, carry
+ * scratch, tuned launch shape).  It is the C form of the reference's
+ * unmanaged containers built over user pointers
+ * (core/src/Morpheus_CsrMatrix.hpp:186-201). */
+typedef struct mph_matrix_s* mph_matrix_t;
+
+/* core/src/Morpheus_CooMatrix.hpp (row_indices, column_indices, values) */
+int mph_matrix_create_coo(mph_matrix_t* out, int64_t nrows, int64_t ncols,
+                          int64_t nnz, const int32_t* row_indices,
+                          const int32_t* column_indices, const void* values,
+                          int value_type);
+/* core/src/Morpheus_CsrMatrix.hpp:534-535 (row_offsets, column_indices, values) */
+int mph_matrix_create_csr(mph_matrix_t* out, int64_t nrows, int64_t ncols,
+                          int64_t nnz, const int32_t* row_offsets,
+                          const int32_t* column_indices, const void* values,
+                          int value_type);
+/* core/src/Morpheus_DiaMatrix.hpp:192-201 (diagonal_offsets, values pad(N,32) x ndiag) */
+int mph_matrix_create_dia(mph_matrix_t* out, int64_t nrows, int64_t ncols,
+                          int64_t nnz, int64_t ndiag, int64_t pitch,
+                          const int32_t* diagonal_offsets, const void* values,
+                          int value_type);
+/* core/src/Morpheus_EllMatrix.hpp:189-202 (column_indices / values N x pad(K,32)) */
+int mph_matrix_create_ell(mph_matrix_t* out, int64_t nrows, int64_t ncols,
+                          int64_t nnz, int64_t entries_per_row, int64_t width,
+                          int64_t pitch, const int32_t* column_indices,
+                          const void* values, int value_type);
+/* core/src/Morpheus_HybMatrix.hpp:101-477 (ell() + coo()) */
+int mph_matrix_create_hyb(mph_matrix_t* out, int64_t nrows, int64_t ncols,
+                          int64_t ell_nnz, int64_t entries_per_row,
+                          int64_t width, int64_t pitch,
+                          const int32_t* ell_column_indices,
+                          const void* ell_values, int64_t coo_nnz,
+                          const int32_t* coo_row_indices,
+                          const int32_t* coo_column_indices,
+                          const void* coo_values, int value_type);
+/* core/src/Morpheus_HdcMatrix.hpp:107-468 (dia() + csr()) */
+int mph_matrix_create_hdc(mph_matrix_t* out, int64_t nrows, int64_t ncols,
+                          int64_t dia_nnz, int64_t ndiag, int64_t pitch,
+                          const int32_t* dia_offsets, const void* dia_values,
+                          int64_t csr_nnz, const int32_t* csr_row_offsets,
+                          const int32_t* csr_column_indices,
+                          const void* csr_values, int value_type);
+/* Releases the analysis owned by the handle (never the caller's arrays). */
+int mph_matrix_destroy(mph_matrix_t A);
+/* Builds the analysis now (multiply builds it lazily on first use).  Must be
+ * called again (or the handle recreated) after the caller changes the
+ * sparsity structure in place. */
+int mph_matrix_analyse(mph_matrix_t A, mph_stream_t stream);
+int mph_matrix_format(mph_matrix_t A);
+
+/* Launch-shape knobs the autotuner searches (0 = library default).
+ *   rows_chunk   : nnz budget of a CSR row block (short rows) / long-row chunk
+ *   lanes        : fixed lanes per row for CSR row blocks (1..64, 0 = adaptive)
+ *   coo_items    : entries per lane of the COO segmented reduction (4, 8, 16)
+ *   rows_per_thread : DIA / ELL rows per thread (1 or 2) */
+typedef struct mph_tuning {
+  int32_t rows_chunk;
+  int32_t lanes;
+  int32_t coo_items;
+  int32_t rows_per_thread;
+} mph_tuning;
+int mph_matrix_set_tuning(mph_matrix_t A, const mph_tuning* t);
+int mph_matrix_get_tuning(mph_matrix_t A, mph_tuning* t);
+
+/* ------------------------------------------------------------ multiply */
+
+/* Morpheus::multiply<ExecSpace>(A, x, y, init)
+ * (core/src/Morpheus_Multiply.hpp:57-61,75-78; HIP backend
+ * core/src/impl/{Coo,Csr,Dia,Ell,Hyb,Hdc}/HIP/Morpheus_Multiply_Impl.hpp):
+ * y = A*x when init != 0, y += A*x otherwise.  Asynchronous on `stream`;
+ * allocates nothing once the handle is analysed (hipGraph-capturable). */
+int mph_multiply(mph_matrix_t A, const void* x, void* y, int init,
+                 mph_stream_t stream);
+
+/* k right-hand sides (SpMM): X is ncols x k and Y is nrows x k, row-major
+ * (element (i, j) at [i*k + j]).  No reference counterpart: the reference
+ * multiply takes one DenseVector (core/src/Morpheus_Multiply.hpp:57-78). */
+int mph_multiply_multi(mph_matrix_t A, int32_t k, const void* X, void* Y,
+                       int init, mph_stream_t stream);
+
+/* One-shot per-format multiply over raw arrays, the direct counterparts of
+ * Impl::multiply for each container
+ * (core/src/impl/Csr/HIP/Morpheus_Multiply_Impl.hpp:50-63 etc.).  They build
+ * a transient analysis (allocating) and release it; use a handle for
+ * repeated products. */
+int mph_csr_multiply(int64_t nrows, int64_t ncols, int64_t nnz,
+                     const int32_t* row_offsets, const int32_t* column_indices,
+                     const void* values, int value_type, const void* x,
+                     void* y, int init, mph_stream_t stream);
+int mph_coo_multiply(int64_t nrows, int64_t ncols, int64_t nnz,
+                     const int32_t* row_indices, const int32_t* column_indices,
+                     const void* values, int value_type, const void* x,
+                     void* y, int init, mph_stream_t stream);
+int mph_dia_multiply(int64_t nrows, int64_t ncols, int64_t nnz, int64_t ndiag,
+                     int64_t pitch, const int32_t* diagonal_offsets,
+                     const void* values, int value_type, const void* x,
+                     void* y, int init, mph_stream_t stream);
+int mph_ell_multiply(int64_t nrows, int64_t ncols, int64_t nnz,
+                     int64_t entries_per_row, int64_t width, int64_t pitch,
+                     const int32_t* column_indices, const void* values,
+                     int value_type, const void* x, void* y, int init,
+                     mph_stream_t stream);
+
+/* ------------------------------------------------------------ convert */
+/* On-GPU format conversion, bit-exact with the reference Serial converters
+ * (core/src/impl/<Fmt>/Serial/Morpheus_Convert_Impl.hpp).  The reference
+ * HIP converters throw NotImplementedException
+ * (core/src/impl/Csr/HIP/Morpheus_Convert_Impl.hpp:39-76); these succeed.
+ * Size queries synchronise `stream`; fills are asynchronous but use
+ * stream-ordered temporary allocations.  Output arrays are caller-owned and
+ * sized from the size queries. */
+
+/* core/src/impl/Csr/Serial/Morpheus_Convert_Impl.hpp:66-97 */
+int mph_convert_csr_to_coo(int64_t nrows, int64_t nnz,
+                           const int32_t* row_offsets, const int32_t* ci,
+                           const void* v, int value_type, int32_t* ri_out,
+                           int32_t* ci_out, void* v_out, mph_stream_t stream);
+/* core/src/impl/Csr/Serial/Morpheus_Convert_Impl.hpp:99-151 */
+int mph_convert_coo_to_csr(int64_t nrows, int64_t nnz, const int32_t* ri,
+                           const int32_t* ci, const void* v, int value_type,
+                           int32_t* ro_out, int32_t* ci_out, void* v_out,
+                           mph_stream_t stream);
+/* core/src/impl/Dia/Serial/Morpheus_Convert_Impl.hpp:130-154 (sizes and
+ * fill-tolerance check, core/src/impl/Morpheus_Utils.hpp:91-105) */
+int mph_convert_coo_to_dia_size(int64_t nrows, int64_t ncols, int64_t nnz,
+                                const int32_t* ri, const int32_t* ci,
+                                int64_t* ndiag, mph_stream_t stream);
+/* core/src/impl/Dia/Serial/Morpheus_Convert_Impl.hpp:120-169 */
+int mph_convert_coo_to_dia(int64_t nrows, int64_t ncols, int64_t nnz,
+                           const int32_t* ri, const int32_t* ci,
+                           const void* v, int value_type, int64_t ndiag,
+                           int64_t pitch, int32_t* offsets_out,
+                           void* values_out, mph_stream_t stream);
+/* core/src/impl/Dia/Serial/Morpheus_Convert_Impl.hpp:72-118 */
+int mph_convert_dia_to_coo(int64_t nrows, int64_t ncols, int64_t nnz,
+                           int64_t ndiag, int64_t pitch,
+                           const int32_t* offsets, const void* values,
+                           int value_type, int32_t* ri_out, int32_t* ci_out,
+                           void* v_out, mph_stream_t stream);
+/* core/src/impl/Ell/Serial/Morpheus_Convert_Impl.hpp:126-152 */
+int mph_convert_coo_to_ell_size(int64_t nrows, int64_t nnz, const int32_t* ri,
+                                int64_t* entries_per_row, mph_stream_t stream);
+/* core/src/impl/Ell/Serial/Morpheus_Convert_Impl.hpp:111-173 */
+int mph_convert_coo_to_ell(int64_t nrows, int64_t nnz, const int32_t* ri,
+                           const int32_t* ci, const void* v, int value_type,
+                           int64_t entries_per_row, int64_t width,
+                           int64_t pitch, int32_t* ci_out, void* v_out,
+                           mph_stream_t stream);
+/* core/src/impl/Ell/Serial/Morpheus_Convert_Impl.hpp:68-109 */
+int mph_convert_ell_to_coo(int64_t nrows, int64_t nnz, int64_t width,
+                           int64_t pitch, const int32_t* ci, const void* v,
+                           int value_type, int32_t* ri_out, int32_t* ci_out,
+                           void* v_out, mph_stream_t stream);
+/* core/src/impl/Hyb/Serial/Morpheus_Convert_Impl.hpp:129-162 */
+int mph_convert_coo_to_hyb_size(int64_t nrows, int64_t nnz, const int32_t* ri,
+                                int64_t* entries_per_row, int64_t* ell_nnz,
+                                int64_t* coo_nnz, mph_stream_t stream);
+/* core/src/impl/Hyb/Serial/Morpheus_Convert_Impl.hpp:115-196 */
+int mph_convert_coo_to_hyb(int64_t nrows, int64_t nnz, const int32_t* ri,
+                           const int32_t* ci, const void* v, int value_type,
+                           int64_t entries_per_row, int64_t width,
+                           int64_t pitch, int32_t* ell_ci, void* ell_v,
+                           int32_t* coo_ri, int32_t* coo_ci, void* coo_v,
+                           mph_stream_t stream);
+/* core/src/impl/Hyb/Serial/Morpheus_Convert_Impl.hpp:62-113 */
+int mph_convert_hyb_to_coo(int64_t nrows, int64_t nnz, int64_t width,
+                           int64_t pitch, const int32_t* ell_ci,
+                           const void* ell_v, int64_t coo_nnz,
+                           const int32_t* coo_ri, const int32_t* coo_ci,
+                           const void* coo_v, int value_type, int32_t* ri_out,
+                           int32_t* ci_out, void* v_out, mph_stream_t stream);
+/* core/src/impl/Hdc/Serial/Morpheus_Convert_Impl.hpp:138-163 */
+int mph_convert_coo_to_hdc_size(int64_t nrows, int64_t ncols, int64_t nnz,
+                                const int32_t* ri, const int32_t* ci,
+                                int64_t* ndiag, int64_t* dia_nnz,
+                                int64_t* csr_nnz, mph_stream_t stream);
+/* core/src/impl/Hdc/Serial/Morpheus_Convert_Impl.hpp:128-226 */
+int mph_convert_coo_to_hdc(int64_t nrows, int64_t ncols, int64_t nnz,
+                           const int32_t* ri, const int32_t* ci,
+                           const void* v, int value_type, int64_t ndiag,
+                           int64_t pitch, int32_t* dia_offsets,
+                           void* dia_values, int32_t* csr_ro, int32_t* csr_ci,
+                           void* csr_v, mph_stream_t stream);
+/* core/src/impl/Hdc/Serial/Morpheus_Convert_Impl.hpp:64-126 */
+int mph_convert_hdc_to_coo(int64_t nrows, int64_t ncols, int64_t nnz,
+                           int64_t ndiag, int64_t pitch,
+                           const int32_t* dia_offsets, const void* dia_values,
+                           int64_t csr_nnz, const int32_t* csr_ro,
+                           const int32_t* csr_ci, const void* csr_v,
+                           int value_type, int32_t* ri_out, int32_t* ci_out,
+                           void* v_out, mph_stream_t stream);
+
+/* core/src/impl/Coo/Serial/Morpheus_Sort_Impl.hpp:111-166: stable sort of
+ * the COO entries by (row, column), in place. */
+int mph_coo_sort_by_row_and_column(int64_t nnz, int32_t* ri, int32_t* ci,
+                                   void* v, int value_type,
+                                   mph_stream_t stream);
+/* core/src/impl/Coo/Serial/Morpheus_Sort_Impl.hpp:168-189 (synchronises) */
+int mph_coo_is_sorted(int64_t nnz, const int32_t* ri, const int32_t* ci,
+                      int* sorted, mph_stream_t stream);
+/* core/src/impl/Coo/Kokkos/Morpheus_NonZeros_Per_Diagonal_Impl.hpp:38-74:
+ * out has nrows+ncols-1 int32 counters; init != 0 zeroes them first. */
+int mph_coo_count_nnz_per_diagonal(int64_t nrows, int64_t ncols, int64_t nnz,
+                                   const int32_t* ri, const int32_t* ci,
+                                   int32_t* out, int init,
+                                   mph_stream_t stream);
+/* core/src/impl/Morpheus_Utils.hpp:91-105 (host-side decision, in float) */
+int mph_exceeds_tolerance(int64_t num_rows, int64_t num_entries,
+                          int64_t num_columns);
+
+/* ------------------------------------------------------------ batched */
+/* Many matrices of uneven size and structure, each with k right-hand sides,
+ * in one persistent launch over a work queue of (matrix, row block) tiles.
+ * No reference counterpart (the reference multiplies one matrix by one
+ * vector per call).  Matrices are given as handles; their formats may differ.
+ * X[i] is ncols_i x k and Y[i] nrows_i x k, row-major. */
+typedef struct mph_batch_s* mph_batch_t;
+int mph_batch_create(mph_batch_t* out, int32_t nmat, const mph_matrix_t* mats,
+                     int32_t k, const void* const* X, void* const* Y,
+                     mph_stream_t stream);
+int mph_batch_destroy(mph_batch_t b);
+/* y_i = A_i x_i (init) or y_i += A_i x_i for all i, in one launch. */
+int mph_batch_run(mph_batch_t b, int init, mph_stream_t stream);
+/* Captures mph_batch_run into a hipGraph once and replays it. */
+int mph_batch_run_graph(mph_batch_t b, int init, mph_stream_t stream);
+/* Number of (matrix, row block) tiles and persistent workgroups. */
+int mph_batch_info(mph_batch_t b, int64_t* ntiles, int32_t* nworkgroups);
+
+/* ------------------------------------------------------------ autotuner */
+/* Times each admissible launch shape of A on `stream` (A's own x/y shapes,
+ * `reps` timed runs each) and keeps the fastest in the handle.  x and y are
+ * scratch vectors of A's column/row count (k = 1) or k-wide. */
+int mph_matrix_autotune(mph_matrix_t A, int32_t k, const void* x, void* y,
+                        int32_t reps, mph_stream_t stream, float* best_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MORPHEUS_AMD_H */

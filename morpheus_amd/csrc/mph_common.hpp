@@ -1,0 +1,184 @@
+// mph_common.hpp -- shared device/host helpers for the MI355X (gfx950) SpMV
+// hot path: value-type traits, wave64 primitives, XCD-aware block mapping,
+// streaming loads and error plumbing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/morpheus_amd.h"
+
+namespace mph {
+
+// ---------------------------------------------------------------- dtypes
+// BF16 storage: the upper 16 bits of an IEEE binary32.  Arithmetic is done in
+// FP32 after widening, so only the bit pattern is needed here.
+struct bf16 {
+  uint16_t bits;
+};
+
+template <typename V>
+struct vtraits;
+
+template <>
+struct vtraits<double> {
+  using acc = double;  // accumulation type
+  using vec = double;  // x / y element type
+  static constexpr int code = MPH_F64;
+  __device__ __forceinline__ static double widen(double v) { return v; }
+};
+
+template <>
+struct vtraits<float> {
+  using acc = float;
+  using vec = float;
+  static constexpr int code = MPH_F32;
+  __device__ __forceinline__ static float widen(float v) { return v; }
+};
+
+template <>
+struct vtraits<bf16> {
+  using acc = float;
+  using vec = float;
+  static constexpr int code = MPH_BF16;
+  __device__ __forceinline__ static float widen(bf16 v) {
+    return __uint_as_float(static_cast<uint32_t>(v.bits) << 16);
+  }
+};
+
+inline size_t value_size(int vt) {
+  return vt == MPH_F64 ? 8 : vt == MPH_F32 ? 4 : 2;
+}
+inline size_t vector_size(int vt) { return vt == MPH_F64 ? 8 : 4; }
+
+// ---------------------------------------------------------------- loads
+// Matrix streams are read exactly once per SpMV: load them non-temporally so
+// they do not evict x (which is gathered and benefits from L2 / MALL reuse).
+template <typename T>
+__device__ __forceinline__ T ld_stream(const T* p) {
+  return __builtin_nontemporal_load(p);
+}
+template <>
+__device__ __forceinline__ bf16 ld_stream<bf16>(const bf16* p) {
+  bf16 r;
+  r.bits = __builtin_nontemporal_load(&p->bits);
+  return r;
+}
+
+// ---------------------------------------------------------------- wave64
+constexpr int kWave = 64;
+
+template <typename T>
+__device__ __forceinline__ T shfl_xor(T v, int m) {
+  return __shfl_xor(v, m, kWave);
+}
+template <typename T>
+__device__ __forceinline__ T shfl_up(T v, int d) {
+  return __shfl_up(v, d, kWave);
+}
+template <typename T>
+__device__ __forceinline__ T shfl_down(T v, int d) {
+  return __shfl_down(v, d, kWave);
+}
+template <typename T>
+__device__ __forceinline__ T shfl(T v, int src) {
+  return __shfl(v, src, kWave);
+}
+
+// Sum over aligned groups of L lanes (L a power of two <= 64).
+template <typename A>
+__device__ __forceinline__ A group_sum(A v, int L) {
+  for (int m = L >> 1; m > 0; m >>= 1) v += shfl_xor(v, m);
+  return v;
+}
+
+// Knuth's TwoSum: s + e == a + b exactly.
+template <typename A>
+__device__ __forceinline__ void two_sum(A a, A b, A& s, A& e) {
+  s = a + b;
+  const A bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+// Compensated (sum, error) pair; combine() is associative up to rounding of
+// the error terms, which is what cross-lane reductions need.
+template <typename A>
+struct csum {
+  A s, c;
+  __device__ __forceinline__ void add(A v) {
+    A t, e;
+    two_sum(s, v, t, e);
+    s = t;
+    c += e;
+  }
+  __device__ __forceinline__ void combine(A s2, A c2) {
+    A t, e;
+    two_sum(s, s2, t, e);
+    s = t;
+    c += e + c2;
+  }
+  __device__ __forceinline__ A value() const { return s + c; }
+};
+
+// ---------------------------------------------------------------- XCDs
+// MI355X dispatches consecutive workgroups round-robin over its 8 XCDs, each
+// with a private L2.  Remapping blockIdx so that each XCD walks one
+// contiguous range of logical blocks keeps neighbouring rows (and the x
+// entries a stencil row shares with them) inside one L2.
+constexpr int kNumXcd = 8;
+
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t nblocks) {
+  const uint32_t xcd = bid % kNumXcd;
+  const uint32_t idx = bid / kNumXcd;
+  const uint32_t q = nblocks / kNumXcd;
+  const uint32_t r = nblocks % kNumXcd;
+  return (xcd < r) ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
+}
+
+// ---------------------------------------------------------------- atomics
+__device__ __forceinline__ int atomic_add_acq_rel(int* p, int v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL,
+                                __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void atomic_store_relaxed(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string& msg);
+
+#define MPH_HIP_CHECK(expr)                                                 \
+  do {                                                                      \
+    hipError_t _e = (expr);                                                 \
+    if (_e != hipSuccess) {                                                 \
+      ::mph::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));  \
+      return (_e == hipErrorOutOfMemory) ? MPH_ERR_OUT_OF_MEMORY            \
+                                         : MPH_ERR_HIP;                     \
+    }                                                                       \
+  } while (0)
+
+#define MPH_CHECK_STATUS(expr) \
+  do {                         \
+    int _s = (expr);           \
+    if (_s != MPH_OK) return _s; \
+  } while (0)
+
+inline int invalid(const std::string& msg) {
+  set_error(msg);
+  return MPH_ERR_INVALID_INPUT;
+}
+
+// device free whose status is deliberately ignored (release paths)
+inline void dfree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+inline int64_t pad_size(int64_t len, int64_t align) {
+  return align * ((len + align - 1) / align);
+}
+
+}  // namespace mph

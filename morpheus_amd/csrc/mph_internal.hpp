@@ -1,0 +1,111 @@
+// mph_internal.hpp -- matrix handles, analysis ("plans") and the launcher
+// entry points shared between the kernel translation units and the C-ABI.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "mph_common.hpp"
+
+namespace mph {
+
+// A row block of the CSR-family kernels.  sub < 0: rows [row, next.row) are
+// reduced from an LDS-staged slice of at most 2*chunk items.  sub >= 0: the
+// sub-th chunk of `chunk` items of the single long row `row`.
+struct RowBlock {
+  int32_t row;
+  int32_t sub;
+};
+
+constexpr int kRowBlockThreads = 256;
+constexpr int kRowBlockMaxRows = 512;    // rows of one short block
+constexpr int kDefaultChunkF64 = 1024;   // LDS items of a short block (x2)
+constexpr int kDefaultChunkF32 = 2048;
+constexpr int kCompensateLen = 1024;     // FP32 rows longer than this use TwoSum
+
+// Analysis of a CSR-shaped item set (CSR itself, the CSR part of HDC, the
+// COO tail of HYB viewed through row offsets built here).
+struct RowBlockPlan {
+  int32_t chunk = 0;
+  int32_t nblocks = 0;
+  RowBlock* blocks = nullptr;    // nblocks + 1 (sentinel)
+  void* partial = nullptr;       // nblocks accumulators (long-row chunks)
+  int32_t* counters = nullptr;   // nblocks arrival counters (zeroed once)
+  int32_t* tail_ro = nullptr;    // HYB only: row offsets of the COO tail
+  int32_t nlong = 0;             // rows split into chunks
+};
+
+// Analysis of a row-sorted COO matrix for the segmented reduction.
+struct CooPlan {
+  int items = 8;          // entries per lane
+  int64_t nchunks = 0;    // wave chunks of 64*items entries
+  int32_t nblocks = 0;
+  void* first_val = nullptr;   // nchunks accumulators
+  void* last_val = nullptr;    // nchunks accumulators
+  int32_t* done = nullptr;     // arrival counter (zeroed once)
+  bool sorted = true;          // rows non-decreasing
+  bool prezero = false;        // y cleared by a fill pass instead of gap writes
+};
+
+struct mph_matrix_impl {
+  int format = MPH_CSR;
+  int vt = MPH_F64;
+  int64_t nrows = 0, ncols = 0, nnz = 0;
+  // COO / HYB-COO
+  const int32_t* coo_ri = nullptr;
+  const int32_t* coo_ci = nullptr;
+  const void* coo_v = nullptr;
+  int64_t coo_nnz = 0;
+  // CSR / HDC-CSR
+  const int32_t* csr_ro = nullptr;
+  const int32_t* csr_ci = nullptr;
+  const void* csr_v = nullptr;
+  int64_t csr_nnz = 0;
+  // DIA / HDC-DIA
+  int64_t ndiag = 0, dia_pitch = 0, dia_nnz = 0;
+  const int32_t* dia_off = nullptr;
+  const void* dia_v = nullptr;
+  // ELL / HYB-ELL
+  int64_t ell_K = 0, ell_width = 0, ell_pitch = 0, ell_nnz = 0;
+  int64_t ell_loop = 0;  // slots visited per row (K, or width if needed)
+  const int32_t* ell_ci = nullptr;
+  const void* ell_v = nullptr;
+
+  mph_tuning tuning{0, 0, 0, 0};
+  bool analysed = false;
+  RowBlockPlan rb;
+  CooPlan coo;
+  int device = 0;
+};
+
+// ---------------------------------------------------------------- launchers
+// All return an mph_status.  `x`/`y` element type follows vtraits<V>::vec.
+int rowblock_analyse(mph_matrix_impl* A, hipStream_t s);
+int rowblock_multiply(mph_matrix_impl* A, int k, const void* x, void* y,
+                      int init, hipStream_t s);
+int coo_analyse(mph_matrix_impl* A, hipStream_t s);
+int coo_multiply(mph_matrix_impl* A, int k, const void* x, void* y, int init,
+                 hipStream_t s);
+int dia_multiply(mph_matrix_impl* A, int k, const void* x, void* y, int init,
+                 hipStream_t s);
+int ell_multiply(mph_matrix_impl* A, int k, const void* x, void* y, int init,
+                 hipStream_t s);
+void release_plans(mph_matrix_impl* A);
+int rows_sorted(const int32_t* ri, int64_t nnz, int64_t nrows, hipStream_t s,
+                int* sorted);
+int ell_padding_only(const int32_t* ci, int64_t pitch, int64_t nrows, int64_t K,
+                     int64_t width, hipStream_t s, int* padding_only);
+
+// device scan helpers (scan.hip)
+// exclusive prefix sum of n int32 counts into out (n+1 entries, out[n] =
+// total); `tmp` must hold scan_tmp_elems(n) int64 values.
+int64_t scan_tmp_elems(int64_t n);
+int exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n,
+                       int64_t* tmp, hipStream_t s);
+int fill_zero(void* p, size_t bytes, hipStream_t s);
+
+int device_cu_count();
+
+}  // namespace mph
+
+struct mph_matrix_s : mph::mph_matrix_impl {};

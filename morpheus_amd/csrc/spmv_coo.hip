@@ -1,0 +1,401 @@
+// spmv_coo.hip -- COO SpMV as a wavefront-segmented reduction.
+//
+// Replaces core/src/impl/Coo/HIP/Morpheus_Multiply_Impl.hpp:51-179 (flat
+// kernel core/src/impl/Coo/Kernels/Morpheus_Multiply_Impl.hpp:121-208 and
+// the carry reduction core/src/impl/DenseVector/Kernels/
+// Morpheus_Segmented_Reduction_Impl.hpp:40-162), which needs a y-zeroing
+// launch, a per-call temporary allocation, a one-block carry pass and a
+// one-thread tail pass.
+//
+// Design (row-sorted COO, which the reference HIP kernel also requires):
+//  * each wavefront owns a chunk of 64*T consecutive entries; every lane
+//    loads T consecutive entries with 16-byte vector loads and reduces them
+//    serially, writing rows that start and end inside the lane directly;
+//  * one segmented inclusive scan across the 64 lanes (6 shuffle steps per
+//    64*T entries) joins the lane boundary segments;
+//  * rows complete inside the chunk are written once with '=' (init) or
+//    '+=' (accumulate); rows cut by chunk boundaries leave a partial in
+//    per-chunk scratch;
+//  * the last workgroup to finish (agent-scope counter) adds the partials of
+//    every cut row in chunk order -- deterministic, one launch;
+//  * with init, rows without entries are zero-filled by the lane that sees
+//    the gap, so no separate y pass exists (a fill pass is used instead when
+//    the analysis finds a gap longer than 256 rows);
+//  * FP32 accumulation carries TwoSum-compensated (sum, error) pairs.
+// Unsorted COO (valid for the reference Serial backend) falls back to an
+// atomic scatter, which is correct but not bitwise reproducible.
+#include "mph_internal.hpp"
+
+namespace mph {
+
+template <typename A, bool C>
+struct Sum;
+
+template <typename A>
+struct Sum<A, false> {
+  A s;
+  __device__ static Sum zero() { return Sum{0}; }
+  __device__ void add(A v) { s += v; }
+  __device__ void merge(const Sum& o) { s += o.s; }
+  __device__ A value() const { return s; }
+  __device__ Sum up(int d) const { return Sum{shfl_up(s, d)}; }
+  __device__ Sum down(int d) const { return Sum{shfl_down(s, d)}; }
+  __device__ void store(A* p, int64_t i) const { p[i] = s; }
+  __device__ static Sum load(const A* p, int64_t i) { return Sum{p[i]}; }
+};
+
+template <typename A>
+struct Sum<A, true> {
+  A s, c;
+  __device__ static Sum zero() { return Sum{0, 0}; }
+  __device__ void add(A v) {
+    A t, e;
+    two_sum(s, v, t, e);
+    s = t;
+    c += e;
+  }
+  __device__ void merge(const Sum& o) {
+    A t, e;
+    two_sum(s, o.s, t, e);
+    s = t;
+    c += e + o.c;
+  }
+  __device__ A value() const { return s + c; }
+  __device__ Sum up(int d) const { return Sum{shfl_up(s, d), shfl_up(c, d)}; }
+  __device__ Sum down(int d) const { return Sum{shfl_down(s, d), shfl_down(c, d)}; }
+  __device__ void store(A* p, int64_t i) const {
+    p[2 * i] = s;
+    p[2 * i + 1] = c;
+  }
+  __device__ static Sum load(const A* p, int64_t i) { return Sum{p[2 * i], p[2 * i + 1]}; }
+};
+
+template <typename V>
+struct CooArgs {
+  int64_t nrows, nnz, nchunks;
+  const int32_t* __restrict__ ri;
+  const int32_t* __restrict__ ci;
+  const V* __restrict__ val;
+  const typename vtraits<V>::vec* __restrict__ x;
+  typename vtraits<V>::vec* __restrict__ y;
+  int init;
+  int gapfill;
+  int vec_ok;  // row / column index arrays are 16-byte aligned
+  typename vtraits<V>::acc* __restrict__ first_val;
+  typename vtraits<V>::acc* __restrict__ last_val;
+  int32_t* __restrict__ done;
+};
+
+template <typename Vec, typename A>
+__device__ __forceinline__ void put_y(Vec* y, int64_t r, A v, int init) {
+  y[r] = init ? (Vec)v : (Vec)((A)y[r] + v);
+}
+
+__device__ __forceinline__ void zero_rows(double* y, int64_t a, int64_t b) {
+  for (int64_t r = a; r < b; r++) y[r] = 0.0;
+}
+__device__ __forceinline__ void zero_rows(float* y, int64_t a, int64_t b) {
+  for (int64_t r = a; r < b; r++) y[r] = 0.0f;
+}
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// load T consecutive int32 starting at p (16-byte aligned when full)
+template <int T>
+__device__ __forceinline__ void load_idx(const int32_t* p, int n, int32_t* out,
+                                         bool vec_ok) {
+  if (vec_ok && n == T && (T % 4) == 0) {
+#pragma unroll
+    for (int k = 0; k < T / 4; k++) {
+      const v4i v = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(p) + k);
+      out[4 * k] = v.x;
+      out[4 * k + 1] = v.y;
+      out[4 * k + 2] = v.z;
+      out[4 * k + 3] = v.w;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < T; k++) out[k] = k < n ? ld_stream(p + k) : 0;
+  }
+}
+
+template <typename V, int T>
+__device__ __forceinline__ void load_val(const V* p, int n,
+                                         typename vtraits<V>::acc* out) {
+#pragma unroll
+  for (int k = 0; k < T; k++) out[k] = k < n ? vtraits<V>::widen(ld_stream(p + k)) : 0;
+}
+
+template <typename V, int T, bool COMP>
+__device__ void coo_chunk(const CooArgs<V>& a, int64_t c, int lane) {
+  using A = typename vtraits<V>::acc;
+  using S = Sum<A, COMP>;
+  constexpr int CH = kWave * T;
+  const int64_t cs = c * CH;
+  const int64_t ce = min(a.nnz, cs + (int64_t)CH);
+  const int64_t i0 = cs + (int64_t)lane * T;
+  const int n = (int)max<int64_t>(0, min<int64_t>(T, ce - i0));
+  const bool active = n > 0;
+  // lanes past the end of the chunk are inactive; the last active lane
+  const int last_lane = (int)((ce - cs - 1) / T);
+
+  const int32_t first_row = a.ri[cs];
+  const int32_t last_row = a.ri[ce - 1];
+  const bool starts_before = cs > 0 && a.ri[cs - 1] == first_row;
+  const bool continues_after = ce < a.nnz && a.ri[ce] == last_row;
+
+  auto write_seg = [&](int32_t row, const S& v) {
+    if (row == first_row && starts_before) {
+      v.store(a.first_val, c);
+    } else if (row == last_row && continues_after) {
+      v.store(a.last_val, c);
+    } else {
+      put_y(a.y, row, v.value(), a.init);
+    }
+  };
+
+  int32_t rows[T], cols[T];
+  A vals[T];
+  load_idx<T>(a.ri + i0, n, rows, a.vec_ok);
+  load_idx<T>(a.ci + i0, n, cols, a.vec_ok);
+  load_val<V, T>(a.val + i0, n, vals);
+
+  // serial pass: head segment, complete middle segments, tail segment
+  int32_t h = rows[0], cur = rows[0];
+  S head = S::zero(), run = S::zero();
+  bool single = true;
+  const bool fill = a.init && a.gapfill;
+#pragma unroll
+  for (int k = 0; k < T; k++) {
+    if (k < n) {
+      const A p = vals[k] * (A)a.x[cols[k]];
+      if (rows[k] != cur) {
+        if (single) {
+          head = run;
+          single = false;
+        } else {
+          write_seg(cur, run);  // a middle segment: complete
+        }
+        if (fill) zero_rows(a.y, (int64_t)cur + 1, rows[k]);
+        cur = rows[k];
+        run = S::zero();
+      }
+      run.add(p);
+    }
+  }
+  const int32_t t = cur;
+
+  // segmented inclusive scan of the lane tails across the wavefront
+  const int32_t t_prev = shfl_up(t, 1);
+  const bool cont = active && lane > 0 && single && h == t_prev;
+  S acc = active ? run : S::zero();
+  int flag = cont ? 0 : 1;
+  for (int d = 1; d < kWave; d <<= 1) {
+    const S o = acc.up(d);
+    const int of = shfl_up(flag, d);
+    if (lane >= d && !flag) {
+      acc.merge(o);
+    }
+    if (lane >= d) flag |= of;
+  }
+  // carry into this lane's head from the previous lanes
+  const S prev_acc = acc.up(1);
+  const int32_t h_next = shfl_down(h, 1);
+  if (!active) return;
+  if (!single) {
+    S hv = head;
+    if (lane > 0 && h == t_prev) hv.merge(prev_acc);
+    write_seg(h, hv);
+  }
+  // the tail segment ends at this lane's boundary
+  if (lane == last_lane) {
+    write_seg(t, acc);
+    if (fill && ce == a.nnz) zero_rows(a.y, (int64_t)t + 1, a.nrows);
+  } else if (h_next != t) {
+    write_seg(t, acc);
+    if (fill) zero_rows(a.y, (int64_t)t + 1, h_next);
+  }
+  // gap before the chunk's first row
+  if (lane == 0 && fill) {
+    const int64_t prev_row = cs == 0 ? -1 : a.ri[cs - 1];
+    zero_rows(a.y, prev_row + 1, first_row);
+  }
+}
+
+template <typename V, int T, bool COMP>
+__device__ void coo_fixup(const CooArgs<V>& a) {
+  using A = typename vtraits<V>::acc;
+  using S = Sum<A, COMP>;
+  constexpr int CH = kWave * T;
+  for (int64_t c = threadIdx.x; c < a.nchunks; c += blockDim.x) {
+    const int64_t cs = c * CH, ce = min(a.nnz, cs + (int64_t)CH);
+    const int32_t fr = a.ri[cs], lr = a.ri[ce - 1];
+    const bool sb = cs > 0 && a.ri[cs - 1] == fr;
+    const bool ca = ce < a.nnz && a.ri[ce] == lr;
+    if (!ca || (sb && fr == lr)) continue;
+    S tot = S::load(a.last_val, c);
+    for (int64_t cc = c + 1; cc < a.nchunks; cc++) {
+      tot.merge(S::load(a.first_val, cc));
+      const int64_t s2 = cc * CH, e2 = min(a.nnz, s2 + (int64_t)CH);
+      const bool whole = a.ri[s2] == a.ri[e2 - 1];
+      const bool more = e2 < a.nnz && a.ri[e2] == lr;
+      if (!(whole && more)) break;
+    }
+    put_y(a.y, lr, tot.value(), a.init);
+  }
+}
+
+template <typename V, int T, bool COMP>
+__global__ __launch_bounds__(256) void coo_segmented_kernel(CooArgs<V> a) {
+  const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int64_t c = (int64_t)b * (256 / kWave) + wave;
+  if (c < a.nchunks) coo_chunk<V, T, COMP>(a, c, lane);
+  // every wave publishes its scratch writes before the arrival count
+  __threadfence();
+  __syncthreads();
+  __shared__ int s_last;
+  if (threadIdx.x == 0)
+    s_last = atomic_add_acq_rel(a.done, 1) == (int)gridDim.x - 1;
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  coo_fixup<V, T, COMP>(a);
+  if (threadIdx.x == 0) atomic_store_relaxed(a.done, 0);
+}
+
+// Unsorted COO: y (cleared beforehand when init) += a*x with atomics.
+template <typename V>
+__global__ void coo_atomic_kernel(CooArgs<V> a) {
+  using A = typename vtraits<V>::acc;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.nnz) return;
+  const A p = vtraits<V>::widen(a.val[i]) * (A)a.x[a.ci[i]];
+  atomicAdd(&a.y[a.ri[i]], p);
+}
+
+template <typename V>
+__global__ void coo_zero_kernel(typename vtraits<V>::vec* y, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = 0;
+}
+
+// max row gap (for the gap-fill decision)
+__global__ void coo_gap_kernel(const int32_t* __restrict__ ri, int64_t nnz,
+                               int64_t nrows, int32_t* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > nnz) return;
+  const int64_t prev = i == 0 ? -1 : ri[i - 1];
+  const int64_t cur = i == nnz ? nrows : ri[i];
+  const int64_t gap = cur - prev - 1;
+  if (gap > 0) atomicMax(out, (int32_t)min<int64_t>(gap, 1 << 30));
+}
+
+namespace {
+constexpr int kGapFillMax = 256;
+}
+
+int coo_analyse(mph_matrix_impl* A, hipStream_t s) {
+  CooPlan& p = A->coo;
+  p.items = A->tuning.coo_items > 0 ? A->tuning.coo_items : 8;
+  if (p.items != 4 && p.items != 8 && p.items != 16)
+    return invalid("coo_items must be 4, 8 or 16");
+  int sorted = 1;
+  MPH_CHECK_STATUS(rows_sorted(A->coo_ri, A->nnz, A->nrows, s, &sorted));
+  p.sorted = sorted;
+  p.nchunks = ceil_div(A->nnz, (int64_t)kWave * p.items);
+  p.nblocks = (int32_t)ceil_div(p.nchunks, 256 / kWave);
+  p.prezero = false;
+  if (A->nnz > 0 && sorted) {
+    int32_t* g = nullptr;
+    MPH_HIP_CHECK(hipMalloc(&g, sizeof(int32_t)));
+    int st = fill_zero(g, sizeof(int32_t), s);
+    if (st == MPH_OK) {
+      hipLaunchKernelGGL(coo_gap_kernel, dim3((unsigned)ceil_div(A->nnz + 1, 256)),
+                         dim3(256), 0, s, A->coo_ri, A->nnz, A->nrows, g);
+      int32_t h = 0;
+      MPH_HIP_CHECK(hipMemcpyAsync(&h, g, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      MPH_HIP_CHECK(hipStreamSynchronize(s));
+      p.prezero = h > kGapFillMax;
+    }
+    dfree(g);
+    if (st != MPH_OK) return st;
+  }
+  const size_t acc = A->vt == MPH_F64 ? 8 : 4;
+  const size_t pair = A->vt == MPH_F64 ? 1 : 2;  // compensated pairs in FP32
+  const size_t n = (size_t)std::max<int64_t>(p.nchunks, 1);
+  MPH_HIP_CHECK(hipMalloc(&p.first_val, acc * pair * n));
+  MPH_HIP_CHECK(hipMalloc(&p.last_val, acc * pair * n));
+  MPH_HIP_CHECK(hipMalloc(&p.done, sizeof(int32_t)));
+  MPH_CHECK_STATUS(fill_zero(p.done, sizeof(int32_t), s));
+  MPH_HIP_CHECK(hipStreamSynchronize(s));
+  return MPH_OK;
+}
+
+namespace {
+
+template <typename V>
+CooArgs<V> coo_args(mph_matrix_impl* A, const void* x, void* y, int init) {
+  CooArgs<V> a{};
+  a.nrows = A->nrows;
+  a.nnz = A->nnz;
+  a.nchunks = A->coo.nchunks;
+  a.ri = A->coo_ri;
+  a.ci = A->coo_ci;
+  a.val = static_cast<const V*>(A->coo_v);
+  a.x = static_cast<const typename vtraits<V>::vec*>(x);
+  a.y = static_cast<typename vtraits<V>::vec*>(y);
+  a.init = init;
+  a.gapfill = !A->coo.prezero;
+  a.vec_ok = (reinterpret_cast<uintptr_t>(A->coo_ri) % 16 == 0) &&
+             (reinterpret_cast<uintptr_t>(A->coo_ci) % 16 == 0);
+  a.first_val = static_cast<typename vtraits<V>::acc*>(A->coo.first_val);
+  a.last_val = static_cast<typename vtraits<V>::acc*>(A->coo.last_val);
+  a.done = A->coo.done;
+  return a;
+}
+
+template <typename V>
+int coo_run(mph_matrix_impl* A, const void* x, void* y, int init,
+            hipStream_t s) {
+  using Acc = typename vtraits<V>::acc;
+  constexpr bool comp = std::is_same<Acc, float>::value;
+  CooArgs<V> a = coo_args<V>(A, x, y, init);
+  const CooPlan& p = A->coo;
+  if (init && (p.prezero || !p.sorted || A->nnz == 0)) {
+    if (A->nrows > 0)
+      hipLaunchKernelGGL(coo_zero_kernel<V>, dim3((unsigned)ceil_div(A->nrows, 256)),
+                         dim3(256), 0, s, a.y, A->nrows);
+    MPH_HIP_CHECK(hipGetLastError());
+  }
+  if (A->nnz == 0) return MPH_OK;
+  if (!p.sorted) {
+    hipLaunchKernelGGL(coo_atomic_kernel<V>, dim3((unsigned)ceil_div(A->nnz, 256)),
+                       dim3(256), 0, s, a);
+    MPH_HIP_CHECK(hipGetLastError());
+    return MPH_OK;
+  }
+  const dim3 grid((unsigned)p.nblocks), block(256);
+  switch (p.items) {
+    case 4: hipLaunchKernelGGL((coo_segmented_kernel<V, 4, comp>), grid, block, 0, s, a); break;
+    case 16: hipLaunchKernelGGL((coo_segmented_kernel<V, 16, comp>), grid, block, 0, s, a); break;
+    default: hipLaunchKernelGGL((coo_segmented_kernel<V, 8, comp>), grid, block, 0, s, a); break;
+  }
+  MPH_HIP_CHECK(hipGetLastError());
+  return MPH_OK;
+}
+
+}  // namespace
+
+int coo_multiply(mph_matrix_impl* A, int k, const void* x, void* y, int init,
+                 hipStream_t s) {
+  if (k != 1) return MPH_ERR_NOT_IMPLEMENTED;  // SpMM on COO goes through CSR
+  switch (A->vt) {
+    case MPH_F64: return coo_run<double>(A, x, y, init, s);
+    case MPH_F32: return coo_run<float>(A, x, y, init, s);
+    case MPH_BF16: return coo_run<bf16>(A, x, y, init, s);
+  }
+  return invalid("unknown value type");
+}
+
+}  // namespace mph

@@ -1,0 +1,227 @@
+// spmv_dia_ell.hip -- DIA and ELL SpMV / SpMM.
+//
+// Replaces core/src/impl/Dia/HIP/Morpheus_Multiply_Impl.hpp:41-85 (kernel
+// core/src/impl/Dia/Kernels/Morpheus_Multiply_Impl.hpp:37-84) and
+// core/src/impl/Ell/HIP/Morpheus_Multiply_Impl.hpp:41-80 (kernel
+// core/src/impl/Ell/Kernels/Morpheus_Multiply_Impl.hpp:37-65).
+//
+// Both are one thread per row over column-major (LayoutLeft) storage, so
+// every diagonal / slot is one fully coalesced, non-temporal stream across a
+// wavefront.  DIA offsets are wave-uniform and come through scalar loads;
+// there is no LDS staging and y is written once (the reference zeroes y in
+// a separate launch and then read-modify-writes it per 256-diagonal chunk,
+// and skips the zeroing when ndiag == 0).  ELL visits entries_per_row slots,
+// not the 32-padded width, when the analysis has confirmed that the slots
+// beyond entries_per_row hold only the -1 padding.
+#include "mph_internal.hpp"
+
+namespace mph {
+
+template <typename V>
+struct DiaArgs {
+  int64_t nrows, ncols;
+  int32_t ndiag;
+  int64_t pitch;
+  const int32_t* __restrict__ off;
+  const V* __restrict__ val;
+  const typename vtraits<V>::vec* __restrict__ x;
+  typename vtraits<V>::vec* __restrict__ y;
+  int32_t ld;
+  int init;
+};
+
+template <typename V, int KR, int RPT>
+__global__ __launch_bounds__(256) void dia_kernel(DiaArgs<V> a) {
+  using Acc = typename vtraits<V>::acc;
+  using Vec = typename vtraits<V>::vec;
+  const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t row0 = ((int64_t)b * 256 * RPT) + threadIdx.x;
+  Acc t[RPT][KR];
+#pragma unroll
+  for (int u = 0; u < RPT; u++)
+#pragma unroll
+    for (int q = 0; q < KR; q++) t[u][q] = 0;
+  for (int d = 0; d < a.ndiag; d++) {
+    const int64_t o = a.off[d];
+    const V* vd = a.val + a.pitch * d;
+#pragma unroll
+    for (int u = 0; u < RPT; u++) {
+      const int64_t row = row0 + u * 256;
+      const int64_t col = row + o;
+      if (row < a.nrows && col >= 0 && col < a.ncols) {
+        const Acc v = vtraits<V>::widen(ld_stream(vd + row));
+#pragma unroll
+        for (int q = 0; q < KR; q++) t[u][q] += v * (Acc)a.x[col * a.ld + q];
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < RPT; u++) {
+    const int64_t row = row0 + u * 256;
+    if (row < a.nrows) {
+#pragma unroll
+      for (int q = 0; q < KR; q++) {
+        Vec* yp = a.y + row * a.ld + q;
+        *yp = a.init ? (Vec)t[u][q] : (Vec)((Acc)*yp + t[u][q]);
+      }
+    }
+  }
+}
+
+template <typename V>
+struct EllArgs {
+  int64_t nrows;
+  int32_t loop;  // slots visited per row
+  int64_t pitch;
+  const int32_t* __restrict__ ci;
+  const V* __restrict__ val;
+  const typename vtraits<V>::vec* __restrict__ x;
+  typename vtraits<V>::vec* __restrict__ y;
+  int32_t ld;
+  int init;
+};
+
+template <typename V, int KR, int RPT>
+__global__ __launch_bounds__(256) void ell_kernel(EllArgs<V> a) {
+  using Acc = typename vtraits<V>::acc;
+  using Vec = typename vtraits<V>::vec;
+  const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t row0 = ((int64_t)b * 256 * RPT) + threadIdx.x;
+  Acc t[RPT][KR];
+#pragma unroll
+  for (int u = 0; u < RPT; u++)
+#pragma unroll
+    for (int q = 0; q < KR; q++) t[u][q] = 0;
+  for (int n = 0; n < a.loop; n++) {
+#pragma unroll
+    for (int u = 0; u < RPT; u++) {
+      const int64_t row = row0 + u * 256;
+      if (row < a.nrows) {
+        const int64_t slot = row + a.pitch * n;
+        const int32_t c = ld_stream(a.ci + slot);
+        if (c != -1) {
+          const Acc v = vtraits<V>::widen(ld_stream(a.val + slot));
+#pragma unroll
+          for (int q = 0; q < KR; q++) t[u][q] += v * (Acc)a.x[(int64_t)c * a.ld + q];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < RPT; u++) {
+    const int64_t row = row0 + u * 256;
+    if (row < a.nrows) {
+#pragma unroll
+      for (int q = 0; q < KR; q++) {
+        Vec* yp = a.y + row * a.ld + q;
+        *yp = a.init ? (Vec)t[u][q] : (Vec)((Acc)*yp + t[u][q]);
+      }
+    }
+  }
+}
+
+namespace {
+
+template <typename V, int KR, int RPT>
+int launch_dia_k(mph_matrix_impl* A, const void* x, void* y, int ld, int init,
+                 hipStream_t s) {
+  DiaArgs<V> a{};
+  a.nrows = A->nrows;
+  a.ncols = A->ncols;
+  a.ndiag = (int32_t)A->ndiag;
+  a.pitch = A->dia_pitch;
+  a.off = A->dia_off;
+  a.val = static_cast<const V*>(A->dia_v);
+  a.x = static_cast<const typename vtraits<V>::vec*>(x);
+  a.y = static_cast<typename vtraits<V>::vec*>(y);
+  a.ld = ld;
+  a.init = init;
+  const int64_t nb = ceil_div(A->nrows, 256 * RPT);
+  if (nb == 0) return MPH_OK;
+  hipLaunchKernelGGL((dia_kernel<V, KR, RPT>), dim3((unsigned)nb), dim3(256), 0,
+                     s, a);
+  MPH_HIP_CHECK(hipGetLastError());
+  return MPH_OK;
+}
+
+template <typename V, int KR, int RPT>
+int launch_ell_k(mph_matrix_impl* A, const void* x, void* y, int ld, int init,
+                 hipStream_t s) {
+  EllArgs<V> a{};
+  a.nrows = A->nrows;
+  a.loop = (int32_t)A->ell_loop;
+  a.pitch = A->ell_pitch;
+  a.ci = A->ell_ci;
+  a.val = static_cast<const V*>(A->ell_v);
+  a.x = static_cast<const typename vtraits<V>::vec*>(x);
+  a.y = static_cast<typename vtraits<V>::vec*>(y);
+  a.ld = ld;
+  a.init = init;
+  const int64_t nb = ceil_div(A->nrows, 256 * RPT);
+  if (nb == 0) return MPH_OK;
+  hipLaunchKernelGGL((ell_kernel<V, KR, RPT>), dim3((unsigned)nb), dim3(256), 0,
+                     s, a);
+  MPH_HIP_CHECK(hipGetLastError());
+  return MPH_OK;
+}
+
+template <typename V, bool DIA, int RPT>
+int launch_fmt(mph_matrix_impl* A, int k, const void* x, void* y, int init,
+               hipStream_t s) {
+  using Vec = typename vtraits<V>::vec;
+  int done = 0;
+  while (done < k) {
+    const int rem = k - done;
+    const Vec* xs = static_cast<const Vec*>(x) + done;
+    Vec* ys = static_cast<Vec*>(y) + done;
+    int kr = rem >= 16 ? 16 : rem >= 8 ? 8 : rem >= 4 ? 4 : rem >= 2 ? 2 : 1;
+    if (k == 1) kr = 1;
+    int st;
+#define MPH_LAUNCH(KRV)                                             \
+  st = DIA ? launch_dia_k<V, KRV, RPT>(A, xs, ys, k, init, s)       \
+           : launch_ell_k<V, KRV, RPT>(A, xs, ys, k, init, s)
+    switch (kr) {
+      case 16: MPH_LAUNCH(16); break;
+      case 8: MPH_LAUNCH(8); break;
+      case 4: MPH_LAUNCH(4); break;
+      case 2: MPH_LAUNCH(2); break;
+      default: MPH_LAUNCH(1); break;
+    }
+#undef MPH_LAUNCH
+    if (st != MPH_OK) return st;
+    done += kr;
+  }
+  return MPH_OK;
+}
+
+template <bool DIA>
+int dispatch(mph_matrix_impl* A, int k, const void* x, void* y, int init,
+             hipStream_t s) {
+  const bool two = A->tuning.rows_per_thread == 2;
+  switch (A->vt) {
+    case MPH_F64:
+      return two ? launch_fmt<double, DIA, 2>(A, k, x, y, init, s)
+                 : launch_fmt<double, DIA, 1>(A, k, x, y, init, s);
+    case MPH_F32:
+      return two ? launch_fmt<float, DIA, 2>(A, k, x, y, init, s)
+                 : launch_fmt<float, DIA, 1>(A, k, x, y, init, s);
+    case MPH_BF16:
+      return two ? launch_fmt<bf16, DIA, 2>(A, k, x, y, init, s)
+                 : launch_fmt<bf16, DIA, 1>(A, k, x, y, init, s);
+  }
+  return invalid("unknown value type");
+}
+
+}  // namespace
+
+int dia_multiply(mph_matrix_impl* A, int k, const void* x, void* y, int init,
+                 hipStream_t s) {
+  return dispatch<true>(A, k, x, y, init, s);
+}
+
+int ell_multiply(mph_matrix_impl* A, int k, const void* x, void* y, int init,
+                 hipStream_t s) {
+  return dispatch<false>(A, k, x, y, init, s);
+}
+
+}  // namespace mph

@@ -1,0 +1,233 @@
+"""GPU parity of Morpheus::multiply for all six formats, through the C-ABI,
+against the CPU oracle (reference Serial restatement).
+
+FP64: every row must satisfy |y - y_ref| <= 4 * nnz_row * u * sum_j |a_ij x_j|
+(u = 2^-53), integer-valued cases must match exactly.  FP32 / BF16 storage:
+max-row relative error vs the compensated FP64 oracle <= 1e-5 / 1e-2.
+Mirrors core/tests/Test_Multiply.hpp:193-271 (init and accumulate, reference
+fixture sizes 10/100/1000) plus the benchmark workloads."""
+import numpy as np
+import pytest
+
+import oracle
+from morpheus_amd import capi
+from morpheus_amd import generators as gen
+from morpheus_amd.formats import (COO_FORMAT, CSR_FORMAT, DIA_FORMAT, ELL_FORMAT,
+                                  HDC_FORMAT, HYB_FORMAT, Coo, Csr, csr_to_coo_host)
+from tests.helpers import fp64_bound, max_rel_err, small_fixture
+
+pytestmark = pytest.mark.gpu
+
+FMT = {"COO": COO_FORMAT, "CSR": CSR_FORMAT, "DIA": DIA_FORMAT, "ELL": ELL_FORMAT,
+       "HYB": HYB_FORMAT, "HDC": HDC_FORMAT}
+ALL = list(FMT)
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def gpu_spmv(Ah, x, dtype="f64", y0=None, init=True, tuning=None, k=1):
+    torch = _torch()
+    from morpheus_amd import device
+    D = device.to_device(Ah, dtype)
+    if tuning:
+        D.set_tuning(**tuning)
+    vdt = device.vector_dtype(dtype)
+    xt = torch.from_numpy(np.ascontiguousarray(x)).to("cuda").to(vdt)
+    if y0 is None:
+        yt = torch.full((Ah.nrows,) if k == 1 else (Ah.nrows, k), float("nan"),
+                        dtype=vdt, device="cuda")
+    else:
+        yt = torch.from_numpy(np.ascontiguousarray(y0)).to("cuda").to(vdt)
+    if k == 1:
+        D.multiply(xt, yt, init)
+    else:
+        D.multiply_multi(xt, yt, init)
+    torch.cuda.synchronize()
+    out = yt.cpu().numpy().astype(np.float64)
+    D.close()
+    return out
+
+
+def check_fp64(A_csr, fmt_host, x, y0=None, init=True, tuning=None):
+    y = gpu_spmv(fmt_host, x, "f64", y0, init, tuning)
+    ref = oracle.spmv(A_csr, x, y0, init)
+    bound = fp64_bound(np.diff(A_csr.row_offsets), oracle.absrow(A_csr, x),
+                       None if init else y0)
+    bad = np.nonzero(~(np.abs(y - ref) <= bound))[0]
+    assert bad.size == 0, f"{bad.size} rows out of bound, first {bad[:5]}: {y[bad[:5]]} vs {ref[bad[:5]]}"
+    return y
+
+
+def as_format(A_csr, name):
+    return oracle.convert(A_csr, FMT[name])
+
+
+# ------------------------------------------------------------ reference fixture
+@pytest.mark.parametrize("name", ALL)
+@pytest.mark.parametrize("init", [True, False])
+def test_small_fixture(gpu, name, init):
+    fx = small_fixture()
+    x = gen.random_vector(10, 5374857)
+    y0 = np.ones(10)
+    check_fp64(fx["CSR"], fx[name], x, y0, init)
+
+
+@pytest.mark.parametrize("n", [10, 100, 1000])
+@pytest.mark.parametrize("name", ALL)
+@pytest.mark.parametrize("init", [True, False])
+def test_reference_sizes(gpu, n, name, init):
+    A = gen.diagonal_test_matrix(n, seed=n)
+    x = gen.random_vector(n, 5374857)
+    y0 = np.linspace(-1, 1, n)
+    check_fp64(A, as_format(A, name), x, y0, init)
+
+
+@pytest.mark.parametrize("name", ALL)
+def test_integer_values_exact(gpu, name):
+    A = gen.diagonal_test_matrix(300, seed=3)
+    A = Csr(A.nrows, A.ncols, A.row_offsets, A.column_indices,
+            np.round(A.values * 4.0))
+    x = np.arange(A.ncols, dtype=np.float64) % 7 - 3
+    y = gpu_spmv(as_format(A, name), x)
+    assert np.array_equal(y, oracle.spmv(A, x))
+
+
+# ------------------------------------------------------------ closed forms
+@pytest.mark.parametrize("name", ALL)
+def test_stencil7_closed_form(gpu, name):
+    g = 40
+    A = gen.stencil3d_7(g)
+    c = np.indices((g, g, g)).reshape(3, -1)
+    expect = sum(((c[k] == 0).astype(int) + (c[k] == g - 1)) for k in range(3)).astype(float)
+    y = gpu_spmv(as_format(A, name), np.ones(g ** 3))
+    assert np.array_equal(y, expect)
+
+
+# ------------------------------------------------------------ long rows
+@pytest.fixture(scope="module")
+def powerlaw():
+    return gen.powerlaw_rows(30000, 30000, 1.5, 16.0, 12000, seed=77)
+
+
+@pytest.mark.parametrize("name", ["CSR", "COO", "HYB", "HDC"])
+def test_powerlaw_long_rows(gpu, powerlaw, name):
+    A = powerlaw
+    x = gen.random_vector(A.ncols, 1)
+    check_fp64(A, as_format(A, name), x)
+
+
+@pytest.mark.parametrize("name", ["CSR", "COO", "HYB"])
+def test_deterministic_bitwise(gpu, powerlaw, name):
+    Ah = as_format(powerlaw, name)
+    x = gen.random_vector(powerlaw.ncols, 2)
+    a = gpu_spmv(Ah, x)
+    b = gpu_spmv(Ah, x)
+    assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+@pytest.mark.parametrize("tuning", [dict(lanes=1), dict(lanes=8), dict(lanes=64),
+                                    dict(rows_chunk=256), dict(rows_chunk=2048),
+                                    dict(coo_items=4), dict(coo_items=16),
+                                    dict(rows_per_thread=2)])
+@pytest.mark.parametrize("name", ["CSR", "COO", "HYB", "HDC", "DIA", "ELL"])
+def test_tuning_variants(gpu, name, tuning):
+    A = gen.powerlaw_rows(5000, 5000, 1.5, 10.0, 3000, seed=5) if name in ("CSR", "COO", "HYB", "HDC") \
+        else gen.stencil3d_7(16)
+    x = gen.random_vector(A.ncols, 3)
+    check_fp64(A, as_format(A, name), x, tuning=tuning)
+
+
+# ------------------------------------------------------------ edge cases
+def _with_empty_rows(n=5000, seed=4):
+    rng = np.random.default_rng(seed)
+    rows = np.sort(rng.choice(np.arange(300, n - 300), size=400, replace=False))
+    rows = np.concatenate([rows, np.full(50, rows[10])])  # one longer row
+    cols = rng.integers(0, n, rows.size)
+    key = np.unique(rows.astype(np.int64) * n + cols)
+    rows, cols = key // n, key % n
+    counts = np.bincount(rows, minlength=n)
+    ro = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(counts, out=ro[1:])
+    return Csr(n, n, ro, cols, rng.standard_normal(rows.size))
+
+
+@pytest.mark.parametrize("name", ALL)
+@pytest.mark.parametrize("init", [True, False])
+def test_empty_rows_and_long_gaps(gpu, name, init):
+    A = _with_empty_rows()
+    x = gen.random_vector(A.ncols, 9)
+    y0 = np.full(A.nrows, 0.5)
+    try:
+        Ah = as_format(A, name)
+    except oracle.FormatConversion:
+        pytest.skip("inadmissible for this format")
+    check_fp64(A, Ah, x, y0, init)
+
+
+@pytest.mark.parametrize("name", ALL)
+def test_no_entries(gpu, name):
+    A = Csr(64, 64, np.zeros(65), [], [])
+    y = gpu_spmv(oracle.convert(A, FMT[name]) if name != "CSR" else A, np.ones(64))
+    assert np.array_equal(y, np.zeros(64))
+
+
+def test_unsorted_coo_falls_back(gpu, powerlaw):
+    coo = csr_to_coo_host(powerlaw)
+    perm = np.random.default_rng(0).permutation(coo.nnz)
+    shuffled = Coo(coo.nrows, coo.ncols, coo.row_indices[perm], coo.column_indices[perm],
+                   coo.values[perm])
+    x = gen.random_vector(coo.ncols, 5)
+    y = gpu_spmv(shuffled, x)
+    ref = oracle.spmv(powerlaw, x)
+    # atomics: order-dependent rounding, still inside the bound
+    bound = fp64_bound(np.diff(powerlaw.row_offsets), oracle.absrow(powerlaw, x))
+    assert np.all(np.abs(y - ref) <= bound)
+
+
+# ------------------------------------------------------------ reduced precision
+@pytest.mark.parametrize("dtype,tol", [("f32", 1e-5), ("bf16", 1e-2)])
+@pytest.mark.parametrize("name", ALL)
+def test_reduced_precision_error(gpu, powerlaw, dtype, tol, name):
+    A = powerlaw if name not in ("DIA", "ELL") else gen.banded(20000, 4, 0.0, 3)
+    x = gen.random_vector(A.ncols, 11)
+    y = gpu_spmv(as_format(A, name), x, dtype)
+    ref = oracle.spmv_compensated(A, x)
+    err = max_rel_err(y, ref, oracle.absrow(A, x))
+    assert err <= tol, err
+
+
+# ------------------------------------------------------------ SpMM
+@pytest.mark.parametrize("name", ["CSR", "DIA", "ELL", "HYB", "HDC"])
+@pytest.mark.parametrize("k", [2, 8, 11])
+def test_spmm_matches_columnwise(gpu, name, k):
+    A = gen.powerlaw_rows(4000, 4000, 1.5, 8.0, 2500, seed=8) if name not in ("DIA", "ELL") \
+        else gen.poisson2d(60)
+    Ah = as_format(A, name)
+    X = np.random.default_rng(k).uniform(-1, 1, (A.ncols, k))
+    Y = gpu_spmv(Ah, X, k=k)
+    absrow = np.stack([oracle.absrow(A, X[:, j]) for j in range(k)], 1)
+    ref = np.stack([oracle.spmv(A, X[:, j]) for j in range(k)], 1)
+    bound = fp64_bound(np.diff(A.row_offsets)[:, None], absrow)
+    assert np.all(np.abs(Y - ref) <= bound)
+
+
+def test_one_shot_entry_points(gpu):
+    torch = _torch()
+    import ctypes as C
+    A = gen.poisson2d(50)
+    x = gen.random_vector(A.ncols, 1)
+    ro = torch.from_numpy(A.row_offsets).cuda()
+    ci = torch.from_numpy(A.column_indices).cuda()
+    v = torch.from_numpy(A.values).cuda()
+    xt = torch.from_numpy(x).cuda()
+    yt = torch.empty(A.nrows, dtype=torch.float64, device="cuda")
+    p = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    capi.call("mph_csr_multiply", A.nrows, A.ncols, A.nnz, p(ro), p(ci), p(v), capi.F64,
+              p(xt), p(yt), 1, s)
+    torch.cuda.synchronize()
+    ref = oracle.spmv(A, x)
+    assert np.allclose(yt.cpu().numpy(), ref, rtol=0, atol=1e-13)
