@@ -82,8 +82,32 @@ struct HybFallback {
   mph_matrix_s tail;
 };
 
+// one pass over every index array before the first product
+int check_inputs(mph_matrix_t A, hipStream_t s) {
+  const int64_t n = A->nrows, m = A->ncols;
+  switch (A->format) {
+    case MPH_COO:
+      MPH_CHECK_STATUS(check_index_range(A->coo_ri, A->coo_nnz, n, false, s, "COO row_indices"));
+      return check_index_range(A->coo_ci, A->coo_nnz, m, false, s, "COO column_indices");
+    case MPH_CSR:
+    case MPH_HDC:
+      MPH_CHECK_STATUS(check_row_offsets(A->csr_ro, n, A->csr_nnz, s, "CSR row_offsets"));
+      return check_index_range(A->csr_ci, A->csr_nnz, m, false, s, "CSR column_indices");
+    case MPH_ELL:
+      return check_index_range(A->ell_ci, A->ell_width * A->ell_pitch, m, true, s,
+                               "ELL column_indices");
+    case MPH_HYB:
+      MPH_CHECK_STATUS(check_index_range(A->ell_ci, A->ell_width * A->ell_pitch, m, true, s,
+                                         "HYB ELL column_indices"));
+      MPH_CHECK_STATUS(check_index_range(A->coo_ri, A->coo_nnz, n, false, s, "HYB COO row_indices"));
+      return check_index_range(A->coo_ci, A->coo_nnz, m, false, s, "HYB COO column_indices");
+  }
+  return MPH_OK;
+}
+
 int analyse(mph_matrix_t A, hipStream_t s) {
   if (A->analysed) return MPH_OK;
+  MPH_CHECK_STATUS(check_inputs(A, s));
   int st = MPH_OK;
   switch (A->format) {
     case MPH_COO:

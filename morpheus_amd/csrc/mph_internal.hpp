@@ -106,6 +106,14 @@ int fill_zero(void* p, size_t bytes, hipStream_t s);
 
 int device_cu_count();
 
+// validate.hip
+int check_index_range(const int32_t* idx, int64_t n, int64_t hi, bool allow_m1,
+                      hipStream_t s, const char* what);
+int check_row_offsets(const int32_t* ro, int64_t nrows, int64_t nnz,
+                      hipStream_t s, const char* what);
+int validate_rowblock_plan(const int32_t* d_ro, int64_t nrows,
+                           const RowBlockPlan& p, hipStream_t s);
+
 }  // namespace mph
 
 struct mph_matrix_s : mph::mph_matrix_impl {};

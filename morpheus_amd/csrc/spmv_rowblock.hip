@@ -395,13 +395,14 @@ __global__ void rb_fill_kernel(const int32_t* __restrict__ ro, int64_t nrows,
                                RowBlock* __restrict__ blocks) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrows) return;
+  // the end sentinel is written whether or not the last row starts a block
+  if (r == nrows - 1) blocks[first[nrows]] = RowBlock{(int32_t)nrows, -1};
   const int32_t f = first[r], c = first[r + 1] - f;
   if (c == 0) return;
   bool lg;
   int64_t len;
   rb_is_start(ro, r, chunk, lg, len);
   for (int32_t q = 0; q < c; q++) blocks[f + q] = RowBlock{(int32_t)r, lg ? q : -1};
-  if (r == nrows - 1) blocks[first[nrows]] = RowBlock{(int32_t)nrows, -1};
 }
 
 // HYB: row offsets of a row-sorted COO tail.  Entry i is the first of its
@@ -540,6 +541,7 @@ int rowblock_analyse(mph_matrix_impl* A, hipStream_t s) {
                          p.chunk, first, p.blocks);
       MPH_HIP_CHECK(hipGetLastError());
       MPH_HIP_CHECK(hipStreamSynchronize(s));
+      st = validate_rowblock_plan(ro, nrows, p, s);
     }
   }
   dfree(cnt);
