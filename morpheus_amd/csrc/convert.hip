@@ -1,0 +1,1262 @@
+// convert.hip -- on-GPU format conversion, bit-exact with the reference
+// Serial converters (core/src/impl/<Fmt>/Serial/Morpheus_Convert_Impl.hpp),
+// which the reference HIP backend does not provide (it throws
+// NotImplementedException, core/src/impl/Csr/HIP/Morpheus_Convert_Impl.hpp:39-76).
+//
+// Conversion is pure indexing, so every step is a histogram, a prefix sum,
+// a compaction or a scatter:
+//  * the sequential "walks" of the Serial code become prefix sums over flag
+//    arrays (entry positions) and over run heads (position of an entry inside
+//    its run of equal row indices, the row_id/n walk of the ELL and HYB
+//    converters);
+//  * where the Serial code can write one slot twice (repeated (row, col)
+//    entries, rows split into several runs) the later entry in input order
+//    wins, reproduced with an atomicMax on the entry index per slot;
+//  * the stable (row, col) sort of core/src/impl/Coo/Serial/
+//    Morpheus_Sort_Impl.hpp:111-166 is a stable LSD radix sort (4-bit
+//    digits, per-tile histograms, ranks from wave ballots), skipped when the
+//    entries are already sorted, as in the reference;
+//  * values are moved as raw bits; the zero test of DIA -> COO (which drops
+//    +0.0 and -0.0) is a test of the bits without the sign.
+#include <algorithm>
+#include <vector>
+
+#include "mph_internal.hpp"
+
+namespace mph {
+
+namespace {
+
+constexpr int kT = 256;
+
+inline unsigned blocks_for(int64_t n, int per = kT) {
+  return (unsigned)std::max<int64_t>(1, ceil_div(n, per));
+}
+
+// ------------------------------------------------------------- raw values
+template <int B>
+struct RawT;
+template <>
+struct RawT<2> {
+  using T = uint16_t;
+  static constexpr T mag = 0x7fffu;
+};
+template <>
+struct RawT<4> {
+  using T = uint32_t;
+  static constexpr T mag = 0x7fffffffu;
+};
+template <>
+struct RawT<8> {
+  using T = uint64_t;
+  static constexpr T mag = 0x7fffffffffffffffull;
+};
+
+// ------------------------------------------------------------- small kernels
+__global__ void fill_i32_kernel(int32_t* p, int64_t n, int32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void row_hist_kernel(const int32_t* __restrict__ ri, int64_t nnz,
+                                int32_t* __restrict__ cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nnz) atomicAdd(&cnt[ri[i]], 1);
+}
+
+__global__ void max_kernel(const int32_t* __restrict__ a, int64_t n,
+                           int32_t* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) atomicMax(out, a[i]);
+}
+
+// sum over rows of min(cnt[r], K) -> out (int64 via two int32 atomics is
+// avoided: per-thread values fit, totals use unsigned long long atomics)
+__global__ void clipped_sum_kernel(const int32_t* __restrict__ cnt, int64_t n,
+                                   int32_t K, unsigned long long* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const int32_t c = cnt[i];
+    atomicAdd(out, (unsigned long long)(c < K ? c : K));
+  }
+}
+
+__global__ void unsorted_kernel(const int32_t* __restrict__ ri,
+                                const int32_t* __restrict__ ci, int64_t nnz,
+                                int32_t* __restrict__ flag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i + 1 >= nnz) return;
+  if (ri[i] > ri[i + 1] || (ri[i] == ri[i + 1] && ci[i] > ci[i + 1]))
+    atomicOr(flag, 1);
+}
+
+// first index p > 0 with ri[p] < ri[p-1] (rows decrease), else n
+__global__ void first_descent_kernel(const int32_t* __restrict__ ri, int64_t n,
+                                     int32_t* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 1 && i < n && ri[i] < ri[i - 1]) atomicMin(out, (int32_t)i);
+}
+
+// row of entry j of a CSR: the last r with ro[r] <= j (ro non-decreasing)
+__device__ __forceinline__ int32_t row_of(const int32_t* ro, int64_t nrows, int64_t j) {
+  int64_t lo = 0, hi = nrows;  // invariant: ro[lo] <= j < ro[hi]
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (ro[mid] <= j) lo = mid; else hi = mid;
+  }
+  return (int32_t)lo;
+}
+
+__global__ void expand_rows_kernel(const int32_t* __restrict__ ro, int64_t nrows,
+                                   int64_t nnz, int64_t shift,
+                                   int32_t* __restrict__ ri_out) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < nnz) ri_out[j + shift] = row_of(ro, nrows, j);
+}
+
+template <typename W>
+__global__ void gather_kernel(const W* __restrict__ src, const int32_t* __restrict__ perm,
+                              int64_t n, W* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[perm[i]];
+}
+
+// ------------------------------------------------------------- radix sort
+constexpr int kRadixBits = 4;
+constexpr int kRadix = 1 << kRadixBits;
+constexpr int kRounds = 4;
+constexpr int kTile = kT * kRounds;
+
+__global__ __launch_bounds__(kT) void radix_hist_kernel(const int32_t* __restrict__ key,
+                                                        int64_t n, int shift,
+                                                        int32_t* __restrict__ hist,
+                                                        int64_t nb) {
+  __shared__ int32_t h[kRadix];
+  if (threadIdx.x < kRadix) h[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+  for (int k = 0; k < kRounds; k++) {
+    const int64_t i = base + k * kT + threadIdx.x;
+    if (i < n) atomicAdd(&h[(key[i] >> shift) & (kRadix - 1)], 1);
+  }
+  __syncthreads();
+  if (threadIdx.x < kRadix) hist[threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter: items keep their input order inside each digit bucket.
+__global__ __launch_bounds__(kT) void radix_scatter_kernel(
+    const int32_t* __restrict__ key, const int32_t* __restrict__ other,
+    const int32_t* __restrict__ perm, int64_t n, int shift,
+    const int32_t* __restrict__ offs, int64_t nb, int32_t* __restrict__ key_out,
+    int32_t* __restrict__ other_out, int32_t* __restrict__ perm_out) {
+  __shared__ int32_t running[kRadix];
+  __shared__ int32_t wcount[kT / kWave][kRadix];
+  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  if (threadIdx.x < kRadix) running[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+  for (int k = 0; k < kRounds; k++) {
+    const int64_t i = base + k * kT + threadIdx.x;
+    const bool valid = i < n;
+    const int d = valid ? (key[i] >> shift) & (kRadix - 1) : kRadix;
+    int rank = 0;
+    for (int v = 0; v < kRadix; v++) {
+      const uint64_t m = __ballot(d == v);
+      if (d == v) rank = __popcll(m & lt);
+      if (lane == 0) wcount[wave][v] = __popcll(m);
+    }
+    __syncthreads();
+    if (valid) {
+      int32_t before = running[d];
+      for (int w = 0; w < wave; w++) before += wcount[w][d];
+      const int64_t dst = (int64_t)offs[d * nb + blockIdx.x] + before + rank;
+      key_out[dst] = key[i];
+      other_out[dst] = other[i];
+      perm_out[dst] = perm[i];
+    }
+    __syncthreads();
+    if (threadIdx.x < kRadix) {
+      int32_t t = 0;
+      for (int w = 0; w < kT / kWave; w++) t += wcount[w][threadIdx.x];
+      running[threadIdx.x] += t;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void iota_kernel(int32_t* p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = (int32_t)i;
+}
+
+int bits_for(int64_t maxv) {
+  int b = 0;
+  while (b < 31 && (int64_t(1) << b) <= maxv) b++;
+  return b;
+}
+
+// RAII bag of stream-ordered temporaries
+struct Tmp {
+  hipStream_t s;
+  std::vector<void*> ptrs;
+  explicit Tmp(hipStream_t st) : s(st) {}
+  ~Tmp() {
+    for (void* p : ptrs) (void)hipFreeAsync(p, s);
+  }
+  template <typename T>
+  int alloc(T** p, int64_t n) {
+    *p = nullptr;
+    hipError_t e = hipMallocAsync(reinterpret_cast<void**>(p),
+                                  sizeof(T) * (size_t)std::max<int64_t>(n, 1), s);
+    if (e != hipSuccess) {
+      set_error(std::string("convert temporary: ") + hipGetErrorString(e));
+      return e == hipErrorOutOfMemory ? MPH_ERR_OUT_OF_MEMORY : MPH_ERR_HIP;
+    }
+    ptrs.push_back(*p);
+    return MPH_OK;
+  }
+};
+
+int sync_read_i32(const int32_t* d, int32_t* h, hipStream_t s) {
+  MPH_HIP_CHECK(hipMemcpyAsync(h, d, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  MPH_HIP_CHECK(hipStreamSynchronize(s));
+  return MPH_OK;
+}
+
+int fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s) {
+  if (n <= 0) return MPH_OK;
+  hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for(n)), dim3(kT), 0, s, p, n, v);
+  MPH_HIP_CHECK(hipGetLastError());
+  return MPH_OK;
+}
+
+int copy_dev(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return MPH_OK;
+  MPH_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s));
+  return MPH_OK;
+}
+
+int gather_values(const void* src, const int32_t* perm, int64_t n, void* dst,
+                  int vsize, hipStream_t s) {
+  if (n <= 0) return MPH_OK;
+  const dim3 g(blocks_for(n)), b(kT);
+  if (vsize == 8)
+    hipLaunchKernelGGL(gather_kernel<uint64_t>, g, b, 0, s, (const uint64_t*)src, perm, n, (uint64_t*)dst);
+  else if (vsize == 4)
+    hipLaunchKernelGGL(gather_kernel<uint32_t>, g, b, 0, s, (const uint32_t*)src, perm, n, (uint32_t*)dst);
+  else
+    hipLaunchKernelGGL(gather_kernel<uint16_t>, g, b, 0, s, (const uint16_t*)src, perm, n, (uint16_t*)dst);
+  MPH_HIP_CHECK(hipGetLastError());
+  return MPH_OK;
+}
+
+// stable sort of (row, col) pairs; by_col = false sorts by row only.
+// On return perm[i] is the input position of output entry i.
+int radix_sort_pairs(int32_t* ri, int32_t* ci, int64_t n, int64_t nrows,
+                     int64_t ncols, bool by_col, int32_t* perm, hipStream_t s) {
+  if (n <= 1) {
+    if (n == 1) hipLaunchKernelGGL(iota_kernel, dim3(1), dim3(kT), 0, s, perm, n);
+    return MPH_OK;
+  }
+  Tmp t(s);
+  int32_t *r2, *c2, *p2, *hist, *offs;
+  int64_t* tmp;
+  const int64_t nb = ceil_div(n, kTile);
+  MPH_CHECK_STATUS(t.alloc(&r2, n));
+  MPH_CHECK_STATUS(t.alloc(&c2, n));
+  MPH_CHECK_STATUS(t.alloc(&p2, n));
+  MPH_CHECK_STATUS(t.alloc(&hist, kRadix * nb));
+  MPH_CHECK_STATUS(t.alloc(&offs, kRadix * nb + 1));
+  MPH_CHECK_STATUS(t.alloc(&tmp, scan_tmp_elems(kRadix * nb)));
+  hipLaunchKernelGGL(iota_kernel, dim3(blocks_for(n)), dim3(kT), 0, s, perm, n);
+  int32_t *R = ri, *Cc = ci, *P = perm;
+  int32_t *R2 = r2, *C2 = c2, *P2 = p2;
+  auto pass = [&](bool on_col, int shift) -> int {
+    const int32_t* key = on_col ? Cc : R;
+    const int32_t* oth = on_col ? R : Cc;
+    int32_t* kout = on_col ? C2 : R2;
+    int32_t* oout = on_col ? R2 : C2;
+    hipLaunchKernelGGL(radix_hist_kernel, dim3((unsigned)nb), dim3(kT), 0, s, key, n, shift,
+                       hist, nb);
+    MPH_CHECK_STATUS(exclusive_scan_i32(hist, offs, kRadix * nb, tmp, s));
+    hipLaunchKernelGGL(radix_scatter_kernel, dim3((unsigned)nb), dim3(kT), 0, s, key, oth, P, n,
+                       shift, offs, nb, kout, oout, P2);
+    MPH_HIP_CHECK(hipGetLastError());
+    std::swap(R, R2);
+    std::swap(Cc, C2);
+    std::swap(P, P2);
+    return MPH_OK;
+  };
+  if (by_col)
+    for (int sh = 0; sh < bits_for(ncols - 1); sh += kRadixBits) MPH_CHECK_STATUS(pass(true, sh));
+  for (int sh = 0; sh < bits_for(nrows - 1); sh += kRadixBits) MPH_CHECK_STATUS(pass(false, sh));
+  if (R != ri) {
+    MPH_CHECK_STATUS(copy_dev(ri, R, sizeof(int32_t) * n, s));
+    MPH_CHECK_STATUS(copy_dev(ci, Cc, sizeof(int32_t) * n, s));
+    MPH_CHECK_STATUS(copy_dev(perm, P, sizeof(int32_t) * n, s));
+  }
+  MPH_HIP_CHECK(hipStreamSynchronize(s));  // temporaries are released on return
+  return MPH_OK;
+}
+
+int is_sorted_dev(const int32_t* ri, const int32_t* ci, int64_t nnz, hipStream_t s,
+                  int* sorted) {
+  *sorted = 1;
+  if (nnz <= 1) return MPH_OK;
+  Tmp t(s);
+  int32_t* flag;
+  MPH_CHECK_STATUS(t.alloc(&flag, 1));
+  MPH_CHECK_STATUS(fill_i32(flag, 1, 0, s));
+  hipLaunchKernelGGL(unsorted_kernel, dim3(blocks_for(nnz)), dim3(kT), 0, s, ri, ci, nnz, flag);
+  int32_t h = 0;
+  MPH_CHECK_STATUS(sync_read_i32(flag, &h, s));
+  *sorted = !h;
+  return MPH_OK;
+}
+
+// in-place stable (row, col) sort of COO arrays
+int sort_coo(int64_t nnz, int64_t nrows, int64_t ncols, int32_t* ri, int32_t* ci,
+             void* v, int vsize, hipStream_t s) {
+  if (nnz <= 1) return MPH_OK;
+  Tmp t(s);
+  int32_t* perm;
+  void* vtmp;
+  int32_t* mx;
+  MPH_CHECK_STATUS(t.alloc(&mx, 2));
+  MPH_CHECK_STATUS(fill_i32(mx, 2, 0, s));
+  hipLaunchKernelGGL(max_kernel, dim3(blocks_for(nnz)), dim3(kT), 0, s, ri, nnz, mx);
+  hipLaunchKernelGGL(max_kernel, dim3(blocks_for(nnz)), dim3(kT), 0, s, ci, nnz, mx + 1);
+  int32_t hm[2];
+  MPH_HIP_CHECK(hipMemcpyAsync(hm, mx, sizeof(hm), hipMemcpyDeviceToHost, s));
+  MPH_HIP_CHECK(hipStreamSynchronize(s));
+  // counting-sort bins run to the largest key (Sort_Impl.hpp:137-142)
+  nrows = std::min<int64_t>(nrows, (int64_t)hm[0] + 1);
+  ncols = std::min<int64_t>(ncols, (int64_t)hm[1] + 1);
+  MPH_CHECK_STATUS(t.alloc(&perm, nnz));
+  MPH_CHECK_STATUS(t.alloc(reinterpret_cast<uint8_t**>(&vtmp), nnz * vsize));
+  MPH_CHECK_STATUS(radix_sort_pairs(ri, ci, nnz, nrows, ncols, true, perm, s));
+  MPH_CHECK_STATUS(copy_dev(vtmp, v, (size_t)nnz * vsize, s));
+  MPH_CHECK_STATUS(gather_values(vtmp, perm, nnz, v, vsize, s));
+  MPH_HIP_CHECK(hipStreamSynchronize(s));
+  return MPH_OK;
+}
+
+// "if (!is_sorted) sort_by_row_and_column" of the Serial converters
+int finish_coo(int64_t nnz, int64_t nrows, int64_t ncols, int32_t* ri, int32_t* ci,
+               void* v, int vsize, hipStream_t s) {
+  int sorted = 1;
+  MPH_CHECK_STATUS(is_sorted_dev(ri, ci, nnz, s, &sorted));
+  if (!sorted) MPH_CHECK_STATUS(sort_coo(nnz, nrows, ncols, ri, ci, v, vsize, s));
+  return MPH_OK;
+}
+
+// ------------------------------------------------------------- emitters
+// DIA part: flags over (diagonal d, row r) in diagonal-major order.
+// (row r, diagonal d) lies on the stored diagonal iff its column is inside
+// [0, ncols): rows i_start + n, n < N of :97-101
+template <typename W>
+__global__ void dia_flag_kernel(int64_t nrows, int64_t ncols, int64_t ndiag, int64_t pitch,
+                                const int32_t* __restrict__ off, const void* __restrict__ val_,
+                                int32_t* __restrict__ flag) {
+  const W* val = static_cast<const W*>(val_);
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nrows * ndiag) return;
+  const int64_t d = t / nrows, r = t % nrows;
+  const int64_t c = r + off[d];
+  const bool in = c >= 0 && c < ncols;
+  flag[t] = (in && (val[r + pitch * d] & RawT<sizeof(W)>::mag) != 0) ? 1 : 0;
+}
+
+template <typename W>
+__global__ void dia_emit_kernel(int64_t nrows, int64_t ndiag, int64_t pitch,
+                                const int32_t* __restrict__ off, const void* __restrict__ val_,
+                                const int32_t* __restrict__ flag, const int32_t* __restrict__ pos,
+                                int32_t* __restrict__ ri, int32_t* __restrict__ ci,
+                                void* __restrict__ vo_) {
+  const W* val = static_cast<const W*>(val_);
+  W* vo = static_cast<W*>(vo_);
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nrows * ndiag || !flag[t]) return;
+  const int64_t d = t / nrows, r = t % nrows;
+  const int64_t p = pos[t];
+  ri[p] = (int32_t)r;
+  ci[p] = (int32_t)(r + off[d]);
+  vo[p] = val[r + pitch * d];
+}
+
+// zero the COO slots [from, nnz) left unfilled
+template <typename W>
+__global__ void zero_tail_kernel(int64_t from, int64_t nnz, int32_t* ri, int32_t* ci, void* v_) {
+  W* v = static_cast<W*>(v_);
+  const int64_t i = from + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nnz) {
+    ri[i] = 0;
+    ci[i] = 0;
+    v[i] = 0;
+  }
+}
+
+// ELL part: flags over (row r, slot n) in row-major order.
+__global__ void ell_flag_kernel(int64_t nrows, int64_t width, int64_t pitch,
+                                const int32_t* __restrict__ eci, int32_t* __restrict__ flag) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nrows * width) return;
+  const int64_t r = t / width, n = t % width;
+  flag[t] = eci[r + pitch * n] != -1 ? 1 : 0;
+}
+
+template <typename W>
+__global__ void ell_emit_kernel(int64_t nrows, int64_t width, int64_t pitch,
+                                const int32_t* __restrict__ eci, const void* __restrict__ ev_,
+                                const int32_t* __restrict__ flag, const int32_t* __restrict__ pos,
+                                int32_t* __restrict__ ri, int32_t* __restrict__ ci,
+                                void* __restrict__ vo_) {
+  const W* ev = static_cast<const W*>(ev_);
+  W* vo = static_cast<W*>(vo_);
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nrows * width || !flag[t]) return;
+  const int64_t r = t / width, n = t % width;
+  const int64_t p = pos[t];
+  ri[p] = (int32_t)r;
+  ci[p] = eci[r + pitch * n];
+  vo[p] = ev[r + pitch * n];
+}
+
+// HYB -> COO: row r's block starts at rowpos[r]; ELL slots first (compacted
+// in slot order), then the row's entries of the row-sorted COO part.
+template <typename W>
+__global__ void hyb_emit_kernel(int64_t nrows, int64_t width, int64_t pitch,
+                                const int32_t* __restrict__ eci, const void* __restrict__ ev_,
+                                const int32_t* __restrict__ tro, const int32_t* __restrict__ tci,
+                                const void* __restrict__ tv_, const int32_t* __restrict__ rowpos,
+                                int32_t* __restrict__ ri, int32_t* __restrict__ ci,
+                                void* __restrict__ vo_) {
+  const W* ev = static_cast<const W*>(ev_);
+  const W* tv = static_cast<const W*>(tv_);
+  W* vo = static_cast<W*>(vo_);
+  const int64_t r = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
+  if (r >= nrows) return;
+  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  int64_t p = rowpos[r];
+  for (int64_t n0 = 0; n0 < width; n0 += kWave) {
+    const int64_t n = n0 + lane;
+    const int32_t c = n < width ? eci[r + pitch * n] : -1;
+    const uint64_t m = __ballot(c != -1);
+    if (c != -1) {
+      const int64_t q = p + __popcll(m & lt);
+      ri[q] = (int32_t)r;
+      ci[q] = c;
+      vo[q] = ev[r + pitch * n];
+    }
+    p += __popcll(m);
+  }
+  const int64_t ts = tro[r], te = tro[r + 1];
+  for (int64_t j = ts + lane; j < te; j += kWave) {
+    const int64_t q = p + (j - ts);
+    ri[q] = (int32_t)r;
+    ci[q] = tci[j];
+    vo[q] = tv[j];
+  }
+}
+
+__global__ void hyb_rowcount_kernel(int64_t nrows, int64_t width, int64_t pitch,
+                                    const int32_t* __restrict__ eci,
+                                    const int32_t* __restrict__ tro,
+                                    int32_t* __restrict__ cnt) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  int32_t c = tro[r + 1] - tro[r];
+  for (int64_t n = 0; n < width; n++) c += eci[r + pitch * n] != -1;
+  cnt[r] = c;
+}
+
+// offsets of a row-sorted COO prefix [0, P): rows in (ri[i-1], ri[i]] start at i
+__global__ void sorted_offsets_kernel(const int32_t* __restrict__ ri, int64_t P,
+                                      int64_t nrows, int32_t* __restrict__ ro) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > P) return;
+  const int64_t prev = i == 0 ? -1 : ri[i - 1];
+  const int64_t cur = i == P ? nrows : ri[i];
+  for (int64_t r = prev + 1; r <= cur; r++) ro[r] = (int32_t)i;
+}
+
+// ------------------------------------------------------------- COO -> X
+// run heads: head[i] = 1 when entry i starts a run of equal rows
+__global__ void run_head_kernel(const int32_t* __restrict__ ri, int64_t nnz,
+                                int32_t* __restrict__ head) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nnz) head[i] = (i == 0 || ri[i] != ri[i - 1]) ? 1 : 0;
+}
+
+// runpos[k] = index of the k-th run head (hscan = exclusive scan of heads)
+__global__ void run_pos_kernel(const int32_t* __restrict__ head,
+                               const int32_t* __restrict__ hscan, int64_t nnz,
+                               int32_t* __restrict__ runpos) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nnz && head[i]) runpos[hscan[i]] = (int32_t)i;
+}
+
+__device__ __forceinline__ int64_t pos_in_run(const int32_t* head, const int32_t* hscan,
+                                              const int32_t* runpos, int64_t i) {
+  const int64_t run = hscan[i] + head[i] - 1;  // inclusive count - 1
+  return i - runpos[run];
+}
+
+// ELL / HYB placement: slot (row, n) claimed by the latest entry
+__global__ void ell_claim_kernel(const int32_t* __restrict__ ri, int64_t nnz, int64_t K,
+                                 int64_t pitch, const int32_t* __restrict__ head,
+                                 const int32_t* __restrict__ hscan,
+                                 const int32_t* __restrict__ runpos,
+                                 int32_t* __restrict__ winner, int32_t* __restrict__ over) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnz) return;
+  const int64_t n = pos_in_run(head, hscan, runpos, i);
+  if (n < K) atomicMax(&winner[ri[i] + pitch * n], (int32_t)i);
+  else if (over) over[i] = 1;
+}
+
+template <typename W>
+__global__ void ell_place_kernel(const int32_t* __restrict__ ri, const int32_t* __restrict__ ci,
+                                 const void* __restrict__ v_, int64_t nnz, int64_t K, int64_t pitch,
+                                 const int32_t* __restrict__ head,
+                                 const int32_t* __restrict__ hscan,
+                                 const int32_t* __restrict__ runpos,
+                                 const int32_t* __restrict__ winner, int32_t* __restrict__ eci,
+                                 void* __restrict__ ev_) {
+  const W* v = static_cast<const W*>(v_);
+  W* ev = static_cast<W*>(ev_);
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnz) return;
+  const int64_t n = pos_in_run(head, hscan, runpos, i);
+  if (n >= K) return;
+  const int64_t slot = ri[i] + pitch * n;
+  if (winner[slot] == (int32_t)i) {
+    eci[slot] = ci[i];
+    ev[slot] = v[i];
+  }
+}
+
+template <typename W>
+__global__ void tail_place_kernel(const int32_t* __restrict__ ri, const int32_t* __restrict__ ci,
+                                  const void* __restrict__ v_, int64_t nnz,
+                                  const int32_t* __restrict__ over,
+                                  const int32_t* __restrict__ opos, int32_t* __restrict__ tri,
+                                  int32_t* __restrict__ tci, void* __restrict__ tv_) {
+  const W* v = static_cast<const W*>(v_);
+  W* tv = static_cast<W*>(tv_);
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnz || !over[i]) return;
+  const int64_t p = opos[i];
+  tri[p] = ri[i];
+  tci[p] = ci[i];
+  tv[p] = v[i];
+}
+
+// DIA / HDC placement: the first offset equal to col - row (binary search of
+// the ascending offsets) -- the first match of the reference's linear scan
+__device__ __forceinline__ int64_t find_first(const int32_t* off, int64_t n, int32_t key) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (off[mid] < key) lo = mid + 1; else hi = mid;
+  }
+  return (lo < n && off[lo] == key) ? lo : -1;
+}
+
+__global__ void dia_claim_kernel(const int32_t* __restrict__ ri, const int32_t* __restrict__ ci,
+                                 int64_t nnz, const int32_t* __restrict__ off, int64_t ndiag,
+                                 int64_t pitch, int32_t* __restrict__ winner,
+                                 int32_t* __restrict__ rest) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnz) return;
+  const int64_t d = find_first(off, ndiag, ci[i] - ri[i]);
+  if (d >= 0) atomicMax(&winner[ri[i] + pitch * d], (int32_t)i);
+  else if (rest) rest[i] = 1;
+}
+
+template <typename W>
+__global__ void dia_place_kernel(const int32_t* __restrict__ ri, const int32_t* __restrict__ ci,
+                                 const void* __restrict__ v_, int64_t nnz,
+                                 const int32_t* __restrict__ off, int64_t ndiag, int64_t pitch,
+                                 const int32_t* __restrict__ winner, void* __restrict__ dv_) {
+  const W* v = static_cast<const W*>(v_);
+  W* dv = static_cast<W*>(dv_);
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnz) return;
+  const int64_t d = find_first(off, ndiag, ci[i] - ri[i]);
+  if (d < 0) return;
+  const int64_t slot = ri[i] + pitch * d;
+  if (winner[slot] == (int32_t)i) dv[slot] = v[i];
+}
+
+__global__ void diag_hist_kernel(const int32_t* __restrict__ ri, const int32_t* __restrict__ ci,
+                                 int64_t nnz, int64_t nrows, int32_t* __restrict__ cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nnz) atomicAdd(&cnt[(int64_t)ci[i] - ri[i] + nrows - 1], 1);
+}
+
+__global__ void nonzero_flag_kernel(const int32_t* __restrict__ cnt, int64_t n, int32_t thr,
+                                    int32_t* __restrict__ flag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flag[i] = cnt[i] >= thr ? 1 : 0;
+}
+
+__global__ void compact_offsets_kernel(const int32_t* __restrict__ flag,
+                                       const int32_t* __restrict__ pos, int64_t n,
+                                       int64_t nrows, int32_t* __restrict__ off) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && flag[i]) off[pos[i]] = (int32_t)(i - nrows + 1);
+}
+
+// HDC sizes: selected diagonals (cnt >= thr), their entries, the rest
+__global__ void hdc_sizes_kernel(const int32_t* __restrict__ cnt, int64_t n, int32_t thr,
+                                 unsigned long long* out /* ndiag, dia_nnz, csr_nnz */) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t c = cnt[i];
+  if (c >= thr) {
+    atomicAdd(&out[0], 1ull);
+    atomicAdd(&out[1], (unsigned long long)c);
+  } else if (c != 0) {
+    atomicAdd(&out[2], (unsigned long long)c);
+  }
+}
+
+// ------------------------------------------------------------- dispatch by size
+#define MPH_LAUNCH_W(kernel, vsize, grid, s, ...)                                  \
+  do {                                                                              \
+    if ((vsize) == 8)                                                               \
+      hipLaunchKernelGGL(kernel<uint64_t>, grid, dim3(kT), 0, s, __VA_ARGS__);      \
+    else if ((vsize) == 4)                                                          \
+      hipLaunchKernelGGL(kernel<uint32_t>, grid, dim3(kT), 0, s, __VA_ARGS__);      \
+    else                                                                            \
+      hipLaunchKernelGGL(kernel<uint16_t>, grid, dim3(kT), 0, s, __VA_ARGS__);      \
+  } while (0)
+
+int check_vt(int vt) {
+  if (vt != MPH_F64 && vt != MPH_F32 && vt != MPH_BF16) return invalid("unknown value type");
+  return MPH_OK;
+}
+
+int check_coo_in(int64_t nrows, int64_t ncols, int64_t nnz, const int32_t* ri,
+                 const int32_t* ci, hipStream_t s) {
+  if (nrows < 0 || ncols < 0 || nnz < 0) return invalid("negative size");
+  if (nnz > 0 && !ri) return invalid("null COO row indices");
+  MPH_CHECK_STATUS(check_index_range(ri, nnz, nrows, false, s, "COO row_indices"));
+  if (ci) MPH_CHECK_STATUS(check_index_range(ci, nnz, ncols, false, s, "COO column_indices"));
+  else if (nnz > 0 && ncols != INT32_MAX) return invalid("null COO column indices");
+  return MPH_OK;
+}
+
+// run heads / scans shared by COO -> ELL and COO -> HYB
+struct Runs {
+  int32_t *head = nullptr, *hscan = nullptr, *runpos = nullptr;
+};
+
+int build_runs(const int32_t* ri, int64_t nnz, Tmp& t, Runs& R, hipStream_t s) {
+  int64_t* tmp;
+  MPH_CHECK_STATUS(t.alloc(&R.head, nnz));
+  MPH_CHECK_STATUS(t.alloc(&R.hscan, nnz + 1));
+  MPH_CHECK_STATUS(t.alloc(&R.runpos, nnz));
+  MPH_CHECK_STATUS(t.alloc(&tmp, scan_tmp_elems(nnz)));
+  hipLaunchKernelGGL(run_head_kernel, dim3(blocks_for(nnz)), dim3(kT), 0, s, ri, nnz, R.head);
+  MPH_CHECK_STATUS(exclusive_scan_i32(R.head, R.hscan, nnz, tmp, s));
+  hipLaunchKernelGGL(run_pos_kernel, dim3(blocks_for(nnz)), dim3(kT), 0, s, R.head, R.hscan, nnz,
+                     R.runpos);
+  MPH_HIP_CHECK(hipGetLastError());
+  return MPH_OK;
+}
+
+// row histogram of COO rows (nrows counters)
+int row_counts(const int32_t* ri, int64_t nnz, int64_t nrows, Tmp& t, int32_t** cnt,
+               hipStream_t s) {
+  MPH_CHECK_STATUS(t.alloc(cnt, nrows + 1));
+  MPH_CHECK_STATUS(fill_i32(*cnt, nrows + 1, 0, s));
+  if (nnz > 0)
+    hipLaunchKernelGGL(row_hist_kernel, dim3(blocks_for(nnz)), dim3(kT), 0, s, ri, nnz, *cnt);
+  MPH_HIP_CHECK(hipGetLastError());
+  return MPH_OK;
+}
+
+// DIA-part emission shared by DIA -> COO and HDC -> COO; returns the count
+int emit_dia_part(int64_t nrows, int64_t ncols, int64_t ndiag, int64_t pitch,
+                  const int32_t* off, const void* val, int vsize, int64_t cap, int32_t* ri,
+                  int32_t* ci, void* v, int64_t* count, hipStream_t s) {
+  *count = 0;
+  const int64_t T = nrows * ndiag;
+  if (T == 0) return MPH_OK;
+  Tmp t(s);
+  int32_t *flag, *pos;
+  int64_t* tmp;
+  MPH_CHECK_STATUS(t.alloc(&flag, T));
+  MPH_CHECK_STATUS(t.alloc(&pos, T + 1));
+  MPH_CHECK_STATUS(t.alloc(&tmp, scan_tmp_elems(T)));
+  MPH_LAUNCH_W(dia_flag_kernel, vsize, dim3(blocks_for(T)), s, nrows, ncols, ndiag, pitch, off,
+               val, flag);
+  MPH_HIP_CHECK(hipGetLastError());
+  MPH_CHECK_STATUS(exclusive_scan_i32(flag, pos, T, tmp, s));
+  int32_t tot = 0;
+  MPH_CHECK_STATUS(sync_read_i32(pos + T, &tot, s));
+  if (tot > cap) return invalid("DIA holds more non-zeros than its nnz");
+  if (tot > 0) {
+    MPH_LAUNCH_W(dia_emit_kernel, vsize, dim3(blocks_for(T)), s, nrows, ndiag, pitch, off,
+                 val, flag, pos, ri, ci,
+                 v);
+    MPH_HIP_CHECK(hipGetLastError());
+  }
+  MPH_HIP_CHECK(hipStreamSynchronize(s));
+  *count = tot;
+  return MPH_OK;
+}
+
+int zero_tail(int64_t from, int64_t nnz, int32_t* ri, int32_t* ci, void* v, int vsize,
+              hipStream_t s) {
+  if (from >= nnz) return MPH_OK;
+  MPH_LAUNCH_W(zero_tail_kernel, vsize, dim3(blocks_for(nnz - from)), s, from, nnz, ri, ci,
+               v);
+  MPH_HIP_CHECK(hipGetLastError());
+  return MPH_OK;
+}
+
+}  // namespace
+
+}  // namespace mph
+
+using namespace mph;
+
+extern "C" {
+
+int mph_convert_csr_to_coo(int64_t nrows, int64_t nnz, const int32_t* ro, const int32_t* ci,
+                           const void* v, int vt, int32_t* ri_out, int32_t* ci_out,
+                           void* v_out, mph_stream_t stream) {
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  MPH_CHECK_STATUS(check_vt(vt));
+  if (nrows < 0 || nnz < 0) return invalid("negative size");
+  MPH_CHECK_STATUS(check_row_offsets(ro, nrows, nnz, s, "CSR row_offsets"));
+  if (nnz == 0) return MPH_OK;
+  hipLaunchKernelGGL(expand_rows_kernel, dim3(blocks_for(nnz)), dim3(kT), 0, s, ro, nrows, nnz,
+                     (int64_t)0, ri_out);
+  MPH_HIP_CHECK(hipGetLastError());
+  MPH_CHECK_STATUS(copy_dev(ci_out, ci, sizeof(int32_t) * nnz, s));
+  MPH_CHECK_STATUS(copy_dev(v_out, v, value_size(vt) * nnz, s));
+  return MPH_OK;
+}
+
+int mph_convert_coo_to_csr(int64_t nrows, int64_t nnz, const int32_t* ri, const int32_t* ci,
+                           const void* v, int vt, int32_t* ro_out, int32_t* ci_out,
+                           void* v_out, mph_stream_t stream) {
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  MPH_CHECK_STATUS(check_vt(vt));
+  MPH_CHECK_STATUS(check_coo_in(nrows, INT32_MAX, nnz, ri, nullptr, s));
+  const int vs = (int)value_size(vt);
+  Tmp t(s);
+  int32_t *cnt, *r2, *c2, *perm;
+  int64_t* tmp;
+  MPH_CHECK_STATUS(row_counts(ri, nnz, nrows, t, &cnt, s));
+  MPH_CHECK_STATUS(t.alloc(&tmp, scan_tmp_elems(nrows)));
+  MPH_CHECK_STATUS(exclusive_scan_i32(cnt, ro_out, nrows, tmp, s));
+  if (nnz == 0) return MPH_OK;
+  // stable scatter by row in input order == stable sort by row
+  int sorted_rows = 1;
+  MPH_CHECK_STATUS(rows_sorted(ri, nnz, nrows, s, &sorted_rows));
+  if (sorted_rows) {
+    MPH_CHECK_STATUS(copy_dev(ci_out, ci, sizeof(int32_t) * nnz, s));
+    MPH_CHECK_STATUS(copy_dev(v_out, v, (size_t)vs * nnz, s));
+    return MPH_OK;
+  }
+  MPH_CHECK_STATUS(t.alloc(&r2, nnz));
+  MPH_CHECK_STATUS(t.alloc(&c2, nnz));
+  MPH_CHECK_STATUS(t.alloc(&perm, nnz));
+  MPH_CHECK_STATUS(copy_dev(r2, ri, sizeof(int32_t) * nnz, s));
+  MPH_CHECK_STATUS(copy_dev(c2, ci, sizeof(int32_t) * nnz, s));
+  MPH_CHECK_STATUS(radix_sort_pairs(r2, c2, nnz, nrows, 0, false, perm, s));
+  MPH_CHECK_STATUS(copy_dev(ci_out, c2, sizeof(int32_t) * nnz, s));
+  MPH_CHECK_STATUS(gather_values(v, perm, nnz, v_out, vs, s));
+  MPH_HIP_CHECK(hipStreamSynchronize(s));
+  return MPH_OK;
+}
+
+int mph_convert_coo_to_dia_size(int64_t nrows, int64_t ncols, int64_t nnz, const int32_t* ri,
+                                const int32_t* ci, int64_t* ndiag, mph_stream_t stream) {
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (!ndiag) return invalid("null ndiag");
+  *ndiag = 0;
+  MPH_CHECK_STATUS(check_coo_in(nrows, ncols, nnz, ri, ci, s));
+  if (nnz == 0) return MPH_OK;
+  Tmp t(s);
+  const int64_t nb = nrows + ncols - 1;
+  int32_t *cnt, *flag, *pos;
+  int64_t* tmp;
+  MPH_CHECK_STATUS(t.alloc(&cnt, nb));
+  MPH_CHECK_STATUS(t.alloc(&flag, nb));
+  MPH_CHECK_STATUS(t.alloc(&pos, nb + 1));
+  MPH_CHECK_STATUS(t.alloc(&tmp, scan_tmp_elems(nb)));
+  MPH_CHECK_STATUS(fill_i32(cnt, nb, 0, s));
+  hipLaunchKernelGGL(diag_hist_kernel, dim3(blocks_for(nnz)), dim3(kT), 0, s, ri, ci, nnz, nrows,
+                     cnt);
+  hipLaunchKernelGGL(nonzero_flag_kernel, dim3(blocks_for(nb)), dim3(kT), 0, s, cnt, nb, 1, flag);
+  MPH_HIP_CHECK(hipGetLastError());
+  MPH_CHECK_STATUS(exclusive_scan_i32(flag, pos, nb, tmp, s));
+  int32_t nd = 0;
+  MPH_CHECK_STATUS(sync_read_i32(pos + nb, &nd, s));
+  *ndiag = nd;
+  if (mph_exceeds_tolerance(nrows, nnz, nd)) {
+    set_error("DiaMatrix fill-in would exceed maximum tolerance");
+    return MPH_ERR_FORMAT_CONVERSION;
+  }
+  return MPH_OK;
+}
+
+int mph_convert_coo_to_dia(int64_t nrows, int64_t ncols, int64_t nnz, const int32_t* ri,
+                           const int32_t* ci, const void* v, int vt, int64_t ndiag,
+                           int64_t pitch, int32_t* off_out, void* val_out,
+                           mph_stream_t stream) {
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  MPH_CHECK_STATUS(check_vt(vt));
+  if (nnz == 0) return MPH_OK;
+  int64_t nd = 0;
+  MPH_CHECK_STATUS(mph_convert_coo_to_dia_size(nrows, ncols, nnz, ri, ci, &nd, stream));
+  if (nd != ndiag) return invalid("ndiag does not match the COO entries");
+  if (pitch < nrows) return invalid("pitch must be >= nrows");
+  const int vs = (int)value_size(vt);
+  Tmp t(s);
+  const int64_t nb = nrows + ncols - 1;
+  int32_t *cnt, *flag, *pos, *win;
+  int64_t* tmp;
+  MPH_CHECK_STATUS(t.alloc(&cnt, nb));
+  MPH_CHECK_STATUS(t.alloc(&flag, nb));
+  MPH_CHECK_STATUS(t.alloc(&pos, nb + 1));
+  MPH_CHECK_STATUS(t.alloc(&tmp, scan_tmp_elems(nb)));
+  MPH_CHECK_STATUS(t.alloc(&win, pitch * ndiag));
+  MPH_CHECK_STATUS(fill_i32(cnt, nb, 0, s));
+  hipLaunchKernelGGL(diag_hist_kernel, dim3(blocks_for(nnz)), dim3(kT), 0, s, ri, ci, nnz, nrows,
+                     cnt);
+  hipLaunchKernelGGL(nonzero_flag_kernel, dim3(blocks_for(nb)), dim3(kT), 0, s, cnt, nb, 1, flag);
+  MPH_CHECK_STATUS(exclusive_scan_i32(flag, pos, nb, tmp, s));
+  hipLaunchKernelGGL(compact_offsets_kernel, dim3(blocks_for(nb)), dim3(kT), 0, s, flag, pos, nb,
+                     nrows, off_out);
+  MPH_CHECK_STATUS(fill_i32(win, pitch * ndiag, -1, s));
+  hipLaunchKernelGGL(dia_claim_kernel, dim3(blocks_for(nnz)), dim3(kT), 0, s, ri, ci, nnz,
+                     off_out, ndiag, pitch, win, (int32_t*)nullptr);
+  MPH_CHECK_STATUS(fill_zero(val_out, (size_t)vs * pitch * ndiag, s));
+  MPH_LAUNCH_W(dia_place_kernel, vs, dim3(blocks_for(nnz)), s, ri, ci,
+               v, nnz, off_out, ndiag, pitch, win,
+               val_out);
+  MPH_HIP_CHECK(hipGetLastError());
+  MPH_HIP_CHECK(hipStreamSynchronize(s));
+  return MPH_OK;
+}
+
+int mph_convert_dia_to_coo(int64_t nrows, int64_t ncols, int64_t nnz, int64_t ndiag,
+                           int64_t pitch, const int32_t* off, const void* val, int vt,
+                           int32_t* ri_out, int32_t* ci_out, void* v_out,
+                           mph_stream_t stream) {
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  MPH_CHECK_STATUS(check_vt(vt));
+  if (nnz == 0) return MPH_OK;
+  if (ndiag > 0 && pitch < nrows) return invalid("pitch must be >= nrows");
+  const int vs = (int)value_size(vt);
+  int64_t cnt = 0;
+  MPH_CHECK_STATUS(emit_dia_part(nrows, ncols, ndiag, pitch, off, val, vs, nnz, ri_out, ci_out,
+                                 v_out, &cnt, s));
+  MPH_CHECK_STATUS(zero_tail(cnt, nnz, ri_out, ci_out, v_out, vs, s));
+  return finish_coo(nnz, nrows, ncols, ri_out, ci_out, v_out, vs, s);
+}
+
+int mph_convert_coo_to_ell_size(int64_t nrows, int64_t nnz, const int32_t* ri, int64_t* K,
+                                mph_stream_t stream) {
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (!K) return invalid("null entries_per_row");
+  *K = 0;
+  MPH_CHECK_STATUS(check_coo_in(nrows, INT32_MAX, nnz, ri, nullptr, s));
+  Tmp t(s);
+  int32_t *cnt, *mx;
+  MPH_CHECK_STATUS(row_counts(ri, nnz, nrows, t, &cnt, s));
+  MPH_CHECK_STATUS(t.alloc(&mx, 1));
+  MPH_CHECK_STATUS(fill_i32(mx, 1, 0, s));
+  if (nrows > 0)
+    hipLaunchKernelGGL(max_kernel, dim3(blocks_for(nrows)), dim3(kT), 0, s, cnt, nrows, mx);
+  MPH_HIP_CHECK(hipGetLastError());
+  int32_t h = 0;
+  MPH_CHECK_STATUS(sync_read_i32(mx, &h, s));
+  *K = h;
+  if (mph_exceeds_tolerance(nrows, nnz, h)) {
+    set_error("EllMatrix fill-in would exceed maximum tolerance");
+    return MPH_ERR_FORMAT_CONVERSION;
+  }
+  return MPH_OK;
+}
+
+int mph_convert_coo_to_ell(int64_t nrows, int64_t nnz, const int32_t* ri, const int32_t* ci,
+                           const void* v, int vt, int64_t K, int64_t width, int64_t pitch,
+                           int32_t* ci_out, void* v_out, mph_stream_t stream) {
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  MPH_CHECK_STATUS(check_vt(vt));
+  int64_t k2 = 0;
+  MPH_CHECK_STATUS(mph_convert_coo_to_ell_size(nrows, nnz, ri, &k2, stream));
+  if (k2 != K) return invalid("entries_per_row does not match the COO entries");
+  if (width < K || pitch < nrows) return invalid("ELL width / pitch too small");
+  const int vs = (int)value_size(vt);
+  MPH_CHECK_STATUS(fill_i32(ci_out, width * pitch, -1, s));
+  MPH_CHECK_STATUS(fill_zero(v_out, (size_t)vs * width * pitch, s));
+  if (nnz == 0) return MPH_OK;
+  Tmp t(s);
+  Runs R;
+  int32_t* win;
+  MPH_CHECK_STATUS(build_runs(ri, nnz, t, R, s));
+  MPH_CHECK_STATUS(t.alloc(&win, width * pitch));
+  MPH_CHECK_STATUS(fill_i32(win, width * pitch, -1, s));
+  hipLaunchKernelGGL(ell_claim_kernel, dim3(blocks_for(nnz)), dim3(kT), 0, s, ri, nnz, width,
+                     pitch, R.head, R.hscan, R.runpos, win, (int32_t*)nullptr);
+  MPH_LAUNCH_W(ell_place_kernel, vs, dim3(blocks_for(nnz)), s, ri, ci,
+               v, nnz, width, pitch, R.head, R.hscan,
+               R.runpos, win, ci_out, v_out);
+  MPH_HIP_CHECK(hipGetLastError());
+  MPH_HIP_CHECK(hipStreamSynchronize(s));
+  return MPH_OK;
+}
+
+int mph_convert_ell_to_coo(int64_t nrows, int64_t nnz, int64_t width, int64_t pitch,
+                           const int32_t* eci, const void* ev, int vt, int32_t* ri_out,
+                           int32_t* ci_out, void* v_out, mph_stream_t stream) {
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  MPH_CHECK_STATUS(check_vt(vt));
+  if (nnz == 0) return MPH_OK;
+  if (width > 0 && pitch < nrows) return invalid("pitch must be >= nrows");
+  const int vs = (int)value_size(vt);
+  const int64_t T = nrows * width;
+  int32_t tot = 0;
+  int64_t ncols_bound = INT32_MAX;
+  {
+    Tmp t(s);
+    int32_t *flag, *pos;
+    int64_t* tmp;
+    MPH_CHECK_STATUS(t.alloc(&flag, T));
+    MPH_CHECK_STATUS(t.alloc(&pos, T + 1));
+    MPH_CHECK_STATUS(t.alloc(&tmp, scan_tmp_elems(T)));
+    if (T > 0)
+      hipLaunchKernelGGL(ell_flag_kernel, dim3(blocks_for(T)), dim3(kT), 0, s, nrows, width, pitch,
+                         eci, flag);
+    MPH_CHECK_STATUS(exclusive_scan_i32(flag, pos, T, tmp, s));
+    MPH_CHECK_STATUS(sync_read_i32(pos + T, &tot, s));
+    if (tot > nnz) return invalid("ELL holds more entries than its nnz");
+    if (tot > 0)
+      MPH_LAUNCH_W(ell_emit_kernel, vs, dim3(blocks_for(T)), s, nrows, width, pitch, eci,
+                   ev, flag, pos, ri_out, ci_out,
+                   v_out);
+    MPH_HIP_CHECK(hipGetLastError());
+    MPH_HIP_CHECK(hipStreamSynchronize(s));
+  }
+  MPH_CHECK_STATUS(zero_tail(tot, nnz, ri_out, ci_out, v_out, vs, s));
+  return finish_coo(nnz, nrows, ncols_bound, ri_out, ci_out, v_out, vs, s);
+}
+
+int mph_convert_coo_to_hyb_size(int64_t nrows, int64_t nnz, const int32_t* ri, int64_t* K,
+                                int64_t* ell_nnz, int64_t* coo_nnz, mph_stream_t stream) {
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (!K || !ell_nnz || !coo_nnz) return invalid("null size outputs");
+  if (nrows <= 0) return invalid("HYB needs at least one row");
+  MPH_CHECK_STATUS(check_coo_in(nrows, INT32_MAX, nnz, ri, nullptr, s));
+  const int64_t k = std::max<int64_t>(1, nnz / nrows);
+  Tmp t(s);
+  int32_t* cnt;
+  unsigned long long* acc;
+  MPH_CHECK_STATUS(row_counts(ri, nnz, nrows, t, &cnt, s));
+  MPH_CHECK_STATUS(t.alloc(&acc, 1));
+  MPH_CHECK_STATUS(fill_zero(acc, sizeof(unsigned long long), s));
+  hipLaunchKernelGGL(clipped_sum_kernel, dim3(blocks_for(nrows)), dim3(kT), 0, s, cnt, nrows,
+                     (int32_t)std::min<int64_t>(k, INT32_MAX), acc);
+  MPH_HIP_CHECK(hipGetLastError());
+  unsigned long long h = 0;
+  MPH_HIP_CHECK(hipMemcpyAsync(&h, acc, sizeof(h), hipMemcpyDeviceToHost, s));
+  MPH_HIP_CHECK(hipStreamSynchronize(s));
+  *K = k;
+  *ell_nnz = (int64_t)h;
+  *coo_nnz = nnz - (int64_t)h;
+  if (mph_exceeds_tolerance(nrows, (int64_t)h, k)) {
+    set_error("EllMatrix fill-in would exceed maximum tolerance");
+    return MPH_ERR_FORMAT_CONVERSION;
+  }
+  return MPH_OK;
+}
+
+int mph_convert_coo_to_hyb(int64_t nrows, int64_t nnz, const int32_t* ri, const int32_t* ci,
+                           const void* v, int vt, int64_t K, int64_t width, int64_t pitch,
+                           int32_t* eci, void* ev, int32_t* cri, int32_t* cci, void* cv,
+                           mph_stream_t stream) {
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  MPH_CHECK_STATUS(check_vt(vt));
+  int64_t k2, en, cn;
+  MPH_CHECK_STATUS(mph_convert_coo_to_hyb_size(nrows, nnz, ri, &k2, &en, &cn, stream));
+  if (k2 != K) return invalid("entries_per_row does not match the COO entries");
+  if (width < K || pitch < nrows) return invalid("ELL width / pitch too small");
+  const int vs = (int)value_size(vt);
+  MPH_CHECK_STATUS(fill_i32(eci, width * pitch, -1, s));
+  MPH_CHECK_STATUS(fill_zero(ev, (size_t)vs * width * pitch, s));
+  if (nnz == 0) return MPH_OK;
+  Tmp t(s);
+  Runs R;
+  int32_t *win, *over, *opos;
+  int64_t* tmp;
+  MPH_CHECK_STATUS(build_runs(ri, nnz, t, R, s));
+  MPH_CHECK_STATUS(t.alloc(&win, width * pitch));
+  MPH_CHECK_STATUS(t.alloc(&over, nnz));
+  MPH_CHECK_STATUS(t.alloc(&opos, nnz + 1));
+  MPH_CHECK_STATUS(t.alloc(&tmp, scan_tmp_elems(nnz)));
+  MPH_CHECK_STATUS(fill_i32(win, width * pitch, -1, s));
+  MPH_CHECK_STATUS(fill_i32(over, nnz, 0, s));
+  hipLaunchKernelGGL(ell_claim_kernel, dim3(blocks_for(nnz)), dim3(kT), 0, s, ri, nnz, K, pitch,
+                     R.head, R.hscan, R.runpos, win, over);
+  MPH_LAUNCH_W(ell_place_kernel, vs, dim3(blocks_for(nnz)), s, ri, ci,
+               v, nnz, K, pitch, R.head, R.hscan,
+               R.runpos, win, eci, ev);
+  MPH_CHECK_STATUS(exclusive_scan_i32(over, opos, nnz, tmp, s));
+  int32_t tail = 0;
+  MPH_CHECK_STATUS(sync_read_i32(opos + nnz, &tail, s));
+  if (tail > cn) return invalid("COO entries are not grouped by row (HYB tail overflow)");
+  if (tail < cn) MPH_CHECK_STATUS(zero_tail(tail, cn, cri, cci, cv, vs, s));
+  if (tail > 0)
+    MPH_LAUNCH_W(tail_place_kernel, vs, dim3(blocks_for(nnz)), s, ri, ci,
+                 v, nnz, over, opos, cri, cci,
+                 cv);
+  MPH_HIP_CHECK(hipGetLastError());
+  MPH_HIP_CHECK(hipStreamSynchronize(s));
+  return MPH_OK;
+}
+
+int mph_convert_hyb_to_coo(int64_t nrows, int64_t nnz, int64_t width, int64_t pitch,
+                           const int32_t* eci, const void* ev, int64_t coo_nnz,
+                           const int32_t* cri, const int32_t* cci, const void* cv, int vt,
+                           int32_t* ri_out, int32_t* ci_out, void* v_out,
+                           mph_stream_t stream) {
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  MPH_CHECK_STATUS(check_vt(vt));
+  if (nnz == 0) return MPH_OK;
+  if (width > 0 && pitch < nrows) return invalid("pitch must be >= nrows");
+  MPH_CHECK_STATUS(check_index_range(cri, coo_nnz, nrows, false, s, "HYB COO row_indices"));
+  const int vs = (int)value_size(vt);
+  int32_t total = 0;
+  {
+    Tmp t(s);
+    int32_t *P, *tro, *cnt, *rowpos;
+    int64_t* tmp;
+    // the reference's forward walk consumes the longest row-sorted prefix
+    MPH_CHECK_STATUS(t.alloc(&P, 1));
+    MPH_CHECK_STATUS(fill_i32(P, 1, (int32_t)coo_nnz, s));
+    if (coo_nnz > 1)
+      hipLaunchKernelGGL(first_descent_kernel, dim3(blocks_for(coo_nnz)), dim3(kT), 0, s, cri,
+                         coo_nnz, P);
+    int32_t hP = 0;
+    MPH_CHECK_STATUS(sync_read_i32(P, &hP, s));
+    MPH_CHECK_STATUS(t.alloc(&tro, nrows + 1));
+    hipLaunchKernelGGL(sorted_offsets_kernel, dim3(blocks_for((int64_t)hP + 1)), dim3(kT), 0, s,
+                       cri, (int64_t)hP, nrows, tro);
+    MPH_CHECK_STATUS(t.alloc(&cnt, nrows));
+    MPH_CHECK_STATUS(t.alloc(&rowpos, nrows + 1));
+    MPH_CHECK_STATUS(t.alloc(&tmp, scan_tmp_elems(nrows)));
+    hipLaunchKernelGGL(hyb_rowcount_kernel, dim3(blocks_for(nrows)), dim3(kT), 0, s, nrows, width,
+                       pitch, eci, tro, cnt);
+    MPH_CHECK_STATUS(exclusive_scan_i32(cnt, rowpos, nrows, tmp, s));
+    MPH_CHECK_STATUS(sync_read_i32(rowpos + nrows, &total, s));
+    if (total > nnz) return invalid("HYB holds more entries than its nnz");
+    const int wpb = kT / kWave;
+    MPH_LAUNCH_W(hyb_emit_kernel, vs, dim3(blocks_for(nrows, wpb)), s, nrows, width, pitch, eci,
+                 ev, tro, cci,
+                 cv, rowpos, ri_out, ci_out,
+                 v_out);
+    MPH_HIP_CHECK(hipGetLastError());
+    MPH_HIP_CHECK(hipStreamSynchronize(s));
+  }
+  MPH_CHECK_STATUS(zero_tail(total, nnz, ri_out, ci_out, v_out, vs, s));
+  return finish_coo(nnz, nrows, INT32_MAX, ri_out, ci_out, v_out, vs, s);
+}
+
+int mph_convert_coo_to_hdc_size(int64_t nrows, int64_t ncols, int64_t nnz, const int32_t* ri,
+                                const int32_t* ci, int64_t* ndiag, int64_t* dia_nnz,
+                                int64_t* csr_nnz, mph_stream_t stream) {
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (!ndiag || !dia_nnz || !csr_nnz) return invalid("null size outputs");
+  if (nrows <= 0 || ncols <= 0) return invalid("HDC needs a non-empty shape");
+  MPH_CHECK_STATUS(check_coo_in(nrows, ncols, nnz, ri, ci, s));
+  const int32_t thr = (int32_t)((double)nrows * 0.2);
+  const int64_t nb = nrows + ncols - 1;
+  Tmp t(s);
+  int32_t* cnt;
+  unsigned long long* acc;
+  MPH_CHECK_STATUS(t.alloc(&cnt, nb));
+  MPH_CHECK_STATUS(t.alloc(&acc, 3));
+  MPH_CHECK_STATUS(fill_i32(cnt, nb, 0, s));
+  MPH_CHECK_STATUS(fill_zero(acc, 3 * sizeof(unsigned long long), s));
+  if (nnz > 0)
+    hipLaunchKernelGGL(diag_hist_kernel, dim3(blocks_for(nnz)), dim3(kT), 0, s, ri, ci, nnz,
+                       nrows, cnt);
+  hipLaunchKernelGGL(hdc_sizes_kernel, dim3(blocks_for(nb)), dim3(kT), 0, s, cnt, nb, thr, acc);
+  MPH_HIP_CHECK(hipGetLastError());
+  unsigned long long h[3];
+  MPH_HIP_CHECK(hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, s));
+  MPH_HIP_CHECK(hipStreamSynchronize(s));
+  *ndiag = (int64_t)h[0];
+  *dia_nnz = (int64_t)h[1];
+  *csr_nnz = (int64_t)h[2];
+  if (mph_exceeds_tolerance(nrows, *dia_nnz, *ndiag)) {
+    set_error("DiaMatrix fill-in would exceed maximum tolerance");
+    return MPH_ERR_FORMAT_CONVERSION;
+  }
+  return MPH_OK;
+}
+
+int mph_convert_coo_to_hdc(int64_t nrows, int64_t ncols, int64_t nnz, const int32_t* ri,
+                           const int32_t* ci, const void* v, int vt, int64_t ndiag,
+                           int64_t pitch, int32_t* off_out, void* dv, int32_t* csr_ro,
+                           int32_t* csr_ci, void* csr_v, mph_stream_t stream) {
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  MPH_CHECK_STATUS(check_vt(vt));
+  int64_t nd, dn, cn;
+  MPH_CHECK_STATUS(mph_convert_coo_to_hdc_size(nrows, ncols, nnz, ri, ci, &nd, &dn, &cn, stream));
+  if (nd != ndiag) return invalid("ndiag does not match the COO entries");
+  if (ndiag > 0 && pitch < nrows) return invalid("pitch must be >= nrows");
+  const int vs = (int)value_size(vt);
+  const int32_t thr = (int32_t)((double)nrows * 0.2);
+  const int64_t nb = nrows + ncols - 1;
+  Tmp t(s);
+  int32_t *cnt, *flag, *pos, *win, *rest, *rpos, *rows_rest, *rcnt;
+  int64_t* tmp;
+  MPH_CHECK_STATUS(t.alloc(&cnt, nb));
+  MPH_CHECK_STATUS(t.alloc(&flag, nb));
+  MPH_CHECK_STATUS(t.alloc(&pos, nb + 1));
+  MPH_CHECK_STATUS(t.alloc(&tmp, scan_tmp_elems(std::max(nb, std::max(nnz, nrows)))));
+  MPH_CHECK_STATUS(fill_i32(cnt, nb, 0, s));
+  if (nnz > 0)
+    hipLaunchKernelGGL(diag_hist_kernel, dim3(blocks_for(nnz)), dim3(kT), 0, s, ri, ci, nnz,
+                       nrows, cnt);
+  hipLaunchKernelGGL(nonzero_flag_kernel, dim3(blocks_for(nb)), dim3(kT), 0, s, cnt, nb, thr,
+                     flag);
+  MPH_CHECK_STATUS(exclusive_scan_i32(flag, pos, nb, tmp, s));
+  if (ndiag > 0)
+    hipLaunchKernelGGL(compact_offsets_kernel, dim3(blocks_for(nb)), dim3(kT), 0, s, flag, pos,
+                       nb, nrows, off_out);
+  MPH_HIP_CHECK(hipGetLastError());
+  // core/src/impl/Hdc/Serial/Morpheus_Convert_Impl.hpp:169-195 records the
+  // midpoint bin as offset 0 (its true offset only for square matrices) and
+  // then sorts the offsets
+  const int64_t mid = nb / 2;
+  if (nrows != ncols && ndiag > 0) {
+    int32_t sel = 0;
+    MPH_CHECK_STATUS(sync_read_i32(flag + mid, &sel, s));
+    if (sel) {
+      std::vector<int32_t> h(ndiag);
+      MPH_HIP_CHECK(hipMemcpyAsync(h.data(), off_out, sizeof(int32_t) * ndiag,
+                                   hipMemcpyDeviceToHost, s));
+      MPH_HIP_CHECK(hipStreamSynchronize(s));
+      const int32_t dmid = (int32_t)(mid - nrows + 1);
+      for (auto& o : h)
+        if (o == dmid) { o = 0; break; }
+      std::sort(h.begin(), h.end());
+      MPH_HIP_CHECK(hipMemcpyAsync(off_out, h.data(), sizeof(int32_t) * ndiag,
+                                   hipMemcpyHostToDevice, s));
+      MPH_HIP_CHECK(hipStreamSynchronize(s));
+    }
+  }
+  // DIA part (later entry wins), the rest in input order to the CSR part
+  MPH_CHECK_STATUS(t.alloc(&win, std::max<int64_t>(pitch * ndiag, 1)));
+  MPH_CHECK_STATUS(t.alloc(&rest, nnz));
+  MPH_CHECK_STATUS(t.alloc(&rpos, nnz + 1));
+  MPH_CHECK_STATUS(fill_i32(win, pitch * ndiag, -1, s));
+  MPH_CHECK_STATUS(fill_i32(rest, nnz, 0, s));
+  MPH_CHECK_STATUS(fill_zero(dv, (size_t)vs * pitch * ndiag, s));
+  if (nnz > 0) {
+    hipLaunchKernelGGL(dia_claim_kernel, dim3(blocks_for(nnz)), dim3(kT), 0, s, ri, ci, nnz,
+                       off_out, ndiag, pitch, win, rest);
+    MPH_LAUNCH_W(dia_place_kernel, vs, dim3(blocks_for(nnz)), s, ri, ci,
+                 v, nnz, off_out, ndiag, pitch, win,
+                 dv);
+  }
+  MPH_CHECK_STATUS(exclusive_scan_i32(rest, rpos, nnz, tmp, s));
+  int32_t nrest = 0;
+  MPH_CHECK_STATUS(sync_read_i32(rpos + nnz, &nrest, s));
+  if (nrest > cn) return invalid("HDC CSR part overflow (non-square midpoint diagonal)");
+  // CSR part: per-row counts of the rest, exclusive scan, row_offsets[n] = cn
+  MPH_CHECK_STATUS(t.alloc(&rows_rest, std::max<int64_t>(cn, 1)));
+  MPH_CHECK_STATUS(t.alloc(&rcnt, nrows + 1));
+  MPH_CHECK_STATUS(fill_i32(rcnt, nrows + 1, 0, s));
+  if (nrest > 0) {
+    MPH_LAUNCH_W(tail_place_kernel, vs, dim3(blocks_for(nnz)), s, ri, ci,
+                 v, nnz, rest, rpos, rows_rest,
+                 csr_ci, csr_v);
+    hipLaunchKernelGGL(row_hist_kernel, dim3(blocks_for(nrest)), dim3(kT), 0, s, rows_rest,
+                       (int64_t)nrest, rcnt);
+  }
+  MPH_CHECK_STATUS(exclusive_scan_i32(rcnt, csr_ro, nrows, tmp, s));
+  // the reference writes row_offsets[nrows] = csr_nnz (:225)
+  hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(1), 0, s, csr_ro + nrows, (int64_t)1,
+                     (int32_t)cn);
+  if (nrest < cn) {
+    MPH_CHECK_STATUS(fill_i32(csr_ci + nrest, cn - nrest, 0, s));
+    MPH_CHECK_STATUS(fill_zero(static_cast<uint8_t*>(csr_v) + (size_t)vs * nrest,
+                               (size_t)vs * (cn - nrest), s));
+  }
+  MPH_HIP_CHECK(hipGetLastError());
+  MPH_HIP_CHECK(hipStreamSynchronize(s));
+  return MPH_OK;
+}
+
+int mph_convert_hdc_to_coo(int64_t nrows, int64_t ncols, int64_t nnz, int64_t ndiag,
+                           int64_t pitch, const int32_t* doff, const void* dval,
+                           int64_t csr_nnz, const int32_t* csr_ro, const int32_t* csr_ci,
+                           const void* csr_v, int vt, int32_t* ri_out, int32_t* ci_out,
+                           void* v_out, mph_stream_t stream) {
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  MPH_CHECK_STATUS(check_vt(vt));
+  if (nnz == 0) return MPH_OK;
+  if (ndiag > 0 && pitch < nrows) return invalid("pitch must be >= nrows");
+  MPH_CHECK_STATUS(check_row_offsets(csr_ro, nrows, csr_nnz, s, "HDC CSR row_offsets"));
+  const int vs = (int)value_size(vt);
+  int64_t nd = 0;
+  MPH_CHECK_STATUS(emit_dia_part(nrows, ncols, ndiag, pitch, doff, dval, vs, nnz, ri_out, ci_out,
+                                 v_out, &nd, s));
+  if (nd + csr_nnz > nnz) return invalid("HDC holds more entries than its nnz");
+  if (csr_nnz > 0) {
+    hipLaunchKernelGGL(expand_rows_kernel, dim3(blocks_for(csr_nnz)), dim3(kT), 0, s, csr_ro,
+                       nrows, csr_nnz, nd, ri_out);
+    MPH_HIP_CHECK(hipGetLastError());
+    MPH_CHECK_STATUS(copy_dev(ci_out + nd, csr_ci, sizeof(int32_t) * csr_nnz, s));
+    MPH_CHECK_STATUS(copy_dev(static_cast<uint8_t*>(v_out) + (size_t)vs * nd, csr_v,
+                              (size_t)vs * csr_nnz, s));
+  }
+  MPH_CHECK_STATUS(zero_tail(nd + csr_nnz, nnz, ri_out, ci_out, v_out, vs, s));
+  return finish_coo(nnz, nrows, ncols, ri_out, ci_out, v_out, vs, s);
+}
+
+int mph_coo_sort_by_row_and_column(int64_t nnz, int32_t* ri, int32_t* ci, void* v, int vt,
+                                   mph_stream_t stream) {
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  MPH_CHECK_STATUS(check_vt(vt));
+  if (nnz <= 1) return MPH_OK;
+  MPH_CHECK_STATUS(check_index_range(ri, nnz, INT32_MAX, false, s, "COO row_indices"));
+  MPH_CHECK_STATUS(check_index_range(ci, nnz, INT32_MAX, false, s, "COO column_indices"));
+  return sort_coo(nnz, INT32_MAX, INT32_MAX, ri, ci, v, (int)value_size(vt), s);
+}
+
+int mph_coo_is_sorted(int64_t nnz, const int32_t* ri, const int32_t* ci, int* sorted,
+                      mph_stream_t stream) {
+  if (!sorted) return invalid("null output");
+  return is_sorted_dev(ri, ci, nnz, reinterpret_cast<hipStream_t>(stream), sorted);
+}
+
+int mph_coo_count_nnz_per_diagonal(int64_t nrows, int64_t ncols, int64_t nnz, const int32_t* ri,
+                                   const int32_t* ci, int32_t* out, int init,
+                                   mph_stream_t stream) {
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  MPH_CHECK_STATUS(check_coo_in(nrows, ncols, nnz, ri, ci, s));
+  const int64_t nb = nrows + ncols - 1;
+  if (init) MPH_CHECK_STATUS(fill_i32(out, nb, 0, s));
+  if (nnz > 0)
+    hipLaunchKernelGGL(diag_hist_kernel, dim3(blocks_for(nnz)), dim3(kT), 0, s, ri, ci, nnz,
+                       nrows, out);
+  MPH_HIP_CHECK(hipGetLastError());
+  return MPH_OK;
+}
+
+}  // extern "C"
