@@ -1,0 +1,300 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X SpMV hot path (FP64 drop-in Morpheus::multiply).
+
+Workload (one "step"): one FP64 SpMV y = A x in each of the six formats on
+its benchmark matrix, matrices and vectors resident in HBM:
+  * CSR, COO, HYB on Z15: 1M rows, power-law(1.5) row lengths, mean 16,
+    capped at 20k, uniform random columns, N(0,1) values;
+  * DIA, HDC, ELL on S7: 3D 7-point stencil on a 128^3 grid.
+ELL on Z15 is inadmissible (the reference converter throws
+FormatConversionException; this build reproduces that), so ELL runs on S7.
+The formats are produced on the GPU by the library's own Convert from CSR.
+
+value = total FLOPs of the step (2 * nnz per SpMV) / step time.  Every SpMV
+is bracketed by HIP events on the stream it is launched on; per-format
+averages give the roofline figures (algorithmic bytes / kernel time).
+The CPU baseline is the OpenMP restatement of the reference multiply timed
+on the host cores on the same six (format, matrix) pairs.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+METRIC = "SpMV GFLOP/s & % HBM roofline per format; batched SpMM FP32/BF16 vs FP64"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=2.0,
+                    help="CPU baseline time budget per (format, matrix) pair")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a 1 GiB device copy (PMC traffic calibration)")
+    return ap.parse_args()
+
+
+def build_workload():
+    import torch
+    from morpheus_amd import capi, device
+    from morpheus_amd import generators as gen
+    from morpheus_amd.formats import (COO_FORMAT, DIA_FORMAT, ELL_FORMAT, HDC_FORMAT,
+                                      HYB_FORMAT)
+
+    t0 = time.time()
+    z15 = gen.Z15()
+    s7 = gen.S7()
+    log(f"generated Z15 (nnz {z15.nnz}) and S7 (nnz {s7.nnz}) in {time.time() - t0:.1f}s")
+    dz = device.to_device(z15, "f64")
+    ds = device.to_device(s7, "f64")
+    mats = {
+        "CSR": (dz, z15),
+        "COO": (device.convert(dz, COO_FORMAT), z15),
+        "HYB": (device.convert(dz, HYB_FORMAT), z15),
+        "DIA": (device.convert(ds, DIA_FORMAT), s7),
+        "HDC": (device.convert(ds, HDC_FORMAT), s7),
+        "ELL": (device.convert(ds, ELL_FORMAT), s7),
+    }
+    try:
+        device.convert(dz, ELL_FORMAT)
+        ell_z15 = "converted"
+    except capi.FormatConversionException:
+        ell_z15 = "FormatConversionException (as the reference)"
+    torch.cuda.synchronize()
+    # interleave matrices so consecutive products never share matrix data
+    order = ["CSR", "DIA", "COO", "ELL", "HYB", "HDC"]
+    vecs = {}
+    for name in order:
+        D, host = mats[name]
+        x = torch.empty(D.ncols, dtype=torch.float64, device="cuda").uniform_(-1.0, 1.0)
+        y = torch.empty(D.nrows, dtype=torch.float64, device="cuda")
+        D.analyse()
+        vecs[name] = (x, y)
+    torch.cuda.synchronize()
+    return mats, vecs, order, ell_z15, (z15, s7)
+
+
+def run_gpu(mats, vecs, order, steps, warmup):
+    import torch
+    stream = torch.cuda.current_stream()
+
+    def step(evs=None):
+        for name in order:
+            D, _ = mats[name]
+            x, y = vecs[name]
+            if evs is not None:
+                evs[name][0].record(stream)
+            D.multiply(x, y, True, stream)
+            if evs is not None:
+                evs[name][1].record(stream)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    per = {n: [] for n in order}
+    evlist = [{n: (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for n in order} for _ in range(steps)]
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    t_start.record(stream)
+    for k in range(steps):
+        step(evlist[k])
+    t_end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    total_ms = t_start.elapsed_time(t_end)
+    for k in range(steps):
+        for n in order:
+            per[n].append(evlist[k][n][0].elapsed_time(evlist[k][n][1]))
+    return total_ms, wall, per
+
+
+def run_cpu(mats, order, budget):
+    import numpy as np
+    import oracle  # the checker's OpenMP restatement: cpu_baseline leg only
+
+    res = {}
+    for name in order:
+        D, _ = mats[name]
+        Ah = D.to_host()
+        fn, _, _ = oracle.omp_spmv_fn(Ah)
+        for _ in range(3):
+            fn()
+        times = []
+        t_end = time.perf_counter() + budget
+        while time.perf_counter() < t_end or len(times) < 5:
+            t0 = time.perf_counter()
+            fn()
+            times.append(time.perf_counter() - t0)
+            if len(times) >= 200:
+                break
+        res[name] = (float(np.median(times)), len(times))
+        log(f"cpu {name}: median {res[name][0] * 1e3:.2f} ms over {len(times)} reps")
+    return res, oracle.omp_threads()
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the SpMV hot path runs on a single MI355X "
+              "(multi-GPU sharding is design-only); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from morpheus_amd import capi
+
+    capi.call("mph_initialize", 0)
+    mats, vecs, order, ell_z15, (z15, s7) = build_workload()
+    total_ms, wall, per = run_gpu(mats, vecs, order, args.steps, args.warmup)
+    ms_per_step = total_ms / args.steps
+    flops_step = sum(2.0 * mats[n][1].nnz for n in order)
+    value = flops_step / (ms_per_step * 1e-3) / 1e9
+
+    formats = {}
+    for n in order:
+        D, host = mats[n]
+        t = float(np.mean(per[n]))
+        bytes_alg = D.algorithmic_bytes()
+        gbs = bytes_alg / (t * 1e-3) / 1e9
+        formats[n] = {
+            "matrix": "Z15" if host is z15 else "S7",
+            "nnz": int(host.nnz),
+            "ms": round(t, 5),
+            "ms_median": round(float(np.median(per[n])), 5),
+            "gflops": round(2.0 * host.nnz / (t * 1e-3) / 1e9, 2),
+            "bytes_alg": int(bytes_alg),
+            "gbs": round(gbs, 1),
+            "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+        }
+        log(f"{n} on {formats[n]['matrix']}: {t * 1e3:.1f} us  {formats[n]['gflops']} GFLOP/s  "
+            f"{gbs:.0f} GB/s  ({100 * gbs / HBM_PEAK_GBS:.1f}% of 8 TB/s)")
+    dom = max(order, key=lambda n: formats[n]["ms"])
+    kernel_names = {
+        "CSR": ["rowblock_spmv_kernel<double, 0, false>", "rowblock_combine_kernel<double, 0, false>"],
+        "COO": ["coo_segmented_kernel<double, 8, false>", "coo_fixup_kernel<double, 8, false>"],
+        "HYB": ["rowblock_spmv_kernel<double, 2, false>", "rowblock_combine_kernel<double, 2, false>"],
+        "HDC": ["dia_kernel<double, 1, 1>"],  # S7's HDC has an empty CSR part
+        "DIA": ["dia_kernel<double, 1, 1>"],
+        "ELL": ["ell_kernel<double, 1, 1>"],
+    }
+    roof = {
+        "bound": "hbm",
+        "kernel": " + ".join(kernel_names[dom]),
+        "format": dom,
+        "matrix": formats[dom]["matrix"],
+        "achieved": formats[dom]["gbs"],
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(formats[dom]["gbs"] / HBM_PEAK_GBS, 4),
+        "traffic": None,
+    }
+    # HBM traffic per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE
+    # passes on this workload (scaled by a copy-kernel calibration), committed
+    # under profiles/
+    tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                         "spmv_f64_six_formats_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tk = json.load(f)["kernels"]
+        if all(k in tk for k in kernel_names[dom]):
+            roof["traffic"] = int(sum(tk[k]["fetch_bytes"] + tk[k]["write_bytes"]
+                                      for k in kernel_names[dom]))
+            roof["traffic_unit"] = "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE)"
+
+    # practical streaming ceiling on this device: an elementwise kernel that
+    # reads 1 GiB and writes 1 GiB (context for the roofline fraction)
+    sa = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    sb = torch.empty_like(sa)
+    for _ in range(2):
+        torch.mul(sa, 1.0, out=sb)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        torch.mul(sa, 1.0, out=sb)
+    e1.record()
+    torch.cuda.synchronize()
+    stream_gbs = 5 * 2 * sa.numel() * 8 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del sa, sb
+    log(f"elementwise 1 GiB read + 1 GiB write: {stream_gbs:.0f} GB/s")
+
+    cpu = None
+    if not args.no_cpu:
+        res, threads = run_cpu(mats, order, args.cpu_seconds)
+        t_cpu = sum(res[n][0] for n in order)
+        cpu = {
+            "value": round(flops_step / t_cpu / 1e9, 3),
+            "unit": "GFLOP/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": ("OpenMP restatement of the reference multiply (static row / nnz split) "
+                       "on the same six (format, matrix) FP64 SpMVs, median of "
+                       + ", ".join(f"{n}:{res[n][1]}" for n in order)
+                       + f" reps; {cpu_model()}"),
+        }
+
+    if args.calibrate_copy:
+        # an elementwise kernel reading 1 GiB and writing 1 GiB (a plain
+        # copy_ may be served by a DMA engine, invisible to kernel counters)
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.mul(a, 1.0, out=b)
+        torch.cuda.synchronize()
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded generators: Z15 seed 15, S7 closed-form stencil, x ~ U[-1,1])",
+        "config": {
+            "workload": ("FP64 drop-in multiply, one SpMV per format per step: CSR/COO/HYB on "
+                         "Z15 (1M rows, power-law(1.5) row lengths, mean 16, cap 20k), "
+                         "DIA/HDC/ELL on S7 (3D 7-point stencil 128^3)"),
+            "index": "int32",
+            "ell_on_z15": ell_z15,
+            "stream_copy_gbs": round(stream_gbs, 1),
+            "wall_s": round(wall, 4),
+        },
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "formats": formats,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

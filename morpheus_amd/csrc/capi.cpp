@@ -36,13 +36,11 @@ void release_plans(mph_matrix_impl* A) {
   RowBlockPlan& p = A->rb;
   dfree(p.blocks);
   if (p.partial) dfree(p.partial);
-  if (p.counters) dfree(p.counters);
   if (p.tail_ro) dfree(p.tail_ro);
   p = RowBlockPlan{};
   CooPlan& c = A->coo;
   if (c.first_val) dfree(c.first_val);
   if (c.last_val) dfree(c.last_val);
-  if (c.done) dfree(c.done);
   c = CooPlan{};
   A->analysed = false;
 }
@@ -171,9 +169,17 @@ int multiply_impl(mph_matrix_t A, int k, const void* x, void* y, int init,
     case MPH_COO:
       if (k == 1) return coo_multiply(A, 1, x, y, init, s);
       return MPH_ERR_NOT_IMPLEMENTED;
-    case MPH_CSR:
-    case MPH_HYB:
     case MPH_HDC:
+      // an HDC whose CSR part is empty is its DIA part
+      if (A->csr_nnz == 0) return dia_multiply(A, k, x, y, init, s);
+      if (A->nrows == 0) return MPH_OK;
+      return rowblock_multiply(A, k, x, y, init, s);
+    case MPH_HYB:
+      // a HYB whose COO part is empty is its ELL part
+      if (A->coo_nnz == 0) return ell_multiply(A, k, x, y, init, s);
+      if (A->nrows == 0) return MPH_OK;
+      return rowblock_multiply(A, k, x, y, init, s);
+    case MPH_CSR:
       if (A->nrows == 0) return MPH_OK;
       return rowblock_multiply(A, k, x, y, init, s);
     case MPH_DIA:

@@ -29,8 +29,7 @@ struct RowBlockPlan {
   int32_t chunk = 0;
   int32_t nblocks = 0;
   RowBlock* blocks = nullptr;    // nblocks + 1 (sentinel)
-  void* partial = nullptr;       // nblocks accumulators (long-row chunks)
-  int32_t* counters = nullptr;   // nblocks arrival counters (zeroed once)
+  void* partial = nullptr;       // per-block accumulators (long-row chunks)
   int32_t* tail_ro = nullptr;    // HYB only: row offsets of the COO tail
   int32_t nlong = 0;             // rows split into chunks
 };
@@ -42,7 +41,6 @@ struct CooPlan {
   int32_t nblocks = 0;
   void* first_val = nullptr;   // nchunks accumulators
   void* last_val = nullptr;    // nchunks accumulators
-  int32_t* done = nullptr;     // arrival counter (zeroed once)
   bool sorted = true;          // rows non-decreasing
   bool prezero = false;        // y cleared by a fill pass instead of gap writes
 };
@@ -111,8 +109,8 @@ int check_index_range(const int32_t* idx, int64_t n, int64_t hi, bool allow_m1,
                       hipStream_t s, const char* what);
 int check_row_offsets(const int32_t* ro, int64_t nrows, int64_t nnz,
                       hipStream_t s, const char* what);
-int validate_rowblock_plan(const int32_t* d_ro, int64_t nrows,
-                           const RowBlockPlan& p, hipStream_t s);
+int validate_rowblock_plan(const int32_t* d_ro, int64_t nrows, RowBlockPlan& p,
+                           hipStream_t s);
 
 }  // namespace mph
 

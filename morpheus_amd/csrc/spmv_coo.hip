@@ -16,8 +16,10 @@
 //  * rows complete inside the chunk are written once with '=' (init) or
 //    '+=' (accumulate); rows cut by chunk boundaries leave a partial in
 //    per-chunk scratch;
-//  * the last workgroup to finish (agent-scope counter) adds the partials of
-//    every cut row in chunk order -- deterministic, one launch;
+//  * a second, small launch (one thread per chunk) adds the partials of
+//    every cut row in chunk order -- deterministic, with no cross-workgroup
+//    synchronisation inside the main kernel (agent-scope fences write back
+//    and invalidate a whole XCD L2 on gfx950);
 //  * with init, rows without entries are zero-filled by the lane that sees
 //    the gap, so no separate y pass exists (a fill pass is used instead when
 //    the analysis finds a gap longer than 256 rows);
@@ -83,7 +85,6 @@ struct CooArgs {
   int vec_ok;  // row / column index arrays are 16-byte aligned
   typename vtraits<V>::acc* __restrict__ first_val;
   typename vtraits<V>::acc* __restrict__ last_val;
-  int32_t* __restrict__ done;
 };
 
 template <typename Vec, typename A>
@@ -160,6 +161,11 @@ __device__ void coo_chunk(const CooArgs<V>& a, int64_t c, int lane) {
   load_idx<T>(a.ci + i0, n, cols, a.vec_ok);
   load_val<V, T>(a.val + i0, n, vals);
 
+  // all gathers of the lane are issued before the serial pass consumes them
+  A prod[T];
+#pragma unroll
+  for (int k = 0; k < T; k++) prod[k] = k < n ? vals[k] * (A)a.x[cols[k]] : (A)0;
+
   // serial pass: head segment, complete middle segments, tail segment
   int32_t h = rows[0], cur = rows[0];
   S head = S::zero(), run = S::zero();
@@ -168,7 +174,7 @@ __device__ void coo_chunk(const CooArgs<V>& a, int64_t c, int lane) {
 #pragma unroll
   for (int k = 0; k < T; k++) {
     if (k < n) {
-      const A p = vals[k] * (A)a.x[cols[k]];
+      const A p = prod[k];
       if (rows[k] != cur) {
         if (single) {
           head = run;
@@ -222,27 +228,29 @@ __device__ void coo_chunk(const CooArgs<V>& a, int64_t c, int lane) {
   }
 }
 
+// Second launch: the chunk where a cut row starts adds the partials of the
+// chunks that continue it, in chunk order.
 template <typename V, int T, bool COMP>
-__device__ void coo_fixup(const CooArgs<V>& a) {
+__global__ void coo_fixup_kernel(CooArgs<V> a) {
   using A = typename vtraits<V>::acc;
   using S = Sum<A, COMP>;
   constexpr int CH = kWave * T;
-  for (int64_t c = threadIdx.x; c < a.nchunks; c += blockDim.x) {
-    const int64_t cs = c * CH, ce = min(a.nnz, cs + (int64_t)CH);
-    const int32_t fr = a.ri[cs], lr = a.ri[ce - 1];
-    const bool sb = cs > 0 && a.ri[cs - 1] == fr;
-    const bool ca = ce < a.nnz && a.ri[ce] == lr;
-    if (!ca || (sb && fr == lr)) continue;
-    S tot = S::load(a.last_val, c);
-    for (int64_t cc = c + 1; cc < a.nchunks; cc++) {
-      tot.merge(S::load(a.first_val, cc));
-      const int64_t s2 = cc * CH, e2 = min(a.nnz, s2 + (int64_t)CH);
-      const bool whole = a.ri[s2] == a.ri[e2 - 1];
-      const bool more = e2 < a.nnz && a.ri[e2] == lr;
-      if (!(whole && more)) break;
-    }
-    put_y(a.y, lr, tot.value(), a.init);
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.nchunks) return;
+  const int64_t cs = c * CH, ce = min(a.nnz, cs + (int64_t)CH);
+  const int32_t fr = a.ri[cs], lr = a.ri[ce - 1];
+  const bool sb = cs > 0 && a.ri[cs - 1] == fr;
+  const bool ca = ce < a.nnz && a.ri[ce] == lr;
+  if (!ca || (sb && fr == lr)) return;
+  S tot = S::load(a.last_val, c);
+  for (int64_t cc = c + 1; cc < a.nchunks; cc++) {
+    tot.merge(S::load(a.first_val, cc));
+    const int64_t s2 = cc * CH, e2 = min(a.nnz, s2 + (int64_t)CH);
+    const bool whole = a.ri[s2] == a.ri[e2 - 1];
+    const bool more = e2 < a.nnz && a.ri[e2] == lr;
+    if (!(whole && more)) break;
   }
+  put_y(a.y, lr, tot.value(), a.init);
 }
 
 template <typename V, int T, bool COMP>
@@ -251,17 +259,6 @@ __global__ __launch_bounds__(256) void coo_segmented_kernel(CooArgs<V> a) {
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const int64_t c = (int64_t)b * (256 / kWave) + wave;
   if (c < a.nchunks) coo_chunk<V, T, COMP>(a, c, lane);
-  // every wave publishes its scratch writes before the arrival count
-  __threadfence();
-  __syncthreads();
-  __shared__ int s_last;
-  if (threadIdx.x == 0)
-    s_last = atomic_add_acq_rel(a.done, 1) == (int)gridDim.x - 1;
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  coo_fixup<V, T, COMP>(a);
-  if (threadIdx.x == 0) atomic_store_relaxed(a.done, 0);
 }
 
 // Unsorted COO: y (cleared beforehand when init) += a*x with atomics.
@@ -326,8 +323,6 @@ int coo_analyse(mph_matrix_impl* A, hipStream_t s) {
   const size_t n = (size_t)std::max<int64_t>(p.nchunks, 1);
   MPH_HIP_CHECK(hipMalloc(&p.first_val, acc * pair * n));
   MPH_HIP_CHECK(hipMalloc(&p.last_val, acc * pair * n));
-  MPH_HIP_CHECK(hipMalloc(&p.done, sizeof(int32_t)));
-  MPH_CHECK_STATUS(fill_zero(p.done, sizeof(int32_t), s));
   MPH_HIP_CHECK(hipStreamSynchronize(s));
   return MPH_OK;
 }
@@ -351,7 +346,6 @@ CooArgs<V> coo_args(mph_matrix_impl* A, const void* x, void* y, int init) {
              (reinterpret_cast<uintptr_t>(A->coo_ci) % 16 == 0);
   a.first_val = static_cast<typename vtraits<V>::acc*>(A->coo.first_val);
   a.last_val = static_cast<typename vtraits<V>::acc*>(A->coo.last_val);
-  a.done = A->coo.done;
   return a;
 }
 
@@ -376,10 +370,20 @@ int coo_run(mph_matrix_impl* A, const void* x, void* y, int init,
     return MPH_OK;
   }
   const dim3 grid((unsigned)p.nblocks), block(256);
+  const dim3 fgrid((unsigned)ceil_div(p.nchunks, 256));
   switch (p.items) {
-    case 4: hipLaunchKernelGGL((coo_segmented_kernel<V, 4, comp>), grid, block, 0, s, a); break;
-    case 16: hipLaunchKernelGGL((coo_segmented_kernel<V, 16, comp>), grid, block, 0, s, a); break;
-    default: hipLaunchKernelGGL((coo_segmented_kernel<V, 8, comp>), grid, block, 0, s, a); break;
+    case 4:
+      hipLaunchKernelGGL((coo_segmented_kernel<V, 4, comp>), grid, block, 0, s, a);
+      hipLaunchKernelGGL((coo_fixup_kernel<V, 4, comp>), fgrid, block, 0, s, a);
+      break;
+    case 16:
+      hipLaunchKernelGGL((coo_segmented_kernel<V, 16, comp>), grid, block, 0, s, a);
+      hipLaunchKernelGGL((coo_fixup_kernel<V, 16, comp>), fgrid, block, 0, s, a);
+      break;
+    default:
+      hipLaunchKernelGGL((coo_segmented_kernel<V, 8, comp>), grid, block, 0, s, a);
+      hipLaunchKernelGGL((coo_fixup_kernel<V, 8, comp>), fgrid, block, 0, s, a);
+      break;
   }
   MPH_HIP_CHECK(hipGetLastError());
   return MPH_OK;

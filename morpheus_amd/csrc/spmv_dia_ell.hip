@@ -41,6 +41,7 @@ __global__ __launch_bounds__(256) void dia_kernel(DiaArgs<V> a) {
   for (int u = 0; u < RPT; u++)
 #pragma unroll
     for (int q = 0; q < KR; q++) t[u][q] = 0;
+#pragma unroll 4
   for (int d = 0; d < a.ndiag; d++) {
     const int64_t o = a.off[d];
     const V* vd = a.val + a.pitch * d;
@@ -92,6 +93,7 @@ __global__ __launch_bounds__(256) void ell_kernel(EllArgs<V> a) {
   for (int u = 0; u < RPT; u++)
 #pragma unroll
     for (int q = 0; q < KR; q++) t[u][q] = 0;
+#pragma unroll 4
   for (int n = 0; n < a.loop; n++) {
 #pragma unroll
     for (int u = 0; u < RPT; u++) {

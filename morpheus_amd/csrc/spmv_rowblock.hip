@@ -16,9 +16,14 @@
 //    once, coalesced and non-temporal, form the products a*x in LDS, then
 //    reduce each row with 1..64 lanes chosen from the block's mean row length
 //    (pow2 >= mean), and write y once (no separate zeroing pass).
-//  * Long-row chunk: the whole workgroup reduces `chunk` items; the last
-//    chunk to arrive (agent-scope acq_rel counter) adds the partials in chunk
-//    order, so FP64 results are deterministic run to run.
+//  * Long-row chunk: the whole workgroup reduces `chunk` items into a
+//    per-block partial with a plain store; a second, small launch adds each
+//    long row's partials in chunk order (and its DIA / ELL part), so FP64
+//    results are deterministic run to run.  There is no cross-workgroup
+//    synchronisation inside a kernel: on gfx950 every agent-scope
+//    release / acquire writes back / invalidates the whole L2 of the issuing
+//    XCD (the eight XCD L2s are not coherent with each other), which costs
+//    far more than the extra launch.
 //  * HYB / HDC: the ELL or DIA part of each row is computed by a
 //    thread-per-row pass (coalesced, column-major) into LDS and added to the
 //    row's CSR / COO-tail sum in the same launch, so y is written once.
@@ -57,7 +62,7 @@ struct RBArgs {
   int32_t ld;
   int init;
   typename vtraits<V>::acc* __restrict__ partial;
-  int32_t* __restrict__ counters;
+  int32_t nblocks;
 };
 
 // --------------------------------------------------------------- prefixes
@@ -136,57 +141,89 @@ __device__ __forceinline__ void store_y(Vec* y, int64_t i, A v, int init) {
   y[i] = init ? (Vec)v : (Vec)((A)y[i] + v);
 }
 
+// Products a_j * x[c_j] for items j = base + tid + k*256, k < kBatch: all
+// index and value loads of the batch are issued before the dependent x
+// gathers, so each thread keeps kBatch gathers in flight.
+constexpr int kBatch = 8;
+
+template <typename V, typename F>
+__device__ __forceinline__ void batched_products(const int32_t* __restrict__ ci,
+                                                 const V* __restrict__ val,
+                                                 const typename vtraits<V>::vec* __restrict__ x,
+                                                 int64_t begin, int64_t end, F&& sink) {
+  using Acc = typename vtraits<V>::acc;
+  for (int64_t base = begin + threadIdx.x; base < end;
+       base += (int64_t)kRowBlockThreads * kBatch) {
+    int32_t c[kBatch];
+    Acc v[kBatch];
+#pragma unroll
+    for (int k = 0; k < kBatch; k++) {
+      const int64_t i = base + (int64_t)k * kRowBlockThreads;
+      c[k] = i < end ? ld_stream(ci + i) : 0;
+      v[k] = i < end ? vtraits<V>::widen(ld_stream(val + i)) : (Acc)0;
+    }
+    Acc xv[kBatch];
+#pragma unroll
+    for (int k = 0; k < kBatch; k++) {
+      const int64_t i = base + (int64_t)k * kRowBlockThreads;
+      xv[k] = i < end ? (Acc)x[c[k]] : (Acc)0;
+    }
+#pragma unroll
+    for (int k = 0; k < kBatch; k++) {
+      const int64_t i = base + (int64_t)k * kRowBlockThreads;
+      if (i < end) sink(i, v[k] * xv[k]);
+    }
+  }
+}
+
 // --------------------------------------------------------------- k = 1
-// Long-row chunk shared by SpMV variants: workgroup sum of one chunk, then
-// last-arriver combination in chunk order.
-template <typename V, int PRE, bool COMP>
+// Long-row chunk: workgroup sum of one chunk of one row into partial[b].
+template <typename V, bool COMP>
 __device__ void long_chunk_spmv(const RBArgs<V>& a, uint32_t b, RowBlock blk,
                                 typename vtraits<V>::acc* s_red) {
   using Acc = typename vtraits<V>::acc;
   const int64_t r = blk.row;
   const int64_t rs = a.ro[r], re = a.ro[r + 1];
-  const int64_t nsub = (re - rs + a.chunk - 1) / a.chunk;
   const int64_t s = rs + (int64_t)blk.sub * a.chunk;
   const int64_t e = min(re, s + (int64_t)a.chunk);
-  __shared__ int s_last;
   if constexpr (COMP) {
     csum<Acc> cs{0, 0};
-    for (int64_t i = s + threadIdx.x; i < e; i += kRowBlockThreads)
-      cs.add(vtraits<V>::widen(ld_stream(a.val + i)) * (Acc)a.x[ld_stream(a.ci + i)]);
+    batched_products<V>(a.ci, a.val, a.x, s, e, [&](int64_t, Acc p) { cs.add(p); });
     block_csum(cs, s_red);
     if (threadIdx.x == 0) {
       a.partial[2 * b] = cs.s;
       a.partial[2 * b + 1] = cs.c;
-      s_last = atomic_add_acq_rel(&a.counters[b - blk.sub], 1) == nsub - 1;
     }
   } else {
     Acc t = 0;
-    for (int64_t i = s + threadIdx.x; i < e; i += kRowBlockThreads)
-      t += vtraits<V>::widen(ld_stream(a.val + i)) * (Acc)a.x[ld_stream(a.ci + i)];
+    batched_products<V>(a.ci, a.val, a.x, s, e, [&](int64_t, Acc p) { t += p; });
     t = block_sum(t, s_red);
-    if (threadIdx.x == 0) {
-      a.partial[b] = t;
-      s_last = atomic_add_acq_rel(&a.counters[b - blk.sub], 1) == nsub - 1;
-    }
+    if (threadIdx.x == 0) a.partial[b] = t;
   }
-  __syncthreads();
-  if (!s_last) return;
-  // the last chunk of this row: combine every partial in chunk order
-  if (threadIdx.x == 0) {
-    const uint32_t b0 = b - blk.sub;
-    Acc pre[1] = {0};
-    row_prefix<V, PRE, 1>(a, r, pre);
-    if constexpr (COMP) {
-      csum<Acc> cs{pre[0], 0};
-      for (int64_t q = 0; q < nsub; q++)
-        cs.combine(a.partial[2 * (b0 + q)], a.partial[2 * (b0 + q) + 1]);
-      store_y(a.y, r, cs.value(), a.init);
-    } else {
-      Acc t = 0;
-      for (int64_t q = 0; q < nsub; q++) t += a.partial[b0 + q];
-      store_y(a.y, r, t + pre[0], a.init);
-    }
-    atomic_store_relaxed(&a.counters[b0], 0);
+}
+
+// Second launch: one thread per long row (the block holding its chunk 0)
+// adds the row's partials in chunk order plus its DIA / ELL part.
+template <typename V, int PRE, bool COMP>
+__global__ void rowblock_combine_kernel(RBArgs<V> a) {
+  using Acc = typename vtraits<V>::acc;
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= a.nblocks) return;
+  const RowBlock blk = a.blocks[b];
+  if (blk.sub != 0) return;
+  const int64_t r = blk.row;
+  const int64_t nsub = ((int64_t)a.ro[r + 1] - a.ro[r] + a.chunk - 1) / a.chunk;
+  Acc pre[1] = {0};
+  row_prefix<V, PRE, 1>(a, r, pre);
+  if constexpr (COMP) {
+    csum<Acc> cs{pre[0], 0};
+    for (int64_t q = 0; q < nsub; q++)
+      cs.combine(a.partial[2 * (b + q)], a.partial[2 * (b + q) + 1]);
+    store_y(a.y, r, cs.value(), a.init);
+  } else {
+    Acc t = 0;
+    for (int64_t q = 0; q < nsub; q++) t += a.partial[b + q];
+    store_y(a.y, r, t + pre[0], a.init);
   }
 }
 
@@ -199,7 +236,7 @@ __global__ __launch_bounds__(kRowBlockThreads) void rowblock_spmv_kernel(
   const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
   const RowBlock blk = a.blocks[b];
   if (blk.sub >= 0) {
-    long_chunk_spmv<V, PRE, COMP>(a, b, blk, s_red);
+    long_chunk_spmv<V, COMP>(a, b, blk, s_red);
     return;
   }
   Acc* s_prod = reinterpret_cast<Acc*>(smem);
@@ -214,10 +251,8 @@ __global__ __launch_bounds__(kRowBlockThreads) void rowblock_spmv_kernel(
 
   for (int i = threadIdx.x; i <= nr; i += kRowBlockThreads)
     s_ro[i] = (int32_t)(a.ro[row0 + i] - p0);
-  for (int i = threadIdx.x; i < n; i += kRowBlockThreads) {
-    const int32_t c = ld_stream(a.ci + p0 + i);
-    s_prod[i] = vtraits<V>::widen(ld_stream(a.val + p0 + i)) * (Acc)a.x[c];
-  }
+  batched_products<V>(a.ci + p0, a.val + p0, a.x, 0, n,
+                      [&](int64_t i, Acc p) { s_prod[i] = p; });
   if constexpr (PRE != PRE_NONE) {
     for (int i = threadIdx.x; i < nr; i += kRowBlockThreads) {
       Acc p[1] = {0};
@@ -263,13 +298,11 @@ __global__ __launch_bounds__(kRowBlockThreads) void rowblock_spmm_kernel(
     RBArgs<V> a) {
   using Acc = typename vtraits<V>::acc;
   __shared__ Acc s_red[KR * kRowBlockThreads / kWave];
-  __shared__ int s_last;
   const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
   const RowBlock blk = a.blocks[b];
   if (blk.sub >= 0) {
     const int64_t r = blk.row;
     const int64_t rs = a.ro[r], re = a.ro[r + 1];
-    const int64_t nsub = (re - rs + a.chunk - 1) / a.chunk;
     const int64_t s = rs + (int64_t)blk.sub * a.chunk;
     const int64_t e = min(re, s + (int64_t)a.chunk);
     Acc t[KR];
@@ -295,24 +328,6 @@ __global__ __launch_bounds__(kRowBlockThreads) void rowblock_spmm_kernel(
       for (int w = 0; w < kRowBlockThreads / kWave; w++)
         u += s_red[threadIdx.x * (kRowBlockThreads / kWave) + w];
       a.partial[(int64_t)b * KR + threadIdx.x] = u;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0)
-      s_last = atomic_add_acq_rel(&a.counters[b - blk.sub], 1) == nsub - 1;
-    __syncthreads();
-    if (!s_last) return;
-    if (threadIdx.x == 0) {
-      const uint32_t b0 = b - blk.sub;
-      Acc tot[KR];
-#pragma unroll
-      for (int q = 0; q < KR; q++) tot[q] = 0;
-      row_prefix<V, PRE, KR>(a, r, tot);
-      for (int64_t j = 0; j < nsub; j++)
-#pragma unroll
-        for (int q = 0; q < KR; q++) tot[q] += a.partial[(int64_t)(b0 + j) * KR + q];
-#pragma unroll
-      for (int q = 0; q < KR; q++) store_y(a.y, r * a.ld + q, tot[q], a.init);
-      atomic_store_relaxed(&a.counters[b0], 0);
     }
     return;
   }
@@ -366,6 +381,26 @@ __global__ __launch_bounds__(kRowBlockThreads) void rowblock_spmm_kernel(
       for (int q = 0; q < KR; q++) store_y(a.y, row * a.ld + q, t[q], a.init);
     }
   }
+}
+
+template <typename V, int PRE, int KR>
+__global__ void rowblock_spmm_combine_kernel(RBArgs<V> a) {
+  using Acc = typename vtraits<V>::acc;
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= a.nblocks) return;
+  const RowBlock blk = a.blocks[b];
+  if (blk.sub != 0) return;
+  const int64_t r = blk.row;
+  const int64_t nsub = ((int64_t)a.ro[r + 1] - a.ro[r] + a.chunk - 1) / a.chunk;
+  Acc tot[KR];
+#pragma unroll
+  for (int q = 0; q < KR; q++) tot[q] = 0;
+  row_prefix<V, PRE, KR>(a, r, tot);
+  for (int64_t j = 0; j < nsub; j++)
+#pragma unroll
+    for (int q = 0; q < KR; q++) tot[q] += a.partial[(b + j) * KR + q];
+#pragma unroll
+  for (int q = 0; q < KR; q++) store_y(a.y, r * a.ld + q, tot[q], a.init);
 }
 
 // --------------------------------------------------------------- analysis
@@ -534,9 +569,7 @@ int rowblock_analyse(mph_matrix_impl* A, hipStream_t s) {
     // partials: up to 16 RHS per block (SpMM) or a compensated pair (SpMV)
     MPH_HIP_CHECK(hipMalloc(&p.blocks, sizeof(RowBlock) * (nb + 1)));
     MPH_HIP_CHECK(hipMalloc(&p.partial, (size_t)accs * 16 * (nb + 1)));
-    MPH_HIP_CHECK(hipMalloc(&p.counters, sizeof(int32_t) * (nb + 1)));
-    st = fill_zero(p.counters, sizeof(int32_t) * (nb + 1), s);
-    if (st == MPH_OK) {
+    {
       hipLaunchKernelGGL(rb_fill_kernel, dim3(g), dim3(256), 0, s, ro, nrows,
                          p.chunk, first, p.blocks);
       MPH_HIP_CHECK(hipGetLastError());
@@ -583,7 +616,7 @@ RBArgs<V> make_args(mph_matrix_impl* A, const void* x, void* y, int ld,
   a.ld = ld;
   a.init = init;
   a.partial = static_cast<typename vtraits<V>::acc*>(A->rb.partial);
-  a.counters = A->rb.counters;
+  a.nblocks = A->rb.nblocks;
   return a;
 }
 
@@ -598,6 +631,9 @@ int launch_spmv(mph_matrix_impl* A, const void* x, void* y, int init,
   hipLaunchKernelGGL((rowblock_spmv_kernel<V, PRE, comp>),
                      dim3((unsigned)A->rb.nblocks), dim3(kRowBlockThreads), lds,
                      s, a);
+  if (A->rb.nlong > 0)
+    hipLaunchKernelGGL((rowblock_combine_kernel<V, PRE, comp>),
+                       dim3((unsigned)ceil_div(A->rb.nblocks, 256)), dim3(256), 0, s, a);
   MPH_HIP_CHECK(hipGetLastError());
   return MPH_OK;
 }
@@ -609,6 +645,9 @@ int launch_spmm_k(mph_matrix_impl* A, const void* x, void* y, int ld, int init,
   hipLaunchKernelGGL((rowblock_spmm_kernel<V, PRE, KR>),
                      dim3((unsigned)A->rb.nblocks), dim3(kRowBlockThreads), 0, s,
                      a);
+  if (A->rb.nlong > 0)
+    hipLaunchKernelGGL((rowblock_spmm_combine_kernel<V, PRE, KR>),
+                       dim3((unsigned)ceil_div(A->rb.nblocks, 256)), dim3(256), 0, s, a);
   MPH_HIP_CHECK(hipGetLastError());
   return MPH_OK;
 }
@@ -648,7 +687,9 @@ int launch_spmm(mph_matrix_impl* A, int k, const void* x, void* y, int init,
 template <typename V>
 int dispatch_pre(mph_matrix_impl* A, int k, const void* x, void* y, int init,
                  hipStream_t s) {
-  const int pre = A->format == MPH_HDC ? PRE_DIA : A->format == MPH_HYB ? PRE_ELL : PRE_NONE;
+  int pre = A->format == MPH_HDC ? PRE_DIA : A->format == MPH_HYB ? PRE_ELL : PRE_NONE;
+  // an HDC without diagonals / a HYB without ELL slots is its CSR-shaped part
+  if ((pre == PRE_DIA && A->ndiag == 0) || (pre == PRE_ELL && A->ell_loop == 0)) pre = PRE_NONE;
   if (k == 1) {
     if (pre == PRE_DIA) return launch_spmv<V, PRE_DIA>(A, x, y, init, s);
     if (pre == PRE_ELL) return launch_spmv<V, PRE_ELL>(A, x, y, init, s);

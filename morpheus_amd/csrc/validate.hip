@@ -87,8 +87,8 @@ int check_row_offsets(const int32_t* ro, int64_t nrows, int64_t nnz,
   return MPH_OK;
 }
 
-int validate_rowblock_plan(const int32_t* d_ro, int64_t nrows,
-                           const RowBlockPlan& p, hipStream_t s) {
+int validate_rowblock_plan(const int32_t* d_ro, int64_t nrows, RowBlockPlan& p,
+                           hipStream_t s) {
   const int64_t nb = p.nblocks;
   std::vector<int32_t> ro(nrows + 1);
   std::vector<RowBlock> bl(nb + 1);
@@ -103,6 +103,7 @@ int validate_rowblock_plan(const int32_t* d_ro, int64_t nrows,
   };
   if (bl[nb].row != nrows || bl[nb].sub != -1) return fail("sentinel");
   if (nb > 0 && bl[0].row != 0) return fail("first block");
+  p.nlong = 0;
   for (int64_t b = 0; b < nb; b++) {
     const RowBlock k = bl[b];
     if (k.row < 0 || k.row >= nrows) return fail("row out of range");
@@ -120,6 +121,7 @@ int validate_rowblock_plan(const int32_t* d_ro, int64_t nrows,
       if (k.sub >= nsub || b - k.sub < 0) return fail("chunk index");
       const RowBlock f = bl[b - k.sub];
       if (f.row != k.row || f.sub != 0) return fail("chunk numbering");
+      if (k.sub == 0) p.nlong++;
       if (k.sub == nsub - 1 && bl[b + 1].row != k.row + 1 && bl[b + 1].row <= k.row)
         return fail("chunk successor");
     }
