@@ -314,6 +314,20 @@ int mph_batch_info(mph_batch_t b, int64_t* ntiles, int32_t* nworkgroups);
 int mph_matrix_autotune(mph_matrix_t A, int32_t k, const void* x, void* y,
                         int32_t reps, mph_stream_t stream, float* best_ms);
 
+/* Converts the CSR matrix to every admissible format on the GPU (formats
+ * whose conversion throws FormatConversionException in the reference are
+ * skipped), times `reps` products with k right-hand sides each, and returns
+ * the fastest format in *best_format (an mph_format).  ms, if not NULL,
+ * receives the median time per format (-1 for skipped formats).  The
+ * counterpart of choosing a DynamicMatrix's active format
+ * (core/src/Morpheus_DynamicMatrix.hpp:409-442) by measurement instead of by
+ * the --morpheus-format flag (core/src/impl/Morpheus_Core.cpp:92-140). */
+int mph_autotune_format(int64_t nrows, int64_t ncols, int64_t nnz,
+                        const int32_t* row_offsets,
+                        const int32_t* column_indices, const void* values,
+                        int value_type, int32_t k, int32_t reps,
+                        mph_stream_t stream, int* best_format, float* ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -121,6 +121,8 @@ _SIGS = {
     "mph_batch_run_graph": (I, [p, I, p]),
     "mph_batch_info": (I, [p, pi64, C.POINTER(i32)]),
     "mph_matrix_autotune": (I, [p, i32, p, p, i32, p, C.POINTER(C.c_float)]),
+    "mph_autotune_format": (I, [i64, i64, i64, p, p, p, I, i32, i32, p, pI,
+                                C.POINTER(C.c_float)]),
 }
 
 _lib = None

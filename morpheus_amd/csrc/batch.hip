@@ -1,0 +1,450 @@
+// batch.hip -- batched SpMM: many matrices of uneven size and structure,
+// each with k right-hand sides, in ONE persistent launch that walks a work
+// queue of (matrix, row block) tiles; replayable through a hipGraph.
+//
+// There is no counterpart in the reference (its multiply takes one matrix and
+// one vector, core/src/Morpheus_Multiply.hpp:57-78); this is the GPU form of
+// running Morpheus::multiply over a set of DynamicMatrix objects.
+//
+// Design (gfx950):
+//  * tiles: a short CSR row block (rows whose items start in one window of
+//    the matrix's row-block analysis, optionally with a fused DIA / ELL part
+//    for HDC / HYB), a whole long row (one workgroup, no partials), or a
+//    256-row slab of a DIA or ELL matrix;
+//  * tiles are sorted largest first on the host, and workgroups (grid = a few
+//    per CU) take the next tile from an atomic counter, so the uneven sizes
+//    balance dynamically; a one-thread kernel before the main kernel clears
+//    the counter, which keeps the pair capturable (no memset nodes);
+//  * X / Y are row-major with k columns: every matrix entry feeds k FMAs from
+//    one contiguous x row (16-byte vector loads when k*sizeof(x) allows).
+#include <algorithm>
+#include <vector>
+
+#include "mph_internal.hpp"
+
+namespace mph {
+
+enum { TILE_SHORT = 0, TILE_WHOLEROW = 1, TILE_DIA = 2, TILE_ELL = 3 };
+enum { BPRE_NONE = 0, BPRE_DIA = 1, BPRE_ELL = 2 };
+
+struct BatchMat {
+  int64_t nrows, ncols;
+  const int32_t* ro;
+  const int32_t* ci;
+  const void* val;
+  int32_t pre;
+  int32_t ndiag;
+  int64_t dpitch;
+  const int32_t* doff;
+  const void* dval;
+  int32_t eloop;
+  int64_t epitch;
+  const int32_t* eci;
+  const void* evalp;
+  const void* x;
+  void* y;
+};
+
+struct BatchTile {
+  int32_t mat;
+  int32_t kind;
+  int32_t r0;
+  int32_t r1;
+};
+
+constexpr int kBT = 256;
+constexpr int kSlab = 256;
+
+template <typename V, int KR>
+struct XRow {
+  using Acc = typename vtraits<V>::acc;
+  using Vec = typename vtraits<V>::vec;
+  // acc[q] += v * x[c*KR + q]
+  __device__ __forceinline__ static void fma(Acc* acc, Acc v, const Vec* __restrict__ x,
+                                             int64_t c) {
+    const Vec* p = x + c * KR;
+    if constexpr ((KR * sizeof(Vec)) % 16 == 0) {
+      typedef Vec v16 __attribute__((ext_vector_type(16 / sizeof(Vec))));
+      constexpr int W = 16 / sizeof(Vec);
+#pragma unroll
+      for (int q = 0; q < KR; q += W) {
+        const v16 xv = *reinterpret_cast<const v16*>(p + q);
+#pragma unroll
+        for (int w = 0; w < W; w++) acc[q + w] += v * (Acc)xv[w];
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < KR; q++) acc[q] += v * (Acc)p[q];
+    }
+  }
+};
+
+template <typename V, int KR>
+__device__ __forceinline__ void store_row(const BatchMat& m, int64_t r,
+                                          const typename vtraits<V>::acc* acc, int init) {
+  using Vec = typename vtraits<V>::vec;
+  using Acc = typename vtraits<V>::acc;
+  Vec* y = static_cast<Vec*>(m.y) + r * KR;
+#pragma unroll
+  for (int q = 0; q < KR; q++) y[q] = init ? (Vec)acc[q] : (Vec)((Acc)y[q] + acc[q]);
+}
+
+template <typename V, int KR>
+__device__ void row_items(const BatchMat& m, int64_t r, int lane, int L,
+                          typename vtraits<V>::acc* acc) {
+  using Acc = typename vtraits<V>::acc;
+  using Vec = typename vtraits<V>::vec;
+  const V* val = static_cast<const V*>(m.val);
+  const Vec* x = static_cast<const Vec*>(m.x);
+  if (m.pre == BPRE_DIA) {
+    const V* dv = static_cast<const V*>(m.dval);
+    for (int d = lane; d < m.ndiag; d += L) {
+      const int64_t c = r + m.doff[d];
+      if (c >= 0 && c < m.ncols)
+        XRow<V, KR>::fma(acc, vtraits<V>::widen(ld_stream(dv + r + m.dpitch * d)), x, c);
+    }
+  } else if (m.pre == BPRE_ELL) {
+    const V* ev = static_cast<const V*>(m.evalp);
+    for (int n = lane; n < m.eloop; n += L) {
+      const int64_t slot = r + m.epitch * n;
+      const int32_t c = ld_stream(m.eci + slot);
+      if (c != -1) XRow<V, KR>::fma(acc, vtraits<V>::widen(ld_stream(ev + slot)), x, c);
+    }
+  }
+  const int64_t st = m.ro[r], en = m.ro[r + 1];
+  for (int64_t j = st + lane; j < en; j += L)
+    XRow<V, KR>::fma(acc, vtraits<V>::widen(ld_stream(val + j)), x, ld_stream(m.ci + j));
+}
+
+template <typename V, int KR>
+__device__ void tile_short(const BatchMat& m, const BatchTile& t, int init) {
+  using Acc = typename vtraits<V>::acc;
+  const int nr = t.r1 - t.r0;
+  const int64_t items = (int64_t)m.ro[t.r1] - m.ro[t.r0];
+  int L = 1;
+  const int64_t mean = nr > 0 ? items / nr : 0;
+  while (L < kWave && L < mean) L <<= 1;
+  if (m.pre != BPRE_NONE && L < 4) L = 4;
+  const int groups = kBT / L;
+  const int g = threadIdx.x / L, l = threadIdx.x % L;
+  for (int rr = g; rr < nr; rr += groups) {
+    const int64_t r = t.r0 + rr;
+    Acc acc[KR];
+#pragma unroll
+    for (int q = 0; q < KR; q++) acc[q] = 0;
+    row_items<V, KR>(m, r, l, L, acc);
+#pragma unroll
+    for (int q = 0; q < KR; q++) acc[q] = group_sum(acc[q], L);
+    if (l == 0) store_row<V, KR>(m, r, acc, init);
+  }
+}
+
+template <typename V, int KR>
+__device__ void tile_wholerow(const BatchMat& m, const BatchTile& t, int init,
+                              typename vtraits<V>::acc* s_red) {
+  using Acc = typename vtraits<V>::acc;
+  Acc acc[KR];
+#pragma unroll
+  for (int q = 0; q < KR; q++) acc[q] = 0;
+  row_items<V, KR>(m, t.r0, threadIdx.x, kBT, acc);
+  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+#pragma unroll
+  for (int q = 0; q < KR; q++) {
+    for (int s = kWave >> 1; s > 0; s >>= 1) acc[q] += shfl_xor(acc[q], s);
+    if (lane == 0) s_red[q * (kBT / kWave) + wave] = acc[q];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    Acc tot[KR];
+#pragma unroll
+    for (int q = 0; q < KR; q++) {
+      tot[q] = 0;
+      for (int w = 0; w < kBT / kWave; w++) tot[q] += s_red[q * (kBT / kWave) + w];
+    }
+    store_row<V, KR>(m, t.r0, tot, init);
+  }
+}
+
+template <typename V, int KR>
+__device__ void tile_dia(const BatchMat& m, const BatchTile& t, int init) {
+  using Acc = typename vtraits<V>::acc;
+  using Vec = typename vtraits<V>::vec;
+  const int64_t r = t.r0 + threadIdx.x;
+  if (r >= t.r1) return;
+  const V* dv = static_cast<const V*>(m.dval);
+  const Vec* x = static_cast<const Vec*>(m.x);
+  Acc acc[KR];
+#pragma unroll
+  for (int q = 0; q < KR; q++) acc[q] = 0;
+  for (int d = 0; d < m.ndiag; d++) {
+    const int64_t c = r + m.doff[d];
+    if (c >= 0 && c < m.ncols)
+      XRow<V, KR>::fma(acc, vtraits<V>::widen(ld_stream(dv + r + m.dpitch * d)), x, c);
+  }
+  store_row<V, KR>(m, r, acc, init);
+}
+
+template <typename V, int KR>
+__device__ void tile_ell(const BatchMat& m, const BatchTile& t, int init) {
+  using Acc = typename vtraits<V>::acc;
+  using Vec = typename vtraits<V>::vec;
+  const int64_t r = t.r0 + threadIdx.x;
+  if (r >= t.r1) return;
+  const V* ev = static_cast<const V*>(m.evalp);
+  const Vec* x = static_cast<const Vec*>(m.x);
+  Acc acc[KR];
+#pragma unroll
+  for (int q = 0; q < KR; q++) acc[q] = 0;
+  for (int n = 0; n < m.eloop; n++) {
+    const int64_t slot = r + m.epitch * n;
+    const int32_t c = ld_stream(m.eci + slot);
+    if (c != -1) XRow<V, KR>::fma(acc, vtraits<V>::widen(ld_stream(ev + slot)), x, c);
+  }
+  store_row<V, KR>(m, r, acc, init);
+}
+
+__global__ void batch_reset_kernel(int32_t* counter) { *counter = 0; }
+
+template <typename V, int KR>
+__global__ __launch_bounds__(kBT) void batch_spmm_kernel(const BatchMat* __restrict__ mats,
+                                                         const BatchTile* __restrict__ tiles,
+                                                         int32_t ntiles, int32_t* counter,
+                                                         int init) {
+  using Acc = typename vtraits<V>::acc;
+  __shared__ Acc s_red[KR * kBT / kWave];
+  __shared__ int32_t s_tile;
+  while (true) {
+    if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);
+    __syncthreads();
+    const int32_t ti = s_tile;
+    __syncthreads();
+    if (ti >= ntiles) break;  // every workgroup reaches this exit
+    const BatchTile t = tiles[ti];
+    const BatchMat m = mats[t.mat];
+    switch (t.kind) {
+      case TILE_SHORT: tile_short<V, KR>(m, t, init); break;
+      case TILE_WHOLEROW: tile_wholerow<V, KR>(m, t, init, s_red); break;
+      case TILE_DIA: tile_dia<V, KR>(m, t, init); break;
+      default: tile_ell<V, KR>(m, t, init); break;
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace mph
+
+// ------------------------------------------------------------------ host
+struct mph_batch_s {
+  int vt = MPH_F32;
+  int k = 1;
+  int32_t ntiles = 0;
+  int32_t nwg = 0;
+  mph::BatchMat* d_mats = nullptr;
+  mph::BatchTile* d_tiles = nullptr;
+  int32_t* d_counter = nullptr;
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+};
+
+namespace mph {
+namespace {
+
+int64_t tile_cost(const BatchTile& t, const std::vector<int32_t>* ro, const mph_matrix_impl* A) {
+  if (t.kind == TILE_DIA) return (int64_t)(t.r1 - t.r0) * std::max<int64_t>(A->ndiag, 1);
+  if (t.kind == TILE_ELL) return (int64_t)(t.r1 - t.r0) * std::max<int64_t>(A->ell_loop, 1);
+  int64_t c = (int64_t)(*ro)[t.r1] - (*ro)[t.r0];
+  if (A->format == MPH_HDC) c += (int64_t)(t.r1 - t.r0) * A->ndiag;
+  if (A->format == MPH_HYB) c += (int64_t)(t.r1 - t.r0) * A->ell_loop;
+  return c;
+}
+
+template <typename V, int KR>
+int launch_batch(mph_batch_s* b, int init, hipStream_t s) {
+  hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(1), 0, s, b->d_counter);
+  hipLaunchKernelGGL((batch_spmm_kernel<V, KR>), dim3((unsigned)b->nwg), dim3(kBT), 0, s,
+                     b->d_mats, b->d_tiles, b->ntiles, b->d_counter, init);
+  MPH_HIP_CHECK(hipGetLastError());
+  return MPH_OK;
+}
+
+template <typename V>
+int launch_k(mph_batch_s* b, int init, hipStream_t s) {
+  switch (b->k) {
+    case 1: return launch_batch<V, 1>(b, init, s);
+    case 2: return launch_batch<V, 2>(b, init, s);
+    case 4: return launch_batch<V, 4>(b, init, s);
+    case 8: return launch_batch<V, 8>(b, init, s);
+    case 16: return launch_batch<V, 16>(b, init, s);
+  }
+  return invalid("batched k must be 1, 2, 4, 8 or 16");
+}
+
+int batch_launch(mph_batch_s* b, int init, hipStream_t s) {
+  if (b->ntiles == 0) return MPH_OK;
+  switch (b->vt) {
+    case MPH_F64: return launch_k<double>(b, init, s);
+    case MPH_F32: return launch_k<float>(b, init, s);
+    case MPH_BF16: return launch_k<bf16>(b, init, s);
+  }
+  return invalid("unknown value type");
+}
+
+}  // namespace
+}  // namespace mph
+
+using namespace mph;
+
+extern "C" {
+
+int mph_batch_create(mph_batch_t* out, int32_t nmat, const mph_matrix_t* mats, int32_t k,
+                     const void* const* X, void* const* Y, mph_stream_t stream) {
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (!out || nmat < 0 || (nmat > 0 && (!mats || !X || !Y))) return invalid("null batch argument");
+  if (k != 1 && k != 2 && k != 4 && k != 8 && k != 16)
+    return invalid("batched k must be 1, 2, 4, 8 or 16");
+  std::vector<BatchMat> hm(nmat);
+  std::vector<BatchTile> tiles;
+  std::vector<int64_t> cost;
+  int vt = nmat > 0 ? mats[0]->vt : MPH_F32;
+  for (int32_t i = 0; i < nmat; i++) {
+    mph_matrix_impl* A = mats[i];
+    if (!A) return invalid("null matrix in batch");
+    if (A->vt != vt) return invalid("all batched matrices must share one value type");
+    if (A->format == MPH_COO) return invalid("COO matrices are batched through CSR");
+    if (!A->analysed) MPH_CHECK_STATUS(mph_matrix_analyse(mats[i], stream));
+    if (A->format == MPH_HYB && A->rb.tail_ro == nullptr && A->coo_nnz > 0)
+      return invalid("HYB with an unsorted COO part cannot be batched");
+    BatchMat m{};
+    m.nrows = A->nrows;
+    m.ncols = A->ncols;
+    m.x = X[i];
+    m.y = Y[i];
+    if (A->nrows > 0 && (!X[i] || !Y[i])) return invalid("null X / Y in batch");
+    const size_t vsz = vector_size(vt);
+    if ((k * vsz) % 16 == 0 && ((reinterpret_cast<uintptr_t>(X[i]) % 16) != 0 ||
+                                (reinterpret_cast<uintptr_t>(Y[i]) % 16) != 0))
+      return invalid("batched X / Y rows must be 16-byte aligned");
+    std::vector<int32_t> ro;
+    // an HDC without CSR entries is its DIA part, a HYB without COO entries
+    // its ELL part: they are cut into slabs like DIA / ELL matrices
+    const bool as_dia = A->format == MPH_DIA || (A->format == MPH_HDC && A->csr_nnz == 0);
+    const bool as_ell = A->format == MPH_ELL || (A->format == MPH_HYB && A->coo_nnz == 0);
+    const bool rowfam = !as_dia && !as_ell;
+    if (rowfam) {
+      m.ro = A->format == MPH_HYB ? A->rb.tail_ro : A->csr_ro;
+      m.ci = A->format == MPH_HYB ? A->coo_ci : A->csr_ci;
+      m.val = A->format == MPH_HYB ? A->coo_v : A->csr_v;
+      m.pre = A->format == MPH_HDC ? BPRE_DIA : A->format == MPH_HYB ? BPRE_ELL : BPRE_NONE;
+      ro.resize(A->nrows + 1);
+      if (A->nrows > 0) {
+        MPH_HIP_CHECK(hipMemcpyAsync(ro.data(), m.ro, sizeof(int32_t) * (A->nrows + 1),
+                                     hipMemcpyDeviceToHost, s));
+        std::vector<RowBlock> bl(A->rb.nblocks + 1);
+        MPH_HIP_CHECK(hipMemcpyAsync(bl.data(), A->rb.blocks, sizeof(RowBlock) * bl.size(),
+                                     hipMemcpyDeviceToHost, s));
+        MPH_HIP_CHECK(hipStreamSynchronize(s));
+        for (int32_t q = 0; q < A->rb.nblocks; q++) {
+          if (bl[q].sub > 0) continue;  // long rows: one whole-row tile
+          BatchTile t{i, bl[q].sub < 0 ? TILE_SHORT : TILE_WHOLEROW, bl[q].row,
+                      bl[q].sub < 0 ? bl[q + 1].row : bl[q].row + 1};
+          tiles.push_back(t);
+          cost.push_back(tile_cost(t, &ro, A));
+        }
+      }
+    } else {
+      for (int64_t r = 0; r < A->nrows; r += kSlab) {
+        BatchTile t{i, as_dia ? TILE_DIA : TILE_ELL, (int32_t)r,
+                    (int32_t)std::min<int64_t>(A->nrows, r + kSlab)};
+        tiles.push_back(t);
+        cost.push_back(tile_cost(t, nullptr, A));
+      }
+    }
+    m.ndiag = (int32_t)A->ndiag;
+    m.dpitch = A->dia_pitch;
+    m.doff = A->dia_off;
+    m.dval = A->dia_v;
+    m.eloop = (int32_t)A->ell_loop;
+    m.epitch = A->ell_pitch;
+    m.eci = A->ell_ci;
+    m.evalp = A->ell_v;
+    hm[i] = m;
+  }
+  // largest tiles first: the dynamic queue then drains evenly
+  std::vector<size_t> order(tiles.size());
+  for (size_t q = 0; q < order.size(); q++) order[q] = q;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+  std::vector<BatchTile> sorted(tiles.size());
+  for (size_t q = 0; q < order.size(); q++) sorted[q] = tiles[order[q]];
+
+  mph_batch_s* b = new mph_batch_s();
+  b->vt = vt;
+  b->k = k;
+  b->ntiles = (int32_t)sorted.size();
+  b->nwg = (int32_t)std::max<int64_t>(1, std::min<int64_t>((int64_t)device_cu_count() * 8,
+                                                           (int64_t)b->ntiles));
+  hipError_t e = hipMalloc(&b->d_mats, sizeof(BatchMat) * std::max<int32_t>(nmat, 1));
+  if (e == hipSuccess) e = hipMalloc(&b->d_tiles, sizeof(BatchTile) * std::max<int32_t>(b->ntiles, 1));
+  if (e == hipSuccess) e = hipMalloc(&b->d_counter, sizeof(int32_t));
+  if (e == hipSuccess && nmat > 0)
+    e = hipMemcpyAsync(b->d_mats, hm.data(), sizeof(BatchMat) * nmat, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && b->ntiles > 0)
+    e = hipMemcpyAsync(b->d_tiles, sorted.data(), sizeof(BatchTile) * b->ntiles,
+                       hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    set_error(std::string("batch create: ") + hipGetErrorString(e));
+    mph_batch_destroy(b);
+    return MPH_ERR_HIP;
+  }
+  *out = b;
+  return MPH_OK;
+}
+
+int mph_batch_destroy(mph_batch_t b) {
+  if (!b) return MPH_OK;
+  for (int i = 0; i < 2; i++) {
+    if (b->exec[i]) (void)hipGraphExecDestroy(b->exec[i]);
+    if (b->graph[i]) (void)hipGraphDestroy(b->graph[i]);
+  }
+  dfree(b->d_mats);
+  dfree(b->d_tiles);
+  dfree(b->d_counter);
+  delete b;
+  return MPH_OK;
+}
+
+int mph_batch_run(mph_batch_t b, int init, mph_stream_t stream) {
+  if (!b) return invalid("null batch");
+  return batch_launch(b, init ? 1 : 0, reinterpret_cast<hipStream_t>(stream));
+}
+
+int mph_batch_run_graph(mph_batch_t b, int init, mph_stream_t stream) {
+  if (!b) return invalid("null batch");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const int g = init ? 1 : 0;
+  if (!b->exec[g]) {
+    hipStream_t cap = nullptr;
+    MPH_HIP_CHECK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+    MPH_HIP_CHECK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+    const int st = batch_launch(b, g, cap);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(cap, &graph);
+    (void)hipStreamDestroy(cap);
+    if (st != MPH_OK) return st;
+    MPH_HIP_CHECK(e);
+    b->graph[g] = graph;
+    MPH_HIP_CHECK(hipGraphInstantiate(&b->exec[g], graph, nullptr, nullptr, 0));
+  }
+  MPH_HIP_CHECK(hipGraphLaunch(b->exec[g], s));
+  return MPH_OK;
+}
+
+int mph_batch_info(mph_batch_t b, int64_t* ntiles, int32_t* nwg) {
+  if (!b) return invalid("null batch");
+  if (ntiles) *ntiles = b->ntiles;
+  if (nwg) *nwg = b->nwg;
+  return MPH_OK;
+}
+
+}  // extern "C"

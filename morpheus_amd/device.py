@@ -358,3 +358,66 @@ def convert(A: DeviceMatrix, fmt: int, stream=None) -> DeviceMatrix:
     if fmt == COO_FORMAT:
         return coo
     return from_coo(coo, fmt, stream)
+
+
+# ------------------------------------------------------------------ batched
+class Batch:
+    """Many device matrices x k right-hand sides in one persistent launch
+    (mph_batch_*).  X[i] is (ncols_i, k) and Y[i] (nrows_i, k), row-major."""
+
+    def __init__(self, mats, k, X, Y, stream=None):
+        self.mats, self.X, self.Y, self.k = list(mats), list(X), list(Y), int(k)
+        n = len(self.mats)
+        hs = (C.c_void_p * n)(*[m.handle.value for m in self.mats])
+        xs = (C.c_void_p * n)(*[x.data_ptr() for x in self.X])
+        ys = (C.c_void_p * n)(*[y.data_ptr() for y in self.Y])
+        self.handle = C.c_void_p()
+        capi.call("mph_batch_create", C.byref(self.handle), n, hs, self.k, xs, ys,
+                  stream_handle(stream))
+
+    def run(self, init=True, stream=None):
+        capi.call("mph_batch_run", self.handle, int(bool(init)), stream_handle(stream))
+
+    def run_graph(self, init=True, stream=None):
+        capi.call("mph_batch_run_graph", self.handle, int(bool(init)), stream_handle(stream))
+
+    def info(self):
+        nt, nw = C.c_int64(), C.c_int32()
+        capi.call("mph_batch_info", self.handle, C.byref(nt), C.byref(nw))
+        return int(nt.value), int(nw.value)
+
+    def close(self):
+        if self.handle:
+            capi.lib().mph_batch_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune_format(A: DeviceMatrix, k: int = 1, reps: int = 5, stream=None):
+    """Fastest format for the CSR matrix A with k right-hand sides, by timed
+    trials on the GPU; returns (format, {format: ms}) with inadmissible
+    formats omitted."""
+    assert A.format == CSR_FORMAT
+    best = C.c_int()
+    ms = (C.c_float * 6)()
+    a = A.arrays
+    capi.call("mph_autotune_format", A.nrows, A.ncols, A.nnz, _p(a["ro"]), _p(a["ci"]),
+              _p(a["v"]), A.vt, int(k), int(reps), stream_handle(stream), C.byref(best), ms)
+    return int(best.value), {f: float(ms[f]) for f in range(6) if ms[f] >= 0}
+
+
+def autotune_launch(A: DeviceMatrix, k: int = 1, reps: int = 5, stream=None) -> float:
+    """Tunes A's launch shape in place by timed trials; returns the best ms."""
+    x = torch.empty((A.ncols, k) if k > 1 else (A.ncols,), dtype=vector_dtype(A.dtype),
+                    device="cuda").uniform_(-1, 1)
+    y = torch.empty((A.nrows, k) if k > 1 else (A.nrows,), dtype=vector_dtype(A.dtype),
+                    device="cuda")
+    best = C.c_float()
+    capi.call("mph_matrix_autotune", A.handle, int(k), _p(x), _p(y), int(reps),
+              stream_handle(stream), C.byref(best))
+    return float(best.value)
