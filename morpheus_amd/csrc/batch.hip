@@ -11,13 +11,22 @@
 //    the matrix's row-block analysis, optionally with a fused DIA / ELL part
 //    for HDC / HYB), a whole long row (one workgroup, no partials), or a
 //    256-row slab of a DIA or ELL matrix;
-//  * tiles are sorted largest first on the host, and workgroups (grid = a few
-//    per CU) take the next tile from an atomic counter, so the uneven sizes
-//    balance dynamically; a one-thread kernel before the main kernel clears
-//    the counter, which keeps the pair capturable (no memset nodes);
+//  * tiles, sorted largest first on the host, are dealt into eight
+//    sub-queues (one per XCD); a workgroup takes `grab` tiles per counter
+//    update from its home sub-queue, then from the others, skipping drained
+//    ones with a relaxed load -- the queue balances the measured time of
+//    mixed batches (tiles of different formats cost differently per entry);
+//  * the alternative static schedule (MPH_BATCH_SCHEDULE=1) packs the tiles
+//    into one list per workgroup by longest-processing-time bin packing on an
+//    estimated cost and needs no atomics: it is faster for uniform batches,
+//    slower for the mixed benchmark batch;
+//  * the counters are cleared by a small kernel before the main kernel, so
+//    both forms stay capturable (no memset nodes);
 //  * X / Y are row-major with k columns: every matrix entry feeds k FMAs from
 //    one contiguous x row (16-byte vector loads when k*sizeof(x) allows).
 #include <algorithm>
+#include <cstdlib>
+#include <queue>
 #include <vector>
 
 #include "mph_internal.hpp"
@@ -54,6 +63,8 @@ struct BatchTile {
 
 constexpr int kBT = 256;
 constexpr int kSlab = 256;
+constexpr int kQueues = 8;  // one sub-queue per XCD
+constexpr int kGrab = 4;    // tiles taken per counter update
 
 template <typename V, int KR>
 struct XRow {
@@ -89,35 +100,117 @@ __device__ __forceinline__ void store_row(const BatchMat& m, int64_t r,
   for (int q = 0; q < KR; q++) y[q] = init ? (Vec)acc[q] : (Vec)((Acc)y[q] + acc[q]);
 }
 
+// Up to kU independent (value, column) pairs are loaded before their x rows
+// are gathered and consumed, so a thread keeps several memory round trips in
+// flight instead of one per diagonal / slot / item.
+// kU shrinks as k grows so that k accumulators plus kU x rows in flight stay
+// within a register budget that keeps 8 waves per SIMD
+template <int KR>
+struct Unroll {
+  static constexpr int value = KR >= 8 ? 2 : (KR >= 4 ? 4 : 8);
+};
+
+template <typename V, int KR>
+__device__ __forceinline__ void consume(typename vtraits<V>::acc* acc,
+                                        const typename vtraits<V>::acc* v, const int64_t* c,
+                                        const typename vtraits<V>::vec* __restrict__ x) {
+  constexpr int kU = Unroll<KR>::value;
+#pragma unroll
+  for (int u = 0; u < kU; u++)
+    if (c[u] >= 0) XRow<V, KR>::fma(acc, v[u], x, c[u]);
+}
+
+// DIA part of row r over diagonals d = d0, d0 + step, ...
+template <typename V, int KR>
+__device__ __forceinline__ void dia_row(const BatchMat& m, int64_t r, int d0, int step,
+                                        typename vtraits<V>::acc* acc) {
+  constexpr int kU = Unroll<KR>::value;
+  using Acc = typename vtraits<V>::acc;
+  using Vec = typename vtraits<V>::vec;
+  const V* dv = static_cast<const V*>(m.dval);
+  const Vec* x = static_cast<const Vec*>(m.x);
+  for (int d = d0; d < m.ndiag; d += kU * step) {
+    Acc v[kU];
+    int64_t c[kU];
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      const int dd = d + u * step;
+      c[u] = -1;
+      v[u] = 0;
+      if (dd < m.ndiag) {
+        const int64_t col = r + m.doff[dd];
+        if (col >= 0 && col < m.ncols) {
+          c[u] = col;
+          v[u] = vtraits<V>::widen(ld_stream(dv + r + m.dpitch * dd));
+        }
+      }
+    }
+    consume<V, KR>(acc, v, c, x);
+  }
+}
+
+// ELL part of row r over slots n = n0, n0 + step, ...
+template <typename V, int KR>
+__device__ __forceinline__ void ell_row(const BatchMat& m, int64_t r, int n0, int step,
+                                        typename vtraits<V>::acc* acc) {
+  constexpr int kU = Unroll<KR>::value;
+  using Acc = typename vtraits<V>::acc;
+  using Vec = typename vtraits<V>::vec;
+  const V* ev = static_cast<const V*>(m.evalp);
+  const Vec* x = static_cast<const Vec*>(m.x);
+  for (int n = n0; n < m.eloop; n += kU * step) {
+    int64_t c[kU];
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      const int nn = n + u * step;
+      c[u] = nn < m.eloop ? (int64_t)ld_stream(m.eci + r + m.epitch * nn) : -1;
+    }
+    Acc v[kU];
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      const int nn = n + u * step;
+      v[u] = c[u] >= 0 ? vtraits<V>::widen(ld_stream(ev + r + m.epitch * nn)) : (Acc)0;
+    }
+    consume<V, KR>(acc, v, c, x);
+  }
+}
+
 template <typename V, int KR>
 __device__ void row_items(const BatchMat& m, int64_t r, int lane, int L,
                           typename vtraits<V>::acc* acc) {
+  constexpr int kU = Unroll<KR>::value;
   using Acc = typename vtraits<V>::acc;
   using Vec = typename vtraits<V>::vec;
   const V* val = static_cast<const V*>(m.val);
   const Vec* x = static_cast<const Vec*>(m.x);
-  if (m.pre == BPRE_DIA) {
-    const V* dv = static_cast<const V*>(m.dval);
-    for (int d = lane; d < m.ndiag; d += L) {
-      const int64_t c = r + m.doff[d];
-      if (c >= 0 && c < m.ncols)
-        XRow<V, KR>::fma(acc, vtraits<V>::widen(ld_stream(dv + r + m.dpitch * d)), x, c);
-    }
-  } else if (m.pre == BPRE_ELL) {
-    const V* ev = static_cast<const V*>(m.evalp);
-    for (int n = lane; n < m.eloop; n += L) {
-      const int64_t slot = r + m.epitch * n;
-      const int32_t c = ld_stream(m.eci + slot);
-      if (c != -1) XRow<V, KR>::fma(acc, vtraits<V>::widen(ld_stream(ev + slot)), x, c);
-    }
-  }
+  if (m.pre == BPRE_DIA) dia_row<V, KR>(m, r, lane, L, acc);
+  else if (m.pre == BPRE_ELL) ell_row<V, KR>(m, r, lane, L, acc);
   const int64_t st = m.ro[r], en = m.ro[r + 1];
-  for (int64_t j = st + lane; j < en; j += L)
-    XRow<V, KR>::fma(acc, vtraits<V>::widen(ld_stream(val + j)), x, ld_stream(m.ci + j));
+  for (int64_t j = st + lane; j < en; j += (int64_t)kU * L) {
+    int64_t c[kU];
+    Acc v[kU];
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      const int64_t jj = j + (int64_t)u * L;
+      c[u] = jj < en ? (int64_t)ld_stream(m.ci + jj) : -1;
+      v[u] = jj < en ? vtraits<V>::widen(ld_stream(val + jj)) : (Acc)0;
+    }
+    consume<V, KR>(acc, v, c, x);
+  }
 }
 
+// A row of the tile longer than kDefer * L items is not left to its group of
+// L lanes (it would serialise the tile): it is queued in LDS and reduced by
+// the whole workgroup after the short rows.
+constexpr int kDefer = 8;
+constexpr int kMaxDeferred = 64;
+
 template <typename V, int KR>
-__device__ void tile_short(const BatchMat& m, const BatchTile& t, int init) {
+__device__ void block_row(const BatchMat& m, int64_t r, int init, typename vtraits<V>::acc* s_red);
+
+template <typename V, int KR>
+__device__ void tile_short(const BatchMat& m, const BatchTile& t, int init,
+                           typename vtraits<V>::acc* s_red, int32_t* s_def, int32_t* s_ndef) {
   using Acc = typename vtraits<V>::acc;
   const int nr = t.r1 - t.r0;
   const int64_t items = (int64_t)m.ro[t.r1] - m.ro[t.r0];
@@ -127,8 +220,25 @@ __device__ void tile_short(const BatchMat& m, const BatchTile& t, int init) {
   if (m.pre != BPRE_NONE && L < 4) L = 4;
   const int groups = kBT / L;
   const int g = threadIdx.x / L, l = threadIdx.x % L;
+  if (threadIdx.x == 0) *s_ndef = 0;
+  __syncthreads();
   for (int rr = g; rr < nr; rr += groups) {
     const int64_t r = t.r0 + rr;
+    const int64_t len = (int64_t)m.ro[r + 1] - m.ro[r];
+    if (len > (int64_t)kDefer * L && L < kWave) {
+      // the group's lane 0 queues the row; the list overflows into the
+      // group path, which stays correct
+      bool queued = false;
+      if (l == 0) {
+        const int slot = atomicAdd(s_ndef, 1);
+        if (slot < kMaxDeferred) {
+          s_def[slot] = (int32_t)r;
+          queued = true;
+        }
+      }
+      queued = __shfl(queued, (threadIdx.x % kWave) - l, kWave);
+      if (queued) continue;
+    }
     Acc acc[KR];
 #pragma unroll
     for (int q = 0; q < KR; q++) acc[q] = 0;
@@ -137,16 +247,23 @@ __device__ void tile_short(const BatchMat& m, const BatchTile& t, int init) {
     for (int q = 0; q < KR; q++) acc[q] = group_sum(acc[q], L);
     if (l == 0) store_row<V, KR>(m, r, acc, init);
   }
+  __syncthreads();
+  const int nd = min(*s_ndef, kMaxDeferred);
+  for (int i = 0; i < nd; i++) {
+    block_row<V, KR>(m, s_def[i], init, s_red);
+    __syncthreads();
+  }
 }
 
+// one row reduced by the whole workgroup
 template <typename V, int KR>
-__device__ void tile_wholerow(const BatchMat& m, const BatchTile& t, int init,
-                              typename vtraits<V>::acc* s_red) {
+__device__ void block_row(const BatchMat& m, int64_t r, int init,
+                          typename vtraits<V>::acc* s_red) {
   using Acc = typename vtraits<V>::acc;
   Acc acc[KR];
 #pragma unroll
   for (int q = 0; q < KR; q++) acc[q] = 0;
-  row_items<V, KR>(m, t.r0, threadIdx.x, kBT, acc);
+  row_items<V, KR>(m, r, threadIdx.x, kBT, acc);
   const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
 #pragma unroll
   for (int q = 0; q < KR; q++) {
@@ -161,73 +278,107 @@ __device__ void tile_wholerow(const BatchMat& m, const BatchTile& t, int init,
       tot[q] = 0;
       for (int w = 0; w < kBT / kWave; w++) tot[q] += s_red[q * (kBT / kWave) + w];
     }
-    store_row<V, KR>(m, t.r0, tot, init);
+    store_row<V, KR>(m, r, tot, init);
   }
+}
+
+template <typename V, int KR>
+__device__ void tile_wholerow(const BatchMat& m, const BatchTile& t, int init,
+                              typename vtraits<V>::acc* s_red) {
+  block_row<V, KR>(m, t.r0, init, s_red);
 }
 
 template <typename V, int KR>
 __device__ void tile_dia(const BatchMat& m, const BatchTile& t, int init) {
   using Acc = typename vtraits<V>::acc;
-  using Vec = typename vtraits<V>::vec;
   const int64_t r = t.r0 + threadIdx.x;
   if (r >= t.r1) return;
-  const V* dv = static_cast<const V*>(m.dval);
-  const Vec* x = static_cast<const Vec*>(m.x);
   Acc acc[KR];
 #pragma unroll
   for (int q = 0; q < KR; q++) acc[q] = 0;
-  for (int d = 0; d < m.ndiag; d++) {
-    const int64_t c = r + m.doff[d];
-    if (c >= 0 && c < m.ncols)
-      XRow<V, KR>::fma(acc, vtraits<V>::widen(ld_stream(dv + r + m.dpitch * d)), x, c);
-  }
+  dia_row<V, KR>(m, r, 0, 1, acc);
   store_row<V, KR>(m, r, acc, init);
 }
 
 template <typename V, int KR>
 __device__ void tile_ell(const BatchMat& m, const BatchTile& t, int init) {
   using Acc = typename vtraits<V>::acc;
-  using Vec = typename vtraits<V>::vec;
   const int64_t r = t.r0 + threadIdx.x;
   if (r >= t.r1) return;
-  const V* ev = static_cast<const V*>(m.evalp);
-  const Vec* x = static_cast<const Vec*>(m.x);
   Acc acc[KR];
 #pragma unroll
   for (int q = 0; q < KR; q++) acc[q] = 0;
-  for (int n = 0; n < m.eloop; n++) {
-    const int64_t slot = r + m.epitch * n;
-    const int32_t c = ld_stream(m.eci + slot);
-    if (c != -1) XRow<V, KR>::fma(acc, vtraits<V>::widen(ld_stream(ev + slot)), x, c);
-  }
+  ell_row<V, KR>(m, r, 0, 1, acc);
   store_row<V, KR>(m, r, acc, init);
 }
 
-__global__ void batch_reset_kernel(int32_t* counter) { *counter = 0; }
+__global__ void batch_reset_kernel(int32_t* counters) {
+  if (threadIdx.x < kQueues) counters[threadIdx.x] = 0;
+}
 
+template <typename V, int KR>
+__device__ __forceinline__ void run_tile(const BatchMat* __restrict__ mats,
+                                         const BatchTile& t, int init,
+                                         typename vtraits<V>::acc* s_red) {
+  __shared__ int32_t s_def[kMaxDeferred];
+  __shared__ int32_t s_ndef;
+  const BatchMat m = mats[t.mat];
+  switch (t.kind) {
+    case TILE_SHORT: tile_short<V, KR>(m, t, init, s_red, s_def, &s_ndef); break;
+    case TILE_WHOLEROW: tile_wholerow<V, KR>(m, t, init, s_red); break;
+    case TILE_DIA: tile_dia<V, KR>(m, t, init); break;
+    default: tile_ell<V, KR>(m, t, init); break;
+  }
+}
+
+// Static schedule: workgroup w walks its own slice woff[w] .. woff[w+1] of
+// the tile list, filled on the host by longest-processing-time bin packing
+// (each tile, largest first, goes to the workgroup with the least estimated
+// work so far).  No counters, no atomics.
+template <typename V, int KR>
+__global__ __launch_bounds__(kBT) void batch_spmm_static_kernel(
+    const BatchMat* __restrict__ mats, const BatchTile* __restrict__ tiles,
+    const int32_t* __restrict__ woff, int init) {
+  using Acc = typename vtraits<V>::acc;
+  __shared__ Acc s_red[KR * kBT / kWave];
+  const int32_t b = woff[blockIdx.x], e = woff[blockIdx.x + 1];
+  for (int32_t i = b; i < e; i++) {
+    run_tile<V, KR>(mats, tiles[i], init, s_red);
+    __syncthreads();
+  }
+}
+
+// qoff[q] .. qoff[q+1]: the tiles of sub-queue q.  A workgroup drains its
+// home sub-queue, then the others; a relaxed load skips a drained sub-queue
+// without a read-modify-write on its counter.
 template <typename V, int KR>
 __global__ __launch_bounds__(kBT) void batch_spmm_kernel(const BatchMat* __restrict__ mats,
                                                          const BatchTile* __restrict__ tiles,
-                                                         int32_t ntiles, int32_t* counter,
+                                                         const int32_t* __restrict__ qoff,
+                                                         int32_t* counters, int32_t grab,
                                                          int init) {
   using Acc = typename vtraits<V>::acc;
   __shared__ Acc s_red[KR * kBT / kWave];
-  __shared__ int32_t s_tile;
-  while (true) {
-    if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);
-    __syncthreads();
-    const int32_t ti = s_tile;
-    __syncthreads();
-    if (ti >= ntiles) break;  // every workgroup reaches this exit
-    const BatchTile t = tiles[ti];
-    const BatchMat m = mats[t.mat];
-    switch (t.kind) {
-      case TILE_SHORT: tile_short<V, KR>(m, t, init); break;
-      case TILE_WHOLEROW: tile_wholerow<V, KR>(m, t, init, s_red); break;
-      case TILE_DIA: tile_dia<V, KR>(m, t, init); break;
-      default: tile_ell<V, KR>(m, t, init); break;
+  __shared__ int32_t s_next;
+  const int home = blockIdx.x % kQueues;
+  for (int qi = 0; qi < kQueues; qi++) {
+    const int q = (home + qi) % kQueues;
+    const int32_t qb = qoff[q], qn = qoff[q + 1] - qoff[q];
+    while (true) {
+      if (threadIdx.x == 0) {
+        int32_t v = __hip_atomic_load(&counters[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_next = v >= qn ? qn : atomicAdd(&counters[q], grab);
+      }
+      __syncthreads();
+      const int32_t first = s_next;
+      __syncthreads();
+      if (first >= qn) break;  // this sub-queue is drained: try the next one
+      const int32_t last = min(first + grab, qn);
+      for (int32_t i = first; i < last; i++) {
+        run_tile<V, KR>(mats, tiles[qb + i], init, s_red);
+        __syncthreads();
+      }
     }
-    __syncthreads();
   }
 }
 
@@ -240,7 +391,12 @@ struct mph_batch_s {
   int32_t ntiles = 0;
   int32_t nwg = 0;
   mph::BatchMat* d_mats = nullptr;
-  mph::BatchTile* d_tiles = nullptr;
+  mph::BatchTile* d_tiles = nullptr;      // queue order (sub-queues)
+  mph::BatchTile* d_tiles_lpt = nullptr;  // per-workgroup lists (static)
+  int32_t* d_woff = nullptr;              // nwg + 1 list offsets
+  int32_t* d_qoff = nullptr;
+  int sched = 0;                          // 0 queues (default), 1 static lists
+  int32_t grab = 4;                       // tiles per counter update
   int32_t* d_counter = nullptr;
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t exec[2] = {nullptr, nullptr};
@@ -249,10 +405,13 @@ struct mph_batch_s {
 namespace mph {
 namespace {
 
+// estimated fixed cost of one tile, in matrix entries
+constexpr int64_t kTileOverhead = 512;
+
 int64_t tile_cost(const BatchTile& t, const std::vector<int32_t>* ro, const mph_matrix_impl* A) {
   if (t.kind == TILE_DIA) return (int64_t)(t.r1 - t.r0) * std::max<int64_t>(A->ndiag, 1);
   if (t.kind == TILE_ELL) return (int64_t)(t.r1 - t.r0) * std::max<int64_t>(A->ell_loop, 1);
-  int64_t c = (int64_t)(*ro)[t.r1] - (*ro)[t.r0];
+  int64_t c = (int64_t)(*ro)[t.r1] - (*ro)[t.r0] + (t.r1 - t.r0);
   if (A->format == MPH_HDC) c += (int64_t)(t.r1 - t.r0) * A->ndiag;
   if (A->format == MPH_HYB) c += (int64_t)(t.r1 - t.r0) * A->ell_loop;
   return c;
@@ -260,9 +419,14 @@ int64_t tile_cost(const BatchTile& t, const std::vector<int32_t>* ro, const mph_
 
 template <typename V, int KR>
 int launch_batch(mph_batch_s* b, int init, hipStream_t s) {
-  hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(1), 0, s, b->d_counter);
-  hipLaunchKernelGGL((batch_spmm_kernel<V, KR>), dim3((unsigned)b->nwg), dim3(kBT), 0, s,
-                     b->d_mats, b->d_tiles, b->ntiles, b->d_counter, init);
+  if (b->sched == 1) {
+    hipLaunchKernelGGL((batch_spmm_static_kernel<V, KR>), dim3((unsigned)b->nwg), dim3(kBT), 0,
+                       s, b->d_mats, b->d_tiles_lpt, b->d_woff, init);
+  } else {
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, b->d_counter);
+    hipLaunchKernelGGL((batch_spmm_kernel<V, KR>), dim3((unsigned)b->nwg), dim3(kBT), 0, s,
+                       b->d_mats, b->d_tiles, b->d_qoff, b->d_counter, b->grab, init);
+  }
   MPH_HIP_CHECK(hipGetLastError());
   return MPH_OK;
 }
@@ -374,8 +538,15 @@ int mph_batch_create(mph_batch_t* out, int32_t nmat, const mph_matrix_t* mats, i
   for (size_t q = 0; q < order.size(); q++) order[q] = q;
   std::stable_sort(order.begin(), order.end(),
                    [&](size_t a, size_t b) { return cost[a] > cost[b]; });
-  std::vector<BatchTile> sorted(tiles.size());
-  for (size_t q = 0; q < order.size(); q++) sorted[q] = tiles[order[q]];
+  // deal the sorted tiles round-robin into the sub-queues
+  std::vector<BatchTile> sorted;
+  sorted.reserve(tiles.size());
+  std::vector<int32_t> qoff(kQueues + 1, 0);
+  for (int q = 0; q < kQueues; q++) {
+    qoff[q] = (int32_t)sorted.size();
+    for (size_t j = q; j < order.size(); j += kQueues) sorted.push_back(tiles[order[j]]);
+  }
+  qoff[kQueues] = (int32_t)sorted.size();
 
   mph_batch_s* b = new mph_batch_s();
   b->vt = vt;
@@ -385,7 +556,45 @@ int mph_batch_create(mph_batch_t* out, int32_t nmat, const mph_matrix_t* mats, i
                                                            (int64_t)b->ntiles));
   hipError_t e = hipMalloc(&b->d_mats, sizeof(BatchMat) * std::max<int32_t>(nmat, 1));
   if (e == hipSuccess) e = hipMalloc(&b->d_tiles, sizeof(BatchTile) * std::max<int32_t>(b->ntiles, 1));
-  if (e == hipSuccess) e = hipMalloc(&b->d_counter, sizeof(int32_t));
+  // longest-processing-time packing into per-workgroup lists
+  std::vector<std::vector<int32_t>> lists(b->nwg);
+  {
+    std::priority_queue<std::pair<int64_t, int32_t>, std::vector<std::pair<int64_t, int32_t>>,
+                        std::greater<std::pair<int64_t, int32_t>>>
+        heap;
+    for (int32_t w = 0; w < b->nwg; w++) heap.push({0, w});
+    for (size_t q = 0; q < order.size(); q++) {
+      auto top = heap.top();
+      heap.pop();
+      lists[top.second].push_back((int32_t)order[q]);
+      heap.push({top.first + cost[order[q]] + kTileOverhead, top.second});
+    }
+  }
+  std::vector<BatchTile> packed;
+  packed.reserve(order.size());
+  std::vector<int32_t> woff(b->nwg + 1, 0);
+  for (int32_t w = 0; w < b->nwg; w++) {
+    woff[w] = (int32_t)packed.size();
+    for (int32_t id : lists[w]) packed.push_back(tiles[id]);
+  }
+  woff[b->nwg] = (int32_t)packed.size();
+  if (e == hipSuccess) e = hipMalloc(&b->d_tiles_lpt, sizeof(BatchTile) * std::max<int32_t>(b->ntiles, 1));
+  if (e == hipSuccess) e = hipMalloc(&b->d_woff, sizeof(int32_t) * (b->nwg + 1));
+  if (e == hipSuccess && b->ntiles > 0)
+    e = hipMemcpyAsync(b->d_tiles_lpt, packed.data(), sizeof(BatchTile) * b->ntiles,
+                       hipMemcpyHostToDevice, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(b->d_woff, woff.data(), sizeof(int32_t) * (b->nwg + 1),
+                       hipMemcpyHostToDevice, s);
+  if (const char* env = std::getenv("MPH_BATCH_SCHEDULE")) b->sched = std::atoi(env) == 1 ? 1 : 0;
+  // about four grabs per workgroup per sub-queue
+  b->grab = (int32_t)std::max<int64_t>(
+      1, std::min<int64_t>(16, (int64_t)b->ntiles / std::max<int64_t>(1, (int64_t)b->nwg * 4)));
+  if (e == hipSuccess) e = hipMalloc(&b->d_counter, sizeof(int32_t) * kQueues);
+  if (e == hipSuccess) e = hipMalloc(&b->d_qoff, sizeof(int32_t) * (kQueues + 1));
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(b->d_qoff, qoff.data(), sizeof(int32_t) * (kQueues + 1),
+                       hipMemcpyHostToDevice, s);
   if (e == hipSuccess && nmat > 0)
     e = hipMemcpyAsync(b->d_mats, hm.data(), sizeof(BatchMat) * nmat, hipMemcpyHostToDevice, s);
   if (e == hipSuccess && b->ntiles > 0)
@@ -409,6 +618,9 @@ int mph_batch_destroy(mph_batch_t b) {
   }
   dfree(b->d_mats);
   dfree(b->d_tiles);
+  dfree(b->d_tiles_lpt);
+  dfree(b->d_woff);
+  dfree(b->d_qoff);
   dfree(b->d_counter);
   delete b;
   return MPH_OK;

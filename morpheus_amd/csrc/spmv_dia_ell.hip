@@ -41,20 +41,31 @@ __global__ __launch_bounds__(256) void dia_kernel(DiaArgs<V> a) {
   for (int u = 0; u < RPT; u++)
 #pragma unroll
     for (int q = 0; q < KR; q++) t[u][q] = 0;
-#pragma unroll 4
-  for (int d = 0; d < a.ndiag; d++) {
-    const int64_t o = a.off[d];
-    const V* vd = a.val + a.pitch * d;
+  // batches of 8 diagonals: offsets, values and x entries of the batch are
+  // all requested before any of them is consumed
+  for (int d0 = 0; d0 < a.ndiag; d0 += 8) {
+    Acc v[RPT][8];
+    int64_t c[RPT][8];
 #pragma unroll
-    for (int u = 0; u < RPT; u++) {
-      const int64_t row = row0 + u * 256;
-      const int64_t col = row + o;
-      if (row < a.nrows && col >= 0 && col < a.ncols) {
-        const Acc v = vtraits<V>::widen(ld_stream(vd + row));
+    for (int w = 0; w < 8; w++) {
+      const int d = d0 + w;
+      const int64_t o = d < a.ndiag ? (int64_t)a.off[d] : 0;
 #pragma unroll
-        for (int q = 0; q < KR; q++) t[u][q] += v * (Acc)a.x[col * a.ld + q];
+      for (int u = 0; u < RPT; u++) {
+        const int64_t row = row0 + u * 256;
+        const int64_t col = row + o;
+        const bool ok = d < a.ndiag && row < a.nrows && col >= 0 && col < a.ncols;
+        c[u][w] = ok ? col : -1;
+        v[u][w] = ok ? vtraits<V>::widen(ld_stream(a.val + a.pitch * d + row)) : (Acc)0;
       }
     }
+#pragma unroll
+    for (int w = 0; w < 8; w++)
+#pragma unroll
+      for (int u = 0; u < RPT; u++)
+        if (c[u][w] >= 0)
+#pragma unroll
+          for (int q = 0; q < KR; q++) t[u][q] += v[u][w] * (Acc)a.x[c[u][w] * a.ld + q];
   }
 #pragma unroll
   for (int u = 0; u < RPT; u++) {
@@ -93,21 +104,32 @@ __global__ __launch_bounds__(256) void ell_kernel(EllArgs<V> a) {
   for (int u = 0; u < RPT; u++)
 #pragma unroll
     for (int q = 0; q < KR; q++) t[u][q] = 0;
-#pragma unroll 4
-  for (int n = 0; n < a.loop; n++) {
+  // batches of 8 slots: indices first, then values and x entries
+  for (int n0 = 0; n0 < a.loop; n0 += 8) {
+    int64_t c[RPT][8];
 #pragma unroll
-    for (int u = 0; u < RPT; u++) {
-      const int64_t row = row0 + u * 256;
-      if (row < a.nrows) {
-        const int64_t slot = row + a.pitch * n;
-        const int32_t c = ld_stream(a.ci + slot);
-        if (c != -1) {
-          const Acc v = vtraits<V>::widen(ld_stream(a.val + slot));
+    for (int w = 0; w < 8; w++)
 #pragma unroll
-          for (int q = 0; q < KR; q++) t[u][q] += v * (Acc)a.x[(int64_t)c * a.ld + q];
-        }
+      for (int u = 0; u < RPT; u++) {
+        const int64_t row = row0 + u * 256;
+        const int n = n0 + w;
+        c[u][w] = (n < a.loop && row < a.nrows) ? (int64_t)ld_stream(a.ci + row + a.pitch * n) : -1;
       }
-    }
+    Acc v[RPT][8];
+#pragma unroll
+    for (int w = 0; w < 8; w++)
+#pragma unroll
+      for (int u = 0; u < RPT; u++) {
+        const int64_t row = row0 + u * 256;
+        v[u][w] = c[u][w] >= 0 ? vtraits<V>::widen(ld_stream(a.val + row + a.pitch * (n0 + w))) : (Acc)0;
+      }
+#pragma unroll
+    for (int w = 0; w < 8; w++)
+#pragma unroll
+      for (int u = 0; u < RPT; u++)
+        if (c[u][w] >= 0)
+#pragma unroll
+          for (int q = 0; q < KR; q++) t[u][q] += v[u][w] * (Acc)a.x[c[u][w] * a.ld + q];
   }
 #pragma unroll
   for (int u = 0; u < RPT; u++) {
