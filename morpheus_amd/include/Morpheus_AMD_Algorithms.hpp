@@ -1,0 +1,716 @@
+// Morpheus_AMD_Algorithms.hpp -- Morpheus::copy, create_mirror[_container],
+// multiply and convert of the MI355X drop-in, over the C-ABI of
+// include/morpheus_amd.h.
+//
+// Reference front-ends:
+//   copy                 core/src/Morpheus_Copy.hpp:51-80
+//   create_mirror[...]   core/src/Morpheus_MirrorContainers.hpp:62-192
+//   multiply             core/src/Morpheus_Multiply.hpp:57-78
+//   convert              core/src/Morpheus_Convert.hpp:54-57, hub
+//                        core/src/impl/Morpheus_Convert_Impl.hpp:83-106
+//   convert (in place)   core/src/Morpheus_Convert.hpp:88-141
+//   dynamic dispatch     core/src/impl/Dynamic/Morpheus_{Multiply,Convert,Copy}_Impl.hpp
+//
+// Execution spaces.  Morpheus::HIP and Morpheus::Generic::HIP run the
+// MI355X kernels on HIPSpace containers.  Morpheus::Serial and
+// Morpheus::OpenMP on HostSpace containers run the same kernels: the matrix
+// is staged into HBM once (kept with the container's analysis until it is
+// modified) and x / y travel per call.  Conversion on the device succeeds for
+// every format pair (the reference HIP converters throw) and is bit-exact
+// with the reference Serial converters.
+//
+// All device work is ordered on the default stream, as in the reference.
+#pragma once
+
+#include "Morpheus_AMD_Containers.hpp"
+
+namespace Morpheus {
+
+namespace Impl {
+
+template <class T>
+inline const void* vp(const T* p) {
+  return static_cast<const void*>(p);
+}
+
+// ------------------------------------------------------------ copy kernels
+template <class A, class B>
+inline void copy_array(const A& src, B& dst, const char* what) {
+  using SM = typename A::memory_space;
+  using DM = typename B::memory_space;
+  static_assert(sizeof(typename A::value_type) == sizeof(typename B::value_type),
+                "Morpheus::copy between different value types");
+  if (src.size() != dst.size())
+    throw RuntimeException(std::string("Morpheus::copy: ") + what + " sizes differ (" +
+                           std::to_string(src.size()) + " != " + std::to_string(dst.size()) +
+                           ")");
+  A::copy_bytes(dst.data(), src.data(), src.size() * sizeof(typename A::value_type),
+                DM::is_device, SM::is_device);
+}
+
+// DenseMatrix deep copy; transposes through host memory when the storage
+// layouts differ.
+template <class S, class D>
+inline void copy_dense_matrix(const S& src, D& dst) {
+  if (src.nrows() != dst.nrows() || src.ncols() != dst.ncols())
+    throw RuntimeException("Morpheus::copy: DenseMatrix shapes differ");
+  using SM = typename S::memory_space;
+  using DM = typename D::memory_space;
+  using T = typename S::non_const_value_type;
+  if (S::is_left == D::is_left || src.nrows() <= 1 || src.ncols() <= 1) {
+    copy_array(src.const_view(), dst.view(), "DenseMatrix");
+    return;
+  }
+  const size_t m = src.nrows(), n = src.ncols();
+  std::vector<T> a(m * n), b(m * n);
+  S::value_array_type::copy_bytes(a.data(), src.data(), a.size() * sizeof(T), false, SM::is_device);
+  for (size_t i = 0; i < m; i++)
+    for (size_t j = 0; j < n; j++)
+      b[D::is_left ? i + j * m : i * n + j] = a[S::is_left ? i + j * m : i * n + j];
+  S::value_array_type::copy_bytes(dst.data(), b.data(), b.size() * sizeof(T), DM::is_device, false);
+}
+
+template <class S, class D>
+inline void check_shape(const S& src, const D& dst) {
+  if (dst.nrows() != src.nrows() || dst.ncols() != src.ncols())
+    throw RuntimeException("Morpheus::copy: destination matrix must have equal shape to the source");
+  if (dst.nnnz() != src.nnnz())
+    throw RuntimeException(
+        "Morpheus::copy: destination matrix must have equal number of non-zeros to the source");
+}
+
+template <class S, class D>
+inline void copy_sparse(const S& src, D& dst) {
+  static_assert(S::format_id == D::format_id,
+                "Morpheus::copy is only available between containers of the same format");
+  check_shape(src, dst);
+  if constexpr (S::format_id == COO_FORMAT) {
+    copy_array(src.crow_indices().const_view(), dst.row_indices().view(), "row_indices");
+    copy_array(src.ccolumn_indices().const_view(), dst.column_indices().view(), "column_indices");
+    copy_array(src.cvalues().const_view(), dst.values().view(), "values");
+  } else if constexpr (S::format_id == CSR_FORMAT) {
+    copy_array(src.crow_offsets().const_view(), dst.row_offsets().view(), "row_offsets");
+    copy_array(src.ccolumn_indices().const_view(), dst.column_indices().view(), "column_indices");
+    copy_array(src.cvalues().const_view(), dst.values().view(), "values");
+  } else if constexpr (S::format_id == DIA_FORMAT) {
+    copy_array(src.cdiagonal_offsets().const_view(), dst.diagonal_offsets().view(),
+               "diagonal_offsets");
+    copy_dense_matrix(src.cvalues(), dst.values());
+  } else if constexpr (S::format_id == ELL_FORMAT) {
+    copy_dense_matrix(src.ccolumn_indices(), dst.column_indices());
+    copy_dense_matrix(src.cvalues(), dst.values());
+  } else if constexpr (S::format_id == HYB_FORMAT) {
+    copy_sparse(src.cell(), dst.ell());
+    copy_sparse(src.ccoo(), dst.coo());
+  } else {
+    copy_sparse(src.cdia(), dst.dia());
+    copy_sparse(src.ccsr(), dst.csr());
+  }
+  dst.touch();
+}
+
+// ------------------------------------------------------------ handles
+template <class M>
+inline std::vector<const void*> signature(const M& A) {
+  std::vector<const void*> s{reinterpret_cast<const void*>(A.nrows()),
+                             reinterpret_cast<const void*>(A.ncols()),
+                             reinterpret_cast<const void*>(A.nnnz())};
+  if constexpr (M::format_id == COO_FORMAT) {
+    s.insert(s.end(), {vp(A.crow_indices().data()), vp(A.ccolumn_indices().data()),
+                       vp(A.cvalues().data())});
+  } else if constexpr (M::format_id == CSR_FORMAT) {
+    s.insert(s.end(), {vp(A.crow_offsets().data()), vp(A.ccolumn_indices().data()),
+                       vp(A.cvalues().data())});
+  } else if constexpr (M::format_id == DIA_FORMAT) {
+    s.insert(s.end(), {vp(A.cdiagonal_offsets().data()), vp(A.cvalues().data()),
+                       reinterpret_cast<const void*>(A.cvalues().nrows()),
+                       reinterpret_cast<const void*>(A.cdiagonal_offsets().size())});
+  } else if constexpr (M::format_id == ELL_FORMAT) {
+    s.insert(s.end(), {vp(A.ccolumn_indices().data()), vp(A.cvalues().data()),
+                       reinterpret_cast<const void*>(A.cvalues().ncols()),
+                       reinterpret_cast<const void*>(A.entries_per_row())});
+  } else if constexpr (M::format_id == HYB_FORMAT) {
+    auto a = signature(A.cell()), b = signature(A.ccoo());
+    s.insert(s.end(), a.begin(), a.end());
+    s.insert(s.end(), b.begin(), b.end());
+  } else {
+    auto a = signature(A.cdia()), b = signature(A.ccsr());
+    s.insert(s.end(), a.begin(), a.end());
+    s.insert(s.end(), b.begin(), b.end());
+  }
+  return s;
+}
+
+// C-ABI handle over the arrays of a HIPSpace container.
+template <class M>
+inline mph_matrix_t create_handle(const M& A) {
+  static_assert(M::memory_space::is_device, "handle over host arrays");
+  constexpr int vt = value_code<typename M::non_const_value_type>::value;
+  const int64_t nr = (int64_t)A.nrows(), nc = (int64_t)A.ncols();
+  mph_matrix_t h = nullptr;
+  int st = MPH_OK;
+  if constexpr (M::format_id == COO_FORMAT) {
+    st = mph_matrix_create_coo(&h, nr, nc, (int64_t)A.nnnz(), A.crow_indices().data(),
+                               A.ccolumn_indices().data(), A.cvalues().data(), vt);
+  } else if constexpr (M::format_id == CSR_FORMAT) {
+    st = mph_matrix_create_csr(&h, nr, nc, (int64_t)A.nnnz(), A.crow_offsets().data(),
+                               A.ccolumn_indices().data(), A.cvalues().data(), vt);
+  } else if constexpr (M::format_id == DIA_FORMAT) {
+    st = mph_matrix_create_dia(&h, nr, nc, (int64_t)A.nnnz(),
+                               (int64_t)A.cdiagonal_offsets().size(), (int64_t)A.cvalues().nrows(),
+                               A.cdiagonal_offsets().data(), A.cvalues().data(), vt);
+  } else if constexpr (M::format_id == ELL_FORMAT) {
+    st = mph_matrix_create_ell(&h, nr, nc, (int64_t)A.nnnz(), (int64_t)A.entries_per_row(),
+                               (int64_t)A.cvalues().ncols(), (int64_t)A.cvalues().nrows(),
+                               A.ccolumn_indices().data(), A.cvalues().data(), vt);
+  } else if constexpr (M::format_id == HYB_FORMAT) {
+    const auto& e = A.cell();
+    const auto& c = A.ccoo();
+    st = mph_matrix_create_hyb(&h, nr, nc, (int64_t)e.nnnz(), (int64_t)e.entries_per_row(),
+                               (int64_t)e.cvalues().ncols(), (int64_t)e.cvalues().nrows(),
+                               e.ccolumn_indices().data(), e.cvalues().data(), (int64_t)c.nnnz(),
+                               c.crow_indices().data(), c.ccolumn_indices().data(),
+                               c.cvalues().data(), vt);
+  } else {
+    const auto& d = A.cdia();
+    const auto& c = A.ccsr();
+    st = mph_matrix_create_hdc(&h, nr, nc, (int64_t)d.nnnz(), (int64_t)d.cdiagonal_offsets().size(),
+                               (int64_t)d.cvalues().nrows(), d.cdiagonal_offsets().data(),
+                               d.cvalues().data(), (int64_t)c.nnnz(), c.crow_offsets().data(),
+                               c.ccolumn_indices().data(), c.cvalues().data(), vt);
+  }
+  check(st, "Morpheus::multiply");
+  return h;
+}
+
+template <class M>
+using device_type_of = typename M::template rebind<typename M::non_const_value_type,
+                                                   typename M::non_const_index_type,
+                                                   typename M::array_layout, HIPSpace>;
+
+// Handle of a concrete sparse container, built (or rebuilt after a change)
+// on demand.  Host containers are staged into HBM first.
+template <class M>
+inline mph_matrix_t matrix_handle(const M& A) {
+  PlanCache& pc = A.plan_cache();
+  auto sig = signature(A);
+  if (pc.handle && !pc.stale && pc.signature == sig) return pc.handle;
+  pc.reset();
+  if constexpr (M::memory_space::is_device) {
+    pc.handle = create_handle(A);
+  } else {
+    using DM = device_type_of<M>;
+    auto dev = std::make_shared<DM>();
+    dev->allocate(A);
+    copy_sparse(A, *dev);
+    pc.handle = create_handle(*dev);
+    pc.device_copy = dev;
+  }
+  pc.signature = std::move(sig);
+  pc.stale = false;
+  return pc.handle;
+}
+
+// ------------------------------------------------------------ multiply
+template <class M, class VX, class VY>
+inline void multiply_concrete(const M& A, const VX& x, VY& y, bool init) {
+  using V = typename M::non_const_value_type;
+  static_assert(std::is_same_v<typename VX::non_const_value_type, V> &&
+                    std::is_same_v<typename VY::non_const_value_type, V>,
+                "Morpheus::multiply: matrix and vectors must hold the same value type");
+  if (x.size() != A.ncols() || y.size() != A.nrows())
+    throw InvalidInputException("Morpheus::multiply: x must have ncols entries and y nrows entries (" +
+                                std::to_string(A.nrows()) + "x" + std::to_string(A.ncols()) +
+                                " matrix, x " + std::to_string(x.size()) + ", y " +
+                                std::to_string(y.size()) + ")");
+  mph_matrix_t h = matrix_handle(A);
+  if constexpr (M::memory_space::is_device) {
+    check(mph_multiply(h, x.data(), y.data(), init ? 1 : 0, nullptr), "Morpheus::multiply");
+  } else {
+    DenseVector<V, HIPSpace> xd(x.size()), yd(y.size());
+    copy_array(x.const_view(), xd.view(), "x");
+    if (!init) copy_array(y.const_view(), yd.view(), "y");
+    check(mph_multiply(h, xd.data(), yd.data(), init ? 1 : 0, nullptr), "Morpheus::multiply");
+    copy_array(yd.const_view(), y.view(), "y");
+  }
+}
+
+template <class Exec, class M>
+constexpr void check_access() {
+  static_assert(Exec::is_device == M::memory_space::is_device,
+                "the execution space cannot access the container's memory space");
+}
+
+// ------------------------------------------------------------ convert
+inline int64_t i64(size_t v) { return (int64_t)v; }
+
+template <class S, class D>
+inline void convert_device(const S& src, D& dst);
+
+// COO -> every format (core/src/impl/<Fmt>/Serial/Morpheus_Convert_Impl.hpp)
+template <class S, class D>
+inline void convert_from_coo(const S& src, D& dst) {
+  constexpr int vt = value_code<typename S::non_const_value_type>::value;
+  const int64_t n = i64(src.nrows()), m = i64(src.ncols()), nnz = i64(src.nnnz());
+  const int* ri = src.crow_indices().data();
+  const int* ci = src.ccolumn_indices().data();
+  const void* v = src.cvalues().data();
+  if constexpr (D::format_id == CSR_FORMAT) {
+    dst.resize(n, m, nnz);
+    check(mph_convert_coo_to_csr(n, nnz, ri, ci, v, vt, dst.row_offsets().data(),
+                                 dst.column_indices().data(), dst.values().data(), nullptr),
+          "Morpheus::convert");
+  } else if constexpr (D::format_id == DIA_FORMAT) {
+    int64_t nd = 0;
+    check(mph_convert_coo_to_dia_size(n, m, nnz, ri, ci, &nd, nullptr), "Morpheus::convert");
+    dst.resize(n, m, nnz, nd, 32);
+    check(mph_convert_coo_to_dia(n, m, nnz, ri, ci, v, vt, nd, i64(dst.values().nrows()),
+                                 dst.diagonal_offsets().data(), dst.values().data(), nullptr),
+          "Morpheus::convert");
+  } else if constexpr (D::format_id == ELL_FORMAT) {
+    int64_t K = 0;
+    check(mph_convert_coo_to_ell_size(n, nnz, ri, &K, nullptr), "Morpheus::convert");
+    dst.resize(n, m, nnz, K, 32);
+    check(mph_convert_coo_to_ell(n, nnz, ri, ci, v, vt, K, i64(dst.values().ncols()),
+                                 i64(dst.values().nrows()), dst.column_indices().data(),
+                                 dst.values().data(), nullptr),
+          "Morpheus::convert");
+  } else if constexpr (D::format_id == HYB_FORMAT) {
+    int64_t K = 0, enz = 0, cnz = 0;
+    check(mph_convert_coo_to_hyb_size(n, nnz, ri, &K, &enz, &cnz, nullptr), "Morpheus::convert");
+    dst.resize(n, m, enz, cnz, K, 32);
+    auto& e = dst.ell();
+    auto& c = dst.coo();
+    check(mph_convert_coo_to_hyb(n, nnz, ri, ci, v, vt, K, i64(e.values().ncols()),
+                                 i64(e.values().nrows()), e.column_indices().data(),
+                                 e.values().data(), c.row_indices().data(),
+                                 c.column_indices().data(), c.values().data(), nullptr),
+          "Morpheus::convert");
+  } else if constexpr (D::format_id == HDC_FORMAT) {
+    int64_t nd = 0, dnz = 0, cnz = 0;
+    check(mph_convert_coo_to_hdc_size(n, m, nnz, ri, ci, &nd, &dnz, &cnz, nullptr),
+          "Morpheus::convert");
+    dst.resize(n, m, dnz, cnz, nd, 32);
+    auto& d = dst.dia();
+    auto& c = dst.csr();
+    check(mph_convert_coo_to_hdc(n, m, nnz, ri, ci, v, vt, nd, i64(d.values().nrows()),
+                                 d.diagonal_offsets().data(), d.values().data(),
+                                 c.row_offsets().data(), c.column_indices().data(),
+                                 c.values().data(), nullptr),
+          "Morpheus::convert");
+  }
+  dst.touch();
+}
+
+// every format -> COO
+template <class S, class D>
+inline void convert_to_coo(const S& src, D& dst) {
+  constexpr int vt = value_code<typename S::non_const_value_type>::value;
+  const int64_t n = i64(src.nrows()), m = i64(src.ncols()), nnz = i64(src.nnnz());
+  dst.resize(n, m, nnz);
+  int* ri = dst.row_indices().data();
+  int* ci = dst.column_indices().data();
+  void* v = dst.values().data();
+  int st = MPH_OK;
+  if constexpr (S::format_id == CSR_FORMAT) {
+    st = mph_convert_csr_to_coo(n, nnz, src.crow_offsets().data(), src.ccolumn_indices().data(),
+                                src.cvalues().data(), vt, ri, ci, v, nullptr);
+  } else if constexpr (S::format_id == DIA_FORMAT) {
+    st = mph_convert_dia_to_coo(n, m, nnz, i64(src.cdiagonal_offsets().size()),
+                                i64(src.cvalues().nrows()), src.cdiagonal_offsets().data(),
+                                src.cvalues().data(), vt, ri, ci, v, nullptr);
+  } else if constexpr (S::format_id == ELL_FORMAT) {
+    st = mph_convert_ell_to_coo(n, nnz, i64(src.cvalues().ncols()), i64(src.cvalues().nrows()),
+                                src.ccolumn_indices().data(), src.cvalues().data(), vt, ri, ci, v,
+                                nullptr);
+  } else if constexpr (S::format_id == HYB_FORMAT) {
+    const auto& e = src.cell();
+    const auto& c = src.ccoo();
+    st = mph_convert_hyb_to_coo(n, nnz, i64(e.cvalues().ncols()), i64(e.cvalues().nrows()),
+                                e.ccolumn_indices().data(), e.cvalues().data(), i64(c.nnnz()),
+                                c.crow_indices().data(), c.ccolumn_indices().data(),
+                                c.cvalues().data(), vt, ri, ci, v, nullptr);
+  } else if constexpr (S::format_id == HDC_FORMAT) {
+    const auto& d = src.cdia();
+    const auto& c = src.ccsr();
+    st = mph_convert_hdc_to_coo(n, m, nnz, i64(d.cdiagonal_offsets().size()),
+                                i64(d.cvalues().nrows()), d.cdiagonal_offsets().data(),
+                                d.cvalues().data(), i64(c.nnnz()), c.crow_offsets().data(),
+                                c.ccolumn_indices().data(), c.cvalues().data(), vt, ri, ci, v,
+                                nullptr);
+  }
+  check(st, "Morpheus::convert");
+  dst.touch();
+}
+
+// Same-format convert is a deep copy into freshly sized storage
+// (core/src/impl/Csr/Serial/Morpheus_Convert_Impl.hpp:38-64).
+template <class S, class D>
+inline void convert_device(const S& src, D& dst) {
+  static_assert(std::is_same_v<typename S::non_const_value_type, typename D::non_const_value_type>,
+                "Morpheus::convert between value types is not supported");
+  if constexpr (S::format_id == D::format_id) {
+    dst.allocate(src);
+    copy_sparse(src, dst);
+  } else if constexpr (S::format_id == COO_FORMAT) {
+    convert_from_coo(src, dst);
+  } else if constexpr (D::format_id == COO_FORMAT) {
+    convert_to_coo(src, dst);
+  } else {
+    // core/src/impl/Morpheus_Convert_Impl.hpp:83-106: through a temporary COO
+    CooMatrix<typename S::non_const_value_type, typename S::non_const_index_type,
+              typename S::array_layout, HIPSpace>
+        tmp;
+    convert_to_coo(src, tmp);
+    convert_from_coo(tmp, dst);
+  }
+}
+
+template <class S, class D>
+inline void convert_concrete(const S& src, D& dst) {
+  if constexpr (S::memory_space::is_device) {
+    convert_device(src, dst);
+  } else {
+    // host containers: the conversion runs on the device and the result
+    // comes back into dst
+    device_type_of<S> sd;
+    sd.allocate(src);
+    copy_sparse(src, sd);
+    device_type_of<D> dd;
+    convert_device(sd, dd);
+    dst.allocate(dd);
+    copy_sparse(dd, dst);
+  }
+}
+
+}  // namespace Impl
+
+// ============================================================ copy
+template <class V1, class... P1, class V2, class... P2>
+void copy(const DenseVector<V1, P1...>& src, DenseVector<V2, P2...>& dst) {
+  Impl::copy_array(src.const_view(), dst.view(), "DenseVector");
+}
+template <class V1, class... P1, class V2, class... P2>
+void copy(const DenseMatrix<V1, P1...>& src, DenseMatrix<V2, P2...>& dst) {
+  Impl::copy_dense_matrix(src, dst);
+}
+// sliced DenseVector copy (core/src/Morpheus_Copy.hpp:70-80)
+template <class V1, class... P1, class V2, class... P2>
+void copy(const DenseVector<V1, P1...>& src, DenseVector<V2, P2...>& dst, const int src_begin,
+          const int src_end, const int dst_begin, const int dst_end) {
+  using SM = typename DenseVector<V1, P1...>::memory_space;
+  using DM = typename DenseVector<V2, P2...>::memory_space;
+  if (src_end - src_begin != dst_end - dst_begin || src_begin < 0 || dst_begin < 0 ||
+      src_end < src_begin || (size_t)src_end > src.size() || (size_t)dst_end > dst.size())
+    throw RuntimeException("Morpheus::copy: invalid slice ranges");
+  Impl::Array<V1, SM>::copy_bytes(dst.data() + dst_begin, src.data() + src_begin,
+                                  (size_t)(src_end - src_begin) * sizeof(V1), DM::is_device,
+                                  SM::is_device);
+}
+
+template <class S, class D>
+std::enable_if_t<is_sparse_matrix_container_v<S> && is_sparse_matrix_container_v<D>> copy(
+    const S& src, D& dst) {
+  Impl::copy_sparse(src, dst);
+}
+
+// DynamicMatrix copies: the active formats must agree
+// (core/src/impl/Dynamic/Morpheus_Copy_Impl.hpp:30-120).
+template <class S, class D>
+std::enable_if_t<(is_dynamic_matrix_format_container_v<S> ||
+                  is_dynamic_matrix_format_container_v<D>) &&
+                 !is_dense_vector_format_container_v<S>>
+copy(const S& src, D& dst) {
+  if (src.format_index() != dst.format_index())
+    throw FormatConversionException(
+        "Morpheus::copy() is only available between the same container types (" +
+        std::to_string(src.format_index()) + " != " + std::to_string(dst.format_index()) +
+        "). Please use Morpheus::convert() instead to perform conversions between different "
+        "types.");
+  auto one = [&](const auto& s) {
+    using SC = std::decay_t<decltype(s)>;
+    if constexpr (is_dynamic_matrix_format_container_v<D>) {
+      std::visit(
+          [&](auto& d) {
+            using DC = std::decay_t<decltype(d)>;
+            if constexpr (SC::format_id == DC::format_id) Impl::copy_sparse(s, d);
+          },
+          dst.formats());
+    } else {
+      if constexpr (SC::format_id == D::format_id) Impl::copy_sparse(s, dst);
+    }
+  };
+  if constexpr (is_dynamic_matrix_format_container_v<S>)
+    std::visit(one, src.const_formats());
+  else
+    one(src);
+}
+
+// ============================================================ mirrors
+template <template <class, class...> class C, class T, class... P>
+typename C<T, P...>::HostMirror create_mirror(const C<T, P...>& src) {
+  typename C<T, P...>::HostMirror m;
+  m.allocate(src);
+  return m;
+}
+
+template <class Space, template <class, class...> class C, class T, class... P>
+auto create_mirror(const C<T, P...>& src) {
+  using S = C<T, P...>;
+  using M = C<typename S::non_const_value_type, typename S::non_const_index_type,
+              typename S::array_layout, typename memory_space_of<Space>::type>;
+  M m;
+  m.allocate(src);
+  return m;
+}
+
+template <template <class, class...> class C, class T, class... P>
+typename C<T, P...>::HostMirror create_mirror_container(const C<T, P...>& src) {
+  using S = C<T, P...>;
+  if constexpr (!S::memory_space::is_device &&
+                std::is_same_v<typename S::HostMirror::array_layout, typename S::array_layout>) {
+    return typename S::HostMirror(src);
+  } else {
+    return create_mirror(src);
+  }
+}
+
+template <class Space, template <class, class...> class C, class T, class... P>
+auto create_mirror_container(const C<T, P...>& src) {
+  using S = C<T, P...>;
+  using Mem = typename memory_space_of<Space>::type;
+  using M = C<typename S::non_const_value_type, typename S::non_const_index_type,
+              typename S::array_layout, Mem>;
+  if constexpr (std::is_same_v<Mem, typename S::memory_space>) {
+    return M(src);
+  } else {
+    M m;
+    m.allocate(src);
+    return m;
+  }
+}
+
+// ============================================================ multiply
+// y = A x (init) or y += A x (core/src/Morpheus_Multiply.hpp:57-78)
+template <typename ExecSpace, typename Matrix, typename VectorIn, typename VectorOut>
+void multiply(const Matrix& A, const VectorIn& x, VectorOut& y, const bool init) {
+  static_assert(is_dense_vector_format_container_v<VectorIn> &&
+                    is_dense_vector_format_container_v<VectorOut>,
+                "Morpheus::multiply: x and y must be DenseVector containers");
+  if constexpr (is_dynamic_matrix_format_container_v<Matrix>) {
+    std::visit(
+        [&](const auto& a) {
+          using M = std::decay_t<decltype(a)>;
+          Impl::check_access<ExecSpace, M>();
+          Impl::multiply_concrete(a, x, y, init);
+        },
+        A.const_formats());
+  } else {
+    Impl::check_access<ExecSpace, Matrix>();
+    Impl::multiply_concrete(A, x, y, init);
+  }
+}
+template <typename ExecSpace, typename Matrix, typename VectorIn, typename VectorOut>
+void multiply(const Matrix& A, const VectorIn& x, VectorOut& y) {
+  multiply<ExecSpace>(A, x, y, true);
+}
+
+// ============================================================ convert
+template <typename ExecSpace, typename SourceType, typename DestinationType>
+std::enable_if_t<!std::is_same_v<DestinationType, formats_e> && !std::is_same_v<DestinationType, int>>
+convert(const SourceType& src, DestinationType& dst) {
+  constexpr bool sdyn = is_dynamic_matrix_format_container_v<SourceType>;
+  constexpr bool ddyn = is_dynamic_matrix_format_container_v<DestinationType>;
+  if constexpr (sdyn && ddyn) {
+    // core/src/impl/Dynamic/Morpheus_Convert_Impl.hpp:71-83
+    dst.set_nrows(src.nrows());
+    dst.set_ncols(src.ncols());
+    dst.set_nnnz(src.nnnz());
+    std::visit(
+        [&](const auto& s, auto& d) {
+          Impl::check_access<ExecSpace, std::decay_t<decltype(s)>>();
+          Impl::convert_concrete(s, d);
+        },
+        src.const_formats(), dst.formats());
+  } else if constexpr (sdyn) {
+    std::visit(
+        [&](const auto& s) {
+          Impl::check_access<ExecSpace, std::decay_t<decltype(s)>>();
+          Impl::convert_concrete(s, dst);
+        },
+        src.const_formats());
+  } else if constexpr (ddyn) {
+    dst.set_nrows(src.nrows());
+    dst.set_ncols(src.ncols());
+    dst.set_nnnz(src.nnnz());
+    Impl::check_access<ExecSpace, SourceType>();
+    std::visit([&](auto& d) { Impl::convert_concrete(src, d); }, dst.formats());
+  } else {
+    Impl::check_access<ExecSpace, SourceType>();
+    Impl::check_access<ExecSpace, DestinationType>();
+    Impl::convert_concrete(src, dst);
+  }
+}
+
+// In-place switch of a DynamicMatrix (core/src/Morpheus_Convert.hpp:88-141):
+// through a temporary COO; failures are reported, not thrown.
+template <typename ExecSpace, typename SourceType>
+conversion_error_e convert(SourceType& src, const formats_e index) {
+  static_assert(is_dynamic_matrix_format_container_v<SourceType>,
+                "Container must be a DynamicMatrix.");
+  using Coo = std::variant_alternative_t<0, typename SourceType::variant_type>;
+  if (src.active_index() == static_cast<int>(index)) return CONV_SUCCESS;
+  Coo temp;
+  if (src.active_index() != COO_FORMAT) {
+    try {
+      Morpheus::convert<ExecSpace>(src, temp);
+    } catch (...) {
+      return DYNAMIC_TO_PROXY;
+    }
+  } else {
+    temp = std::get<0>(src.const_formats());
+  }
+  SourceType dynamic_temp;
+  dynamic_temp.activate(index);
+  try {
+    Morpheus::convert<ExecSpace>(temp, dynamic_temp);
+  } catch (...) {
+    return PROXY_TO_DYNAMIC;
+  }
+  src = dynamic_temp;
+  return CONV_SUCCESS;
+}
+template <typename ExecSpace, typename SourceType>
+conversion_error_e convert(SourceType& src, const int index) {
+  return Morpheus::convert<ExecSpace>(src, static_cast<formats_e>(index));
+}
+
+// ============================================================ sort / analytics
+namespace Impl {
+// Runs f on a HIPSpace view of a COO container (the container itself, or a
+// staged device copy whose contents are copied back when write_back).
+template <class M, class F>
+inline void on_device_coo(M& A, bool write_back, F&& f) {
+  if constexpr (M::memory_space::is_device) {
+    f(A);
+  } else {
+    device_type_of<M> d;
+    d.allocate(A);
+    copy_sparse(A, d);
+    f(d);
+    if (write_back) copy_sparse(d, A);
+  }
+}
+}  // namespace Impl
+
+// Stable sort of COO entries by (row, column)
+// (core/src/Morpheus_Sort.hpp:32-40, core/src/impl/Coo/Serial/Morpheus_Sort_Impl.hpp:111-166).
+template <typename ExecSpace, typename Matrix>
+void sort_by_row_and_column(Matrix& mat) {
+  static_assert(Matrix::format_id == COO_FORMAT, "sort_by_row_and_column takes a CooMatrix");
+  Impl::check_access<ExecSpace, Matrix>();
+  Impl::on_device_coo(mat, true, [](auto& d) {
+    constexpr int vt = Impl::value_code<typename Matrix::non_const_value_type>::value;
+    Impl::check(mph_coo_sort_by_row_and_column((int64_t)d.nnnz(), d.row_indices().data(),
+                                               d.column_indices().data(), d.values().data(), vt,
+                                               nullptr),
+                "Morpheus::sort_by_row_and_column");
+  });
+  mat.touch();
+}
+
+// core/src/Morpheus_Sort.hpp:42-46, core/src/impl/Coo/Serial/Morpheus_Sort_Impl.hpp:168-189
+template <typename ExecSpace, typename Matrix>
+bool is_sorted(Matrix& mat) {
+  static_assert(Matrix::format_id == COO_FORMAT, "is_sorted takes a CooMatrix");
+  Impl::check_access<ExecSpace, Matrix>();
+  int sorted = 0;
+  Impl::on_device_coo(mat, false, [&](auto& d) {
+    Impl::check(mph_coo_is_sorted((int64_t)d.nnnz(), d.crow_indices().data(),
+                                  d.ccolumn_indices().data(), &sorted, nullptr),
+                "Morpheus::is_sorted");
+  });
+  return sorted != 0;
+}
+
+// Non-zeros per diagonal of a COO matrix into a DenseVector of
+// nrows + ncols - 1 counters, diagonal d = col - row + nrows - 1
+// (core/src/Morpheus_MatrixAnalytics.hpp:206-210,
+// core/src/impl/Coo/Kokkos/Morpheus_NonZeros_Per_Diagonal_Impl.hpp:38-74).
+template <typename ExecSpace, typename Matrix, typename Vector>
+void count_nnz_per_diagonal(const Matrix& A, Vector& nnz_per_diagonal, const bool init = true) {
+  static_assert(Matrix::format_id == COO_FORMAT, "count_nnz_per_diagonal takes a CooMatrix");
+  static_assert(std::is_same_v<typename Vector::non_const_value_type, int>,
+                "count_nnz_per_diagonal counts into a DenseVector<int>");
+  Impl::check_access<ExecSpace, Matrix>();
+  const size_t nd = A.nrows() + A.ncols() - 1;
+  if (nnz_per_diagonal.size() != nd)
+    throw InvalidInputException("count_nnz_per_diagonal: vector must hold nrows + ncols - 1 entries");
+  auto run = [&](const auto& d, int* out) {
+    Impl::check(mph_coo_count_nnz_per_diagonal((int64_t)d.nrows(), (int64_t)d.ncols(),
+                                               (int64_t)d.nnnz(), d.crow_indices().data(),
+                                               d.ccolumn_indices().data(), out, init ? 1 : 0,
+                                               nullptr),
+                "Morpheus::count_nnz_per_diagonal");
+  };
+  if constexpr (Matrix::memory_space::is_device) {
+    run(A, nnz_per_diagonal.data());
+  } else {
+    Impl::device_type_of<Matrix> d;
+    d.allocate(A);
+    Impl::copy_sparse(A, d);
+    DenseVector<int, HIPSpace> cnt(nd);
+    Impl::copy_array(nnz_per_diagonal.const_view(), cnt.view(), "nnz_per_diagonal");
+    run(d, cnt.data());
+    Impl::copy_array(cnt.const_view(), nnz_per_diagonal.view(), "nnz_per_diagonal");
+  }
+}
+
+// ============================================================ extensions
+// Launch-shape autotuning of one HIPSpace matrix (no reference counterpart):
+// times the admissible launch shapes of A on x / y and keeps the fastest.
+// Returns the best time per product in milliseconds.
+template <typename Matrix, typename VectorIn, typename VectorOut>
+float tune_multiply(const Matrix& A, const VectorIn& x, VectorOut& y, int reps = 10) {
+  static_assert(Matrix::memory_space::is_device, "tune_multiply takes HIPSpace containers");
+  if (x.size() != A.ncols() || y.size() != A.nrows())
+    throw InvalidInputException("Morpheus::tune_multiply: vector sizes do not match A");
+  float ms = 0.f;
+  Impl::check(mph_matrix_autotune(Impl::matrix_handle(A), 1, x.data(), y.data(), reps, nullptr, &ms),
+              "Morpheus::tune_multiply");
+  return ms;
+}
+
+// Chooses the active format of a HIPSpace DynamicMatrix by timing every
+// admissible format on the device (the measured counterpart of the
+// --morpheus-format flag, core/src/impl/Morpheus_Core.cpp:92-140) and
+// switches A to it.  Returns the chosen format.
+template <typename ExecSpace, typename Dynamic>
+formats_e select_format(Dynamic& A, int reps = 10) {
+  static_assert(is_dynamic_matrix_format_container_v<Dynamic>, "select_format takes a DynamicMatrix");
+  static_assert(ExecSpace::is_device && Dynamic::memory_space::is_device,
+                "select_format runs on HIPSpace containers");
+  using Csr = CsrMatrix<typename Dynamic::non_const_value_type,
+                        typename Dynamic::non_const_index_type, typename Dynamic::array_layout,
+                        HIPSpace>;
+  Csr csr;
+  convert<ExecSpace>(A, csr);
+  int best = CSR_FORMAT;
+  constexpr int vt = Impl::value_code<typename Dynamic::non_const_value_type>::value;
+  Impl::check(mph_autotune_format((int64_t)csr.nrows(), (int64_t)csr.ncols(), (int64_t)csr.nnnz(),
+                                  csr.crow_offsets().data(), csr.ccolumn_indices().data(),
+                                  csr.cvalues().data(), vt, 1, reps, nullptr, &best, nullptr),
+              "Morpheus::select_format");
+  if (convert<ExecSpace>(A, best) != CONV_SUCCESS)
+    throw FormatConversionException("Morpheus::select_format: switch to the chosen format failed");
+  return static_cast<formats_e>(best);
+}
+
+}  // namespace Morpheus
+
+#ifndef MORPHEUS_AMD_NO_KOKKOS_SHIM
+namespace Kokkos {
+template <class Space>
+using Random_XorShift64_Pool = Morpheus::RandomPool<Space>;
+}  // namespace Kokkos
+#endif
