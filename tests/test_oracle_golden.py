@@ -255,3 +255,24 @@ def test_pad_size():
     L = oracle.lib()
     for n, a in [(0, 32), (1, 32), (32, 32), (33, 32), (65_536, 32), (7, 4)]:
         assert L.orc_pad_size(n, a) == pad_size(n, a) == a * ((n + a - 1) // a)
+
+
+@pytest.mark.parametrize("fmt", ["csr", "coo", "dia", "ell"])
+def test_openmp_baseline_matches_serial(fmt):
+    """The OpenMP restatement timed as the CPU baseline
+    (core/src/impl/{Csr,Coo,Dia,Ell}/OpenMP/Morpheus_Multiply_Impl.hpp)
+    computes the same product as the Serial loops, within the FP64 bound
+    (COO boundary rows are combined with atomics, so their order differs)."""
+    from morpheus_amd import generators as gen
+    if fmt in ("csr", "coo"):
+        A = gen.powerlaw_rows(4000, 3000, 1.5, 10.0, 2000, seed=5)
+        if fmt == "coo":
+            A = oracle.to_coo(A)
+    else:
+        A = oracle.from_coo(oracle.to_coo(gen.stencil3d_7(12)), FMT[fmt.upper()])
+    run, x, y = oracle.omp_spmv_fn(A)
+    run()
+    ref = oracle.spmv(A, x)
+    csr = oracle.convert(A, FMT["CSR"])
+    bound = fp64_bound(np.diff(csr.row_offsets), oracle.absrow(csr, x))
+    assert np.all(np.abs(y - ref) <= bound)
