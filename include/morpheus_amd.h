@@ -133,12 +133,18 @@ int mph_matrix_format(mph_matrix_t A);
  *   rows_chunk   : nnz budget of a CSR row block (short rows) / long-row chunk
  *   lanes        : fixed lanes per row for CSR row blocks (1..64, 0 = adaptive)
  *   coo_items    : entries per lane of the COO segmented reduction (4, 8, 16)
- *   rows_per_thread : DIA / ELL rows per thread (1 or 2) */
+ *   rows_per_thread : DIA / ELL rows per thread (1 or 2)
+ *   xtile        : x-tiled long rows of CSR / HYB / HDC SpMV (1 = off,
+ *                  2 = on; 0 = on when x exceeds 2 MiB).  Rows longer than
+ *                  rows_chunk with sorted columns are summed per 64 KiB tile
+ *                  of x staged in LDS, so their x reads are LDS reads instead
+ *                  of scattered cache-line fetches. */
 typedef struct mph_tuning {
   int32_t rows_chunk;
   int32_t lanes;
   int32_t coo_items;
   int32_t rows_per_thread;
+  int32_t xtile;
 } mph_tuning;
 int mph_matrix_set_tuning(mph_matrix_t A, const mph_tuning* t);
 int mph_matrix_get_tuning(mph_matrix_t A, mph_tuning* t);

@@ -68,7 +68,7 @@ pI = C.POINTER(C.c_int)
 
 class Tuning(C.Structure):
     _fields_ = [("rows_chunk", i32), ("lanes", i32), ("coo_items", i32),
-                ("rows_per_thread", i32)]
+                ("rows_per_thread", i32), ("xtile", i32)]
 
 
 _SIGS = {

@@ -37,6 +37,14 @@ void release_plans(mph_matrix_impl* A) {
   dfree(p.blocks);
   if (p.partial) dfree(p.partial);
   if (p.tail_ro) dfree(p.tail_ro);
+  if (p.sblocks) dfree(p.sblocks);
+  if (p.xt_row) dfree(p.xt_row);
+  if (p.xt_lro) dfree(p.xt_lro);
+  if (p.xt_sub) dfree(p.xt_sub);
+  if (p.xt_sub_start) dfree(p.xt_sub_start);
+  if (p.xt_col) dfree(p.xt_col);
+  if (p.xt_val) dfree(p.xt_val);
+  if (p.xt_partial) dfree(p.xt_partial);
   p = RowBlockPlan{};
   CooPlan& c = A->coo;
   if (c.first_val) dfree(c.first_val);
@@ -384,7 +392,9 @@ int mph_matrix_set_tuning(mph_matrix_t A, const mph_tuning* t) {
     return invalid("coo_items must be 4, 8 or 16");
   if (t->rows_per_thread != 0 && t->rows_per_thread != 1 && t->rows_per_thread != 2)
     return invalid("rows_per_thread must be 1 or 2");
-  const bool replan = t->rows_chunk != A->tuning.rows_chunk || t->coo_items != A->tuning.coo_items;
+  if (t->xtile < 0 || t->xtile > 2) return invalid("xtile must be 0, 1 or 2");
+  const bool replan = t->rows_chunk != A->tuning.rows_chunk ||
+                      t->coo_items != A->tuning.coo_items || t->xtile != A->tuning.xtile;
   A->tuning = *t;
   if (replan) release_plans(A);
   return MPH_OK;

@@ -23,6 +23,23 @@ constexpr int kDefaultChunkF64 = 1024;   // LDS items of a short block (x2)
 constexpr int kDefaultChunkF32 = 2048;
 constexpr int kCompensateLen = 1024;     // FP32 rows longer than this use TwoSum
 
+// x-tiled long rows (SpMV): the columns are cut into tiles of 32 KiB of x.
+// The analysis copies the long rows' items tile-major (16-bit column within
+// the tile + value); workgroup (group, tile) stages its x tile in LDS and its
+// waves reduce the group's rows over that tile, one sub-block per wave.
+struct XTSub {
+  int32_t i0;  // first long-row index of the sub-block (the next record ends it)
+  int32_t e0;  // first item of the sub-block (item numbering of the run offsets)
+  int32_t p0;  // first slot of the sub-block in the copy (slots = 8 * ceil(n / 8))
+  int32_t pad;
+};
+constexpr int kXTileThreads = 512;
+constexpr int kXTileBytes = 32 << 10;          // x tile in LDS
+constexpr int kXTileCap = 512;                 // items of a (one-wave) sub-block
+constexpr int kXTileRows = 64;                 // rows of a sub-block
+constexpr int kXTileMaxGroupRows = 1024;       // rows of a group (run offsets in LDS)
+constexpr int64_t kXTileMinXBytes = 2 << 20;  // auto: x larger than this
+
 // Analysis of a CSR-shaped item set (CSR itself, the CSR part of HDC, the
 // COO tail of HYB viewed through row offsets built here).
 struct RowBlockPlan {
@@ -32,6 +49,22 @@ struct RowBlockPlan {
   void* partial = nullptr;       // per-block accumulators (long-row chunks)
   int32_t* tail_ro = nullptr;    // HYB only: row offsets of the COO tail
   int32_t nlong = 0;             // rows split into chunks
+  // x-tiled long rows (SpMV); xt_rows == 0: long rows use position chunks
+  int32_t xt_tile = 0;           // x entries per tile
+  int32_t xt_ntiles = 0;
+  int32_t xt_groups = 0;
+  int32_t xt_gr = 0;             // long rows per group
+  int32_t xt_rows = 0;           // tiled long rows (longest first)
+  int32_t* xt_row = nullptr;     // [xt_rows] row ids
+  int32_t* xt_lro = nullptr;     // [(groups*ntiles) * (gr+1)] item offsets of each row's run
+  XTSub* xt_sub = nullptr;       // sub-blocks, a sentinel after each (group, tile)
+  int32_t* xt_sub_start = nullptr;  // [groups*ntiles + 1] first sub-block of (group, tile)
+  uint16_t* xt_col = nullptr;    // tile-major copy: column - tile * xt_tile
+  void* xt_val = nullptr;        // tile-major copy: values
+  void* xt_partial = nullptr;    // [ntiles * xt_rows] (pairs in FP32), tile-major
+  RowBlock* sblocks = nullptr;   // short blocks, {row, -2} per tiled row, chunks of the rest
+  int32_t nsblocks = 0;
+  int32_t nchunked = 0;          // long rows left to position chunks
 };
 
 // Analysis of a row-sorted COO matrix for the segmented reduction.

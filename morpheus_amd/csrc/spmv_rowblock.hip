@@ -70,23 +70,49 @@ template <typename V, int PRE, int KR>
 __device__ __forceinline__ void row_prefix(const RBArgs<V>& a, int64_t r,
                                            typename vtraits<V>::acc* s) {
   using Acc = typename vtraits<V>::acc;
+  // loads of 8 diagonals / slots are issued before their x reads; the
+  // products are added in diagonal / slot order
+  constexpr int kP = 8;
   if constexpr (PRE == PRE_DIA) {
-    for (int d = 0; d < a.ndiag; d++) {
-      const int64_t col = r + a.doff[d];
-      if (col >= 0 && col < a.ncols) {
-        const Acc v = vtraits<V>::widen(ld_stream(a.dval + r + a.dpitch * d));
+    for (int d0 = 0; d0 < a.ndiag; d0 += kP) {
+      int64_t col[kP];
+      Acc v[kP];
 #pragma unroll
-        for (int q = 0; q < KR; q++) s[q] += v * (Acc)a.x[col * a.ld + q];
+      for (int q = 0; q < kP; q++) {
+        const int d = d0 + q;
+        col[q] = d < a.ndiag ? r + a.doff[d] : -1;
+        const bool in = d < a.ndiag && col[q] >= 0 && col[q] < a.ncols;
+        v[q] = in ? vtraits<V>::widen(ld_stream(a.dval + r + a.dpitch * d)) : (Acc)0;
+        if (!in) col[q] = -1;
+      }
+#pragma unroll
+      for (int q = 0; q < kP; q++) {
+        if (col[q] >= 0) {
+#pragma unroll
+          for (int k = 0; k < KR; k++) s[k] += v[q] * (Acc)a.x[col[q] * a.ld + k];
+        }
       }
     }
   } else if constexpr (PRE == PRE_ELL) {
-    for (int n = 0; n < a.eloop; n++) {
-      const int64_t slot = r + a.epitch * n;
-      const int32_t c = ld_stream(a.eci + slot);
-      if (c != -1) {
-        const Acc v = vtraits<V>::widen(ld_stream(a.evalp + slot));
+    for (int n0 = 0; n0 < a.eloop; n0 += kP) {
+      int32_t c[kP];
+      Acc v[kP];
 #pragma unroll
-        for (int q = 0; q < KR; q++) s[q] += v * (Acc)a.x[(int64_t)c * a.ld + q];
+      for (int q = 0; q < kP; q++) {
+        const int64_t slot = r + a.epitch * (n0 + q);
+        c[q] = n0 + q < a.eloop ? ld_stream(a.eci + slot) : -1;
+      }
+#pragma unroll
+      for (int q = 0; q < kP; q++) {
+        const int64_t slot = r + a.epitch * (n0 + q);
+        v[q] = c[q] != -1 ? vtraits<V>::widen(ld_stream(a.evalp + slot)) : (Acc)0;
+      }
+#pragma unroll
+      for (int q = 0; q < kP; q++) {
+        if (c[q] != -1) {
+#pragma unroll
+          for (int k = 0; k < KR; k++) s[k] += v[q] * (Acc)a.x[(int64_t)c[q] * a.ld + k];
+        }
       }
     }
   }
@@ -235,6 +261,7 @@ __global__ __launch_bounds__(kRowBlockThreads) void rowblock_spmv_kernel(
   __shared__ Acc s_red[2 * kRowBlockThreads / kWave];
   const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
   const RowBlock blk = a.blocks[b];
+  if (blk.sub < -1) return;  // a long row handled by the sliced pass
   if (blk.sub >= 0) {
     long_chunk_spmv<V, COMP>(a, b, blk, s_red);
     return;
@@ -286,6 +313,375 @@ __global__ __launch_bounds__(kRowBlockThreads) void rowblock_spmv_kernel(
     }
     if (l == 0) store_y(a.y, row0 + r, out, a.init);
   }
+}
+
+// --------------------------------------------------------------- x-tiled
+// Long rows with sorted columns, summed per tile of x (64 KiB).  The analysis
+// copies their items tile-major: for (group, tile), the items of the group's
+// rows whose columns fall in the tile, row after row, as (16-bit column
+// within the tile, value), each sub-block (at most kXTileCap items of at
+// most kXTileRows whole runs) stored transposed: its item j = 8a + b sits in
+// slot p0 + b*S + a (S = ceil(n/8)), so lane a holds items 8a..8a+7 while
+// every load instruction reads 64 consecutive slots.  Workgroup (group, tile)
+// copies x[tile*T, +T) into LDS once; its waves then take the sub-blocks in
+// turn, form the products with x read from LDS and reduce the runs with a
+// segmented sum (lane-serial, then a segmented scan across the wave) into the
+// (tile, row) partials.  Scattered x
+// reads thus cost LDS bandwidth rather than one cache-line fill each.  Tiles
+// are dealt to the XCDs by tile index (workgroup id % 8 is its XCD), so an x
+// tile is fetched from memory once per XCD.
+// LDS of xtile_kernel: x tile, a group's run offsets and its sub-block
+// records (a group has at most kXTileMaxGroupRows rows, so
+// at most as many sub-blocks, plus a sentinel).
+inline size_t xtile_lds_bytes(size_t vec, size_t /*acc*/, int32_t tile) {
+  return vec * (size_t)tile + sizeof(int32_t) * (kXTileMaxGroupRows + 2) +
+         sizeof(XTSub) * (kXTileMaxGroupRows + 1);
+}
+
+struct XTArgs {
+  int32_t tile, ntiles, groups, gr, rows;
+  const int32_t* __restrict__ row;
+  const int32_t* __restrict__ lro;
+  const XTSub* __restrict__ sub;
+  const int32_t* __restrict__ sub_start;
+  const uint16_t* __restrict__ col;
+  const void* __restrict__ val;
+  void* __restrict__ partial;
+};
+
+// Items of one sub-block as loaded by a wave: lane l holds the kBatch
+// consecutive items l*kBatch .. l*kBatch + kBatch - 1.
+template <typename V>
+struct XTLoad {
+  uint32_t c[kBatch];
+  typename vtraits<V>::acc v[kBatch];
+};
+
+template <typename V>
+__device__ __forceinline__ void xtile_load(const XTArgs& t, const V* cval, const XTSub& s0,
+                                           int32_t n, int lane, XTLoad<V>& d) {
+  const int32_t jb = lane * kBatch;
+  const int32_t S = (n + kBatch - 1) / kBatch;
+#pragma unroll
+  for (int q = 0; q < kBatch; q++) {
+    const int32_t j = jb + q;
+    const int64_t slot = (int64_t)s0.p0 + (int64_t)q * S + lane;
+    d.c[q] = j < n ? (uint32_t)ld_stream(t.col + slot) : 0u;
+    d.v[q] = j < n ? vtraits<V>::widen(ld_stream(cval + slot)) : (typename vtraits<V>::acc)0;
+  }
+}
+
+// A running sum that is plain (FP64) or TwoSum-compensated (FP32).
+template <typename A, bool C>
+struct XSum {
+  A s, c;
+  __device__ static XSum zero() { return XSum{0, 0}; }
+  __device__ void add(A v) {
+    if constexpr (C) {
+      A t, e;
+      two_sum(s, v, t, e);
+      s = t;
+      c += e;
+    } else {
+      s += v;
+    }
+  }
+  __device__ void merge(const XSum& o) {
+    if constexpr (C) {
+      A t, e;
+      two_sum(s, o.s, t, e);
+      s = t;
+      c += e + o.c;
+    } else {
+      s += o.s;
+    }
+  }
+  __device__ XSum up(int d) const { return XSum{shfl_up(s, d), C ? shfl_up(c, d) : (A)0}; }
+};
+
+template <typename V, bool COMP>
+__global__ __launch_bounds__(kXTileThreads) void xtile_kernel(RBArgs<V> a, XTArgs t) {
+  using Acc = typename vtraits<V>::acc;
+  using Vec = typename vtraits<V>::vec;
+  using S = XSum<Acc, COMP>;
+  constexpr int kWaves = kXTileThreads / kWave;
+  extern __shared__ __align__(16) unsigned char smem[];
+  // LDS: x tile | run offsets of the group | sub-block records
+  Vec* xs = reinterpret_cast<Vec*>(smem);
+  int32_t* s_lro = reinterpret_cast<int32_t*>(xs + t.tile);
+  XTSub* s_sub = reinterpret_cast<XTSub*>(s_lro + kXTileMaxGroupRows + 2);
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const uint32_t id = blockIdx.x;
+  const uint32_t k = id / kNumXcd;
+  const int32_t g = (int32_t)(k % (uint32_t)t.groups);
+  const int32_t tile = (int32_t)((k / (uint32_t)t.groups) * kNumXcd + id % kNumXcd);
+  if (tile >= t.ntiles) return;  // whole workgroup: no barrier reached
+  const int64_t c0 = (int64_t)tile * t.tile;
+  const int32_t nx = (int32_t)min<int64_t>(t.tile, a.ncols - c0);
+  const int64_t blk = (int64_t)g * t.ntiles + tile;
+  const int32_t* lro = t.lro + blk * (t.gr + 1);
+  const int32_t gstart = g * t.gr;
+  const int32_t sb0 = t.sub_start[blk];
+  const int32_t nsub = t.sub_start[blk + 1] - 1 - sb0;  // sub-blocks (a sentinel follows)
+  const V* cval = static_cast<const V*>(t.val);
+  Acc* part = static_cast<Acc*>(t.partial);
+
+  // prologue: the first sub-block's items, the x tile, the run offsets and
+  // the sub-block records, all requested before one barrier
+  XTSub s0{0, 0}, s1{0, 0};
+  XTLoad<V> cur;
+  if (wave < nsub) {
+    s0 = t.sub[sb0 + wave];
+    s1 = t.sub[sb0 + wave + 1];
+    xtile_load<V>(t, cval, s0, s1.e0 - s0.e0, lane, cur);
+  }
+  {
+    constexpr int kPer = 8;  // x loads in flight per thread
+    for (int32_t j0 = threadIdx.x; j0 < nx; j0 += kXTileThreads * kPer) {
+      Vec xv[kPer];
+#pragma unroll
+      for (int q = 0; q < kPer; q++) {
+        const int32_t j = j0 + q * kXTileThreads;
+        xv[q] = j < nx ? a.x[c0 + j] : (Vec)0;
+      }
+#pragma unroll
+      for (int q = 0; q < kPer; q++) {
+        const int32_t j = j0 + q * kXTileThreads;
+        if (j < nx) xs[j] = xv[q];
+      }
+    }
+  }
+  for (int32_t j = threadIdx.x; j <= t.gr; j += kXTileThreads) s_lro[j] = lro[j];
+  for (int32_t j = threadIdx.x; j <= nsub; j += kXTileThreads) s_sub[j] = t.sub[sb0 + j];
+  __syncthreads();
+
+  // each wave takes every kWaves-th sub-block and reduces its runs with a
+  // segmented sum: lane-serial inside a lane, a segmented scan across lanes
+  for (int32_t q = wave; q < nsub; q += kWaves) {
+    const int32_t nr = s1.i0 - s0.i0, n = s1.e0 - s0.e0;
+    const int32_t i0 = s0.i0, e0 = s0.e0;
+    const int32_t* ro = s_lro + (i0 - gstart);  // ro[r] - e0: first item of run r
+    Acc p[kBatch];
+#pragma unroll
+    for (int b = 0; b < kBatch; b++)
+      // the copy holds columns inside the tile; the clamp keeps a corrupted
+      // entry inside the LDS tile
+      p[b] = cur.v[b] * (Acc)xs[min(cur.c[b], (uint32_t)(nx - 1))];
+    if (q + kWaves < nsub) {
+      s0 = s_sub[q + kWaves];
+      s1 = s_sub[q + kWaves + 1];
+      xtile_load<V>(t, cval, s0, s1.e0 - s0.e0, lane, cur);
+    }
+    auto put = [&](int32_t r, const S& v) {
+      const int64_t o = (int64_t)tile * t.rows + i0 + r;
+      if constexpr (COMP) {
+        part[2 * o] = v.s;
+        part[2 * o + 1] = v.c;
+      } else {
+        part[o] = v.s;
+      }
+    };
+    const int32_t jb = lane * kBatch;
+    const int cnt = (int)max(0, min(kBatch, n - jb));
+    const bool active = cnt > 0;
+    // run of the lane's first item: the last r with ro[r] <= jb
+    int32_t r = 0;
+    if (active) {
+      int32_t lo = 0, hi = nr - 1;
+      while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (ro[mid] - e0 <= jb)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      r = lo;
+    }
+    const int32_t h = r;
+    int32_t cur_r = r;
+    S head = S::zero(), run = S::zero();
+    bool single = true;
+    int32_t next_start = active ? ro[cur_r + 1] - e0 : 0;
+#pragma unroll
+    for (int b = 0; b < kBatch; b++) {
+      if (b < cnt) {
+        const int32_t j = jb + b;
+        if (j >= next_start) {
+          // a run ends here: the lane's head run, or a run complete in the lane
+          if (single) {
+            head = run;
+            single = false;
+          } else {
+            put(cur_r, run);
+          }
+          do {
+            cur_r++;
+            next_start = ro[cur_r + 1] - e0;
+          } while (j >= next_start);
+          run = S::zero();
+        }
+        run.add(p[b]);
+      }
+    }
+    const int32_t tl = cur_r;
+    // segmented inclusive scan of the lane tails across the wave
+    const int32_t t_prev = shfl_up(tl, 1);
+    const bool cont = active && lane > 0 && single && h == t_prev;
+    S acc = active ? run : S::zero();
+    int flag = cont ? 0 : 1;
+    for (int d = 1; d < kWave; d <<= 1) {
+      const S o = acc.up(d);
+      const int of = shfl_up(flag, d);
+      if (lane >= d && !flag) acc.merge(o);
+      if (lane >= d) flag |= of;
+    }
+    const S prev_acc = acc.up(1);
+    const int32_t h_next = shfl_down(h, 1);
+    const int last_lane = (n - 1) / kBatch;
+    if (active) {
+      if (!single) {
+        S hv = head;
+        if (lane > 0 && h == t_prev) hv.merge(prev_acc);
+        put(h, hv);
+      }
+      if (lane == last_lane || h_next != tl) put(tl, acc);
+    }
+    // runs without items in this tile
+    if (lane < nr && ro[lane] == ro[lane + 1]) put(lane, S::zero());
+  }
+}
+
+// Analysis: the tile-major copy.  Thread (tile q, long row i) copies the
+// row's items [bnd(q, i), bnd(q + 1, i)) to its run of the (group, tile)
+// block, placing item j of a sub-block at slot p0 + (j % 8) * S + j / 8.
+template <typename V>
+__global__ void xtile_copy_kernel(const int32_t* __restrict__ bnd, int32_t nl, int32_t ntiles,
+                                  int32_t gr, int64_t tile, const int32_t* __restrict__ lro,
+                                  const int32_t* __restrict__ run_sub,
+                                  const XTSub* __restrict__ sub, const int32_t* __restrict__ ci,
+                                  const V* __restrict__ val, uint16_t* __restrict__ col_out,
+                                  V* __restrict__ val_out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)nl * ntiles) return;
+  const int64_t q = t / nl, i = t % nl;
+  const int64_t g = i / gr;
+  const int64_t w = (g * ntiles + q) * (gr + 1) + (i - g * gr);
+  const int32_t b = bnd[q * nl + i], e = bnd[(q + 1) * nl + i];
+  const XTSub s0 = sub[run_sub[w]], s1 = sub[run_sub[w] + 1];
+  const int32_t n = s1.e0 - s0.e0;
+  const int32_t S = (n + 7) / 8;
+  const int32_t first = lro[w] - s0.e0;  // the run's first item within the sub-block
+  for (int32_t j = b; j < e; j++) {
+    const int32_t L = first + (j - b);
+    const int64_t slot = (int64_t)s0.p0 + (int64_t)(L % 8) * S + L / 8;
+    col_out[slot] = (uint16_t)(ci[j] - q * tile);
+    val_out[slot] = val[j];
+  }
+}
+
+// Combine: a 256-thread block adds the tile partials of 32 long rows; thread
+// (range, row) adds its row's partials over one of 8 contiguous tile ranges
+// (loads coalesced across the 32 rows), then the range sums are added in
+// range order with the row's DIA / ELL part -- a fixed order, so results are
+// reproducible.
+constexpr int kXCombRows = 32;
+constexpr int kXCombRanges = 256 / kXCombRows;
+
+template <typename V, int PRE, bool COMP>
+__global__ __launch_bounds__(256) void xtile_combine_kernel(RBArgs<V> a, XTArgs t) {
+  using Acc = typename vtraits<V>::acc;
+  __shared__ Acc s_sum[kXCombRanges][kXCombRows];
+  __shared__ Acc s_err[kXCombRanges][kXCombRows];
+  const int ri = threadIdx.x % kXCombRows, rg = threadIdx.x / kXCombRows;
+  const int64_t i = (int64_t)blockIdx.x * kXCombRows + ri;
+  const Acc* part = static_cast<const Acc*>(t.partial);
+  const int32_t per = (t.ntiles + kXCombRanges - 1) / kXCombRanges;
+  const int32_t q0 = rg * per, q1 = min(t.ntiles, q0 + per);
+  csum<Acc> cs{0, 0};
+  Acc s = 0;
+  if (i < t.rows) {
+    for (int32_t qb = q0; qb < q1; qb += kBatch) {
+      Acc ps[kBatch], pc[kBatch];
+#pragma unroll
+      for (int b = 0; b < kBatch; b++) {
+        const int64_t o = (int64_t)(qb + b) * t.rows + i;
+        if constexpr (COMP) {
+          ps[b] = qb + b < q1 ? part[2 * o] : (Acc)0;
+          pc[b] = qb + b < q1 ? part[2 * o + 1] : (Acc)0;
+        } else {
+          ps[b] = qb + b < q1 ? part[o] : (Acc)0;
+          pc[b] = 0;
+        }
+      }
+#pragma unroll
+      for (int b = 0; b < kBatch; b++) {
+        if (qb + b < q1) {
+          if constexpr (COMP)
+            cs.combine(ps[b], pc[b]);
+          else
+            s += ps[b];
+        }
+      }
+    }
+  }
+  s_sum[rg][ri] = COMP ? cs.s : s;
+  s_err[rg][ri] = COMP ? cs.c : (Acc)0;
+  __syncthreads();
+  if (rg != 0 || i >= t.rows) return;
+  const int64_t r = t.row[i];
+  Acc pre[1] = {0};
+  row_prefix<V, PRE, 1>(a, r, pre);
+  if constexpr (COMP) {
+    csum<Acc> tot{pre[0], 0};
+    for (int q = 0; q < kXCombRanges; q++) tot.combine(s_sum[q][ri], s_err[q][ri]);
+    store_y(a.y, r, tot.value(), a.init);
+  } else {
+    Acc tot = 0;
+    for (int q = 0; q < kXCombRanges; q++) tot += s_sum[q][ri];
+    store_y(a.y, r, tot + pre[0], a.init);
+  }
+}
+
+// Analysis: per long row i and tile q = 0..ntiles, the first item whose
+// column is >= q * tile (lower bound in the sorted columns), stored
+// tile-major at out[q * nl + i]; unsorted[i] flags rows whose columns are
+// not sorted.
+__global__ void xtile_bounds_kernel(const int32_t* __restrict__ ro,
+                                    const int32_t* __restrict__ ci,
+                                    const int32_t* __restrict__ rows, int32_t nl,
+                                    int32_t ntiles, int64_t tile, int32_t* __restrict__ out,
+                                    int32_t* __restrict__ unsorted) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)nl * (ntiles + 1)) return;
+  const int64_t q = t / nl;
+  const int64_t i = t % nl;
+  const int32_t r = rows[i];
+  const int32_t b = ro[r], e = ro[r + 1];
+  if (q == 0) {
+    out[t] = b;
+    int32_t bad = 0;
+    for (int32_t j = b + 1; j < e; j++)
+      if (ci[j - 1] > ci[j]) {
+        bad = 1;
+        break;
+      }
+    unsorted[i] = bad;
+    return;
+  }
+  if (q == ntiles) {
+    out[t] = e;
+    return;
+  }
+  const int64_t key = q * tile;
+  int32_t lo = b, hi = e;
+  while (lo < hi) {
+    const int32_t mid = lo + (hi - lo) / 2;
+    if ((int64_t)ci[mid] < key)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  out[t] = lo;
 }
 
 // --------------------------------------------------------------- k > 1
@@ -526,6 +922,301 @@ int rows_sorted(const int32_t* ri, int64_t nnz, int64_t nrows, hipStream_t s,
   return st;
 }
 
+namespace {
+
+template <typename V>
+void launch_xtile_copy(const int32_t* bnd, int32_t nl, int32_t ntiles, int32_t gr, int64_t tile,
+                       const int32_t* lro, const int32_t* run_sub, const XTSub* sub,
+                       const int32_t* ci, const void* val, uint16_t* col_out, void* val_out,
+                       hipStream_t s) {
+  const int64_t n = (int64_t)nl * ntiles;
+  hipLaunchKernelGGL((xtile_copy_kernel<V>), dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s,
+                     bnd, nl, ntiles, gr, tile, lro, run_sub, sub, ci,
+                     static_cast<const V*>(val), col_out, static_cast<V*>(val_out));
+}
+
+// The x-tiled plan of the long rows (see xtile_kernel), built on the host
+// from the validated row blocks and the per-row tile bounds (computed on the
+// device and checked here), then the tile-major copy is made on the device.
+// Rows with unsorted columns leave the plan off; rows with a run longer than
+// kXTileCap / 2 items in some tile stay with the position chunks.
+int xtile_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
+  RowBlockPlan& p = A->rb;
+  const int64_t nrows = A->nrows;
+  const size_t vec_bytes = A->vt == MPH_F64 ? 8 : 4;
+  const int mode = A->tuning.xtile;
+  const bool on = mode == 2 || (mode == 0 && (int64_t)A->ncols * (int64_t)vec_bytes >= kXTileMinXBytes);
+  if (!on || p.nlong == 0 || nrows == 0 || A->ncols == 0) return MPH_OK;
+  const int32_t* d_ci = A->format == MPH_HYB ? A->coo_ci : A->csr_ci;
+  const void* d_val = A->format == MPH_HYB ? A->coo_v : A->csr_v;
+
+  std::vector<int32_t> ro(nrows + 1);
+  std::vector<RowBlock> bl(p.nblocks + 1);
+  MPH_HIP_CHECK(hipMemcpyAsync(ro.data(), d_ro, sizeof(int32_t) * (nrows + 1),
+                               hipMemcpyDeviceToHost, s));
+  MPH_HIP_CHECK(hipMemcpyAsync(bl.data(), p.blocks, sizeof(RowBlock) * (p.nblocks + 1),
+                               hipMemcpyDeviceToHost, s));
+  MPH_HIP_CHECK(hipStreamSynchronize(s));
+  std::vector<int32_t> lrows;
+  for (int64_t b = 0; b < p.nblocks; b++)
+    if (bl[b].sub == 0) lrows.push_back(bl[b].row);
+  // longest first, so neighbouring rows of a sub-block have similar runs
+  std::stable_sort(lrows.begin(), lrows.end(), [&](int32_t u, int32_t v) {
+    return ro[u + 1] - ro[u] > ro[v + 1] - ro[v];
+  });
+  const int32_t nl = (int32_t)lrows.size();
+  const int32_t tile = (int32_t)(kXTileBytes / vec_bytes);
+  const int32_t ntiles = (int32_t)ceil_div((int64_t)A->ncols, (int64_t)tile);
+
+  // per-row tile bounds on the device
+  std::vector<int32_t> bnd((size_t)nl * (ntiles + 1)), uns(nl);
+  {
+    int32_t *d_rows = nullptr, *d_bnd = nullptr, *d_uns = nullptr;
+    int st = MPH_OK;
+    if (hipMalloc(&d_rows, sizeof(int32_t) * nl) != hipSuccess ||
+        hipMalloc(&d_bnd, sizeof(int32_t) * bnd.size()) != hipSuccess ||
+        hipMalloc(&d_uns, sizeof(int32_t) * nl) != hipSuccess) {
+      set_error("hipMalloc failed in the x-tile analysis");
+      st = MPH_ERR_OUT_OF_MEMORY;
+    }
+    if (st == MPH_OK &&
+        hipMemcpyAsync(d_rows, lrows.data(), sizeof(int32_t) * nl, hipMemcpyHostToDevice, s) !=
+            hipSuccess)
+      st = MPH_ERR_HIP;
+    if (st == MPH_OK) {
+      hipLaunchKernelGGL(xtile_bounds_kernel, dim3((unsigned)ceil_div((int64_t)bnd.size(), 256)),
+                         dim3(256), 0, s, d_ro, d_ci, d_rows, nl, ntiles, (int64_t)tile, d_bnd,
+                         d_uns);
+      if (hipGetLastError() != hipSuccess ||
+          hipMemcpyAsync(bnd.data(), d_bnd, sizeof(int32_t) * bnd.size(), hipMemcpyDeviceToHost,
+                         s) != hipSuccess ||
+          hipMemcpyAsync(uns.data(), d_uns, sizeof(int32_t) * nl, hipMemcpyDeviceToHost, s) !=
+              hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess) {
+        set_error("x-tile bounds kernel failed");
+        st = MPH_ERR_HIP;
+      }
+    }
+    dfree(d_rows);
+    dfree(d_bnd);
+    dfree(d_uns);
+    if (st != MPH_OK) return st;
+  }
+  for (int32_t i = 0; i < nl; i++)
+    if (uns[i]) return MPH_OK;  // unsorted columns: keep the position chunks
+  auto B = [&](int32_t q, int32_t i) { return bnd[(size_t)q * nl + i]; };
+  for (int32_t i = 0; i < nl; i++) {
+    const int32_t r = lrows[i];
+    if (B(0, i) != ro[r] || B(ntiles, i) != ro[r + 1]) {
+      set_error("x-tile analysis check failed: row bounds");
+      return MPH_ERR_RUNTIME;
+    }
+    for (int32_t q = 0; q < ntiles; q++)
+      if (B(q, i) > B(q + 1, i)) {
+        set_error("x-tile analysis check failed: bounds not monotone");
+        return MPH_ERR_RUNTIME;
+      }
+  }
+  // tiled rows: every run fits half a sub-block
+  std::vector<int32_t> trow;     // indices into lrows
+  std::vector<char> tiled(nrows, 0);
+  for (int32_t i = 0; i < nl; i++) {
+    int32_t mx = 0;
+    for (int32_t q = 0; q < ntiles; q++) mx = std::max(mx, B(q + 1, i) - B(q, i));
+    if (mx <= kXTileCap / 2) {
+      trow.push_back(i);
+      tiled[lrows[i]] = 1;
+    }
+  }
+  const int32_t nt = (int32_t)trow.size();
+  if (nt == 0) return MPH_OK;
+
+  // groups: about eight (group, tile) workgroups per CU (a group re-reads the
+  // x tiles, which its XCD's L2 holds); rows are dealt to the groups by length
+  // rank (group g gets ranks g, g + G, ...), so every group has the same mix
+  const int64_t want = 8 * (int64_t)device_cu_count();
+  int32_t ngroups = (int32_t)std::min<int64_t>(nt, std::max<int64_t>(1, ceil_div(want, (int64_t)ntiles)));
+  ngroups = (int32_t)std::max<int64_t>(ngroups, ceil_div((int64_t)nt, (int64_t)kXTileMaxGroupRows));
+  const int32_t gr = (int32_t)ceil_div((int64_t)nt, (int64_t)ngroups);
+  ngroups = (int32_t)ceil_div((int64_t)nt, (int64_t)gr);
+  const int64_t nblk = (int64_t)ngroups * ntiles;
+  {
+    std::vector<int32_t> dealt;
+    dealt.reserve(nt);
+    for (int32_t g = 0; g < ngroups; g++)
+      for (int64_t rank = g; rank < nt; rank += ngroups) dealt.push_back(trow[rank]);
+    trow.swap(dealt);
+  }
+
+  // run offsets, sub-blocks and the tiled rows' bounds (compacted)
+  std::vector<int32_t> lro((size_t)nblk * (gr + 1));
+  std::vector<int32_t> run_sub((size_t)nblk * (gr + 1), 0);
+  std::vector<XTSub> subs;
+  std::vector<int32_t> sub_start(nblk + 1);
+  std::vector<int32_t> tb((size_t)nt * (ntiles + 1));
+  for (int32_t j = 0; j < nt; j++)
+    for (int32_t q = 0; q <= ntiles; q++) tb[(size_t)q * nt + j] = B(q, trow[j]);
+  int64_t total = 0;  // items
+  int64_t slots = 0;  // copy slots (sub-blocks padded to multiples of 8)
+  for (int32_t g = 0; g < ngroups; g++) {
+    const int32_t j0 = g * gr, j1 = std::min(nt, j0 + gr);
+    for (int32_t q = 0; q < ntiles; q++) {
+      const int64_t k = (int64_t)g * ntiles + q;
+      int32_t* L = &lro[(size_t)k * (gr + 1)];
+      int32_t* RS = &run_sub[(size_t)k * (gr + 1)];
+      sub_start[k] = (int32_t)subs.size();
+      int32_t cur_rows = 0;
+      int64_t cur_items = 0;
+      for (int32_t j = j0; j < j1; j++) {
+        const int32_t len = B(q + 1, trow[j]) - B(q, trow[j]);
+        if (cur_rows == 0 || cur_rows == kXTileRows || cur_items + len > kXTileCap) {
+          if (cur_rows > 0) slots += 8 * ceil_div(cur_items, (int64_t)8);
+          subs.push_back(XTSub{j, (int32_t)total, (int32_t)slots, 0});
+          cur_rows = 0;
+          cur_items = 0;
+        }
+        L[j - j0] = (int32_t)total;
+        RS[j - j0] = (int32_t)subs.size() - 1;
+        total += len;
+        cur_rows++;
+        cur_items += len;
+      }
+      if (cur_rows > 0) slots += 8 * ceil_div(cur_items, (int64_t)8);
+      for (int32_t j = j1; j <= j0 + gr; j++) L[j - j0] = (int32_t)total;
+      subs.push_back(XTSub{j1, (int32_t)total, (int32_t)slots, 0});  // sentinel
+      if (slots > INT32_MAX) {
+        set_error("x-tile copy exceeds the int32 item range");
+        return MPH_ERR_INVALID_INPUT;
+      }
+    }
+  }
+  sub_start[nblk] = (int32_t)subs.size();
+  {
+    int64_t expect = 0;
+    for (int32_t j = 0; j < nt; j++) expect += ro[lrows[trow[j]] + 1] - ro[lrows[trow[j]]];
+    if (expect != total) {
+      set_error("x-tile analysis check failed: copy size");
+      return MPH_ERR_RUNTIME;
+    }
+  }
+  std::vector<int32_t> trow_id(nt);
+  for (int32_t j = 0; j < nt; j++) trow_id[j] = lrows[trow[j]];
+
+  // short blocks with the tiled rows skipped; other long rows keep their chunks
+  std::vector<RowBlock> sb;
+  int32_t nchunked = 0;
+  for (int64_t b = 0; b < p.nblocks; b++) {
+    const RowBlock k = bl[b];
+    if (k.sub < 0) {
+      sb.push_back(k);
+    } else if (tiled[k.row]) {
+      if (k.sub == 0) sb.push_back(RowBlock{k.row, -2});
+    } else {
+      sb.push_back(k);
+      if (k.sub == 0) nchunked++;
+    }
+  }
+  sb.push_back(RowBlock{(int32_t)nrows, -1});
+
+  // device side
+  const size_t vsz = A->vt == MPH_F64 ? 8 : A->vt == MPH_F32 ? 4 : 2;
+  const size_t acc = A->vt == MPH_F64 ? 8 : 4, pair = A->vt == MPH_F64 ? 1 : 2;
+  int32_t *d_row = nullptr, *d_lro = nullptr, *d_ss = nullptr, *d_tb = nullptr, *d_rs = nullptr;
+  XTSub* d_sub = nullptr;
+  uint16_t* d_col = nullptr;
+  void *d_cv = nullptr, *d_part = nullptr;
+  RowBlock* d_sb = nullptr;
+  auto release = [&]() {
+    dfree(d_row);
+    dfree(d_lro);
+    dfree(d_ss);
+    dfree(d_tb);
+    dfree(d_rs);
+    dfree(d_sub);
+    dfree(d_col);
+    dfree(d_cv);
+    dfree(d_part);
+    dfree(d_sb);
+  };
+  if (hipMalloc(&d_row, sizeof(int32_t) * nt) != hipSuccess ||
+      hipMalloc(&d_lro, sizeof(int32_t) * lro.size()) != hipSuccess ||
+      hipMalloc(&d_ss, sizeof(int32_t) * sub_start.size()) != hipSuccess ||
+      hipMalloc(&d_tb, sizeof(int32_t) * tb.size()) != hipSuccess ||
+      hipMalloc(&d_rs, sizeof(int32_t) * run_sub.size()) != hipSuccess ||
+      hipMalloc(&d_sub, sizeof(XTSub) * subs.size()) != hipSuccess ||
+      hipMalloc(&d_col, sizeof(uint16_t) * std::max<int64_t>(slots, 1)) != hipSuccess ||
+      hipMalloc(&d_cv, vsz * std::max<int64_t>(slots, 1)) != hipSuccess ||
+      hipMalloc(&d_part, acc * pair * (size_t)ntiles * nt) != hipSuccess ||
+      hipMalloc(&d_sb, sizeof(RowBlock) * sb.size()) != hipSuccess) {
+    release();
+    set_error("hipMalloc failed in the x-tile analysis");
+    return MPH_ERR_OUT_OF_MEMORY;
+  }
+  bool ok = hipMemcpyAsync(d_row, trow_id.data(), sizeof(int32_t) * nt, hipMemcpyHostToDevice, s) == hipSuccess &&
+            hipMemcpyAsync(d_lro, lro.data(), sizeof(int32_t) * lro.size(), hipMemcpyHostToDevice, s) == hipSuccess &&
+            hipMemcpyAsync(d_ss, sub_start.data(), sizeof(int32_t) * sub_start.size(), hipMemcpyHostToDevice, s) == hipSuccess &&
+            hipMemcpyAsync(d_tb, tb.data(), sizeof(int32_t) * tb.size(), hipMemcpyHostToDevice, s) == hipSuccess &&
+            hipMemcpyAsync(d_rs, run_sub.data(), sizeof(int32_t) * run_sub.size(), hipMemcpyHostToDevice, s) == hipSuccess &&
+            hipMemcpyAsync(d_sub, subs.data(), sizeof(XTSub) * subs.size(), hipMemcpyHostToDevice, s) == hipSuccess &&
+            hipMemcpyAsync(d_sb, sb.data(), sizeof(RowBlock) * sb.size(), hipMemcpyHostToDevice, s) == hipSuccess;
+  if (ok) {
+    if (A->vt == MPH_F64)
+      launch_xtile_copy<double>(d_tb, nt, ntiles, gr, tile, d_lro, d_rs, d_sub, d_ci, d_val, d_col,
+                                     d_cv, s);
+    else if (A->vt == MPH_F32)
+      launch_xtile_copy<float>(d_tb, nt, ntiles, gr, tile, d_lro, d_rs, d_sub, d_ci, d_val, d_col,
+                                     d_cv, s);
+    else
+      launch_xtile_copy<uint16_t>(d_tb, nt, ntiles, gr, tile, d_lro, d_rs, d_sub, d_ci, d_val, d_col,
+                                     d_cv, s);
+    ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(s) == hipSuccess;
+  }
+  if (!ok) {
+    release();
+    set_error("x-tile copy failed");
+    return MPH_ERR_HIP;
+  }
+  dfree(d_tb);
+  dfree(d_rs);
+  // the x tile plus the sub-block products exceed the default 64 KiB of LDS
+  {
+    hipError_t e = hipSuccess;
+    const size_t xlds = xtile_lds_bytes(vec_bytes, acc, tile);
+    if (A->vt == MPH_F64)
+      e = hipFuncSetAttribute(reinterpret_cast<const void*>(&xtile_kernel<double, false>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)xlds);
+    else if (A->vt == MPH_F32)
+      e = hipFuncSetAttribute(reinterpret_cast<const void*>(&xtile_kernel<float, true>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)xlds);
+    else
+      e = hipFuncSetAttribute(reinterpret_cast<const void*>(&xtile_kernel<bf16, true>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)xlds);
+    if (e != hipSuccess) {
+      release();
+      set_error(std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
+      return MPH_ERR_HIP;
+    }
+  }
+  p.sblocks = d_sb;
+  p.nsblocks = (int32_t)sb.size() - 1;
+  p.nchunked = nchunked;
+  p.xt_row = d_row;
+  p.xt_lro = d_lro;
+  p.xt_sub = d_sub;
+  p.xt_sub_start = d_ss;
+  p.xt_col = d_col;
+  p.xt_val = d_cv;
+  p.xt_partial = d_part;
+  p.xt_tile = tile;
+  p.xt_ntiles = ntiles;
+  p.xt_groups = ngroups;
+  p.xt_gr = gr;
+  p.xt_rows = nt;
+  return MPH_OK;
+}
+
+}  // namespace
+
 // Builds the row blocks of A's CSR-shaped part (CSR, HDC's CSR part or the
 // HYB tail viewed through offsets built here).
 int rowblock_analyse(mph_matrix_impl* A, hipStream_t s) {
@@ -580,6 +1271,7 @@ int rowblock_analyse(mph_matrix_impl* A, hipStream_t s) {
   dfree(cnt);
   dfree(first);
   dfree(tmp);
+  if (st == MPH_OK) st = xtile_analyse(A, ro, s);
   return st;
 }
 
@@ -628,6 +1320,28 @@ int launch_spmv(mph_matrix_impl* A, const void* x, void* y, int init,
   const size_t lds = sizeof(Acc) * (2 * (size_t)a.chunk + (PRE ? kRowBlockMaxRows : 0)) +
                      sizeof(int32_t) * (kRowBlockMaxRows + 1);
   constexpr bool comp = std::is_same<Acc, float>::value;
+  const RowBlockPlan& p = A->rb;
+  if (p.xt_rows > 0) {
+    RBArgs<V> sa = a;
+    sa.blocks = p.sblocks;
+    sa.nblocks = p.nsblocks;
+    if (p.nsblocks > 0)
+      hipLaunchKernelGGL((rowblock_spmv_kernel<V, PRE, comp>), dim3((unsigned)p.nsblocks),
+                         dim3(kRowBlockThreads), lds, s, sa);
+    if (p.nchunked > 0)
+      hipLaunchKernelGGL((rowblock_combine_kernel<V, PRE, comp>),
+                         dim3((unsigned)ceil_div(p.nsblocks, 256)), dim3(256), 0, s, sa);
+    XTArgs t{p.xt_tile, p.xt_ntiles, p.xt_groups, p.xt_gr, p.xt_rows, p.xt_row, p.xt_lro,
+             p.xt_sub, p.xt_sub_start, p.xt_col, p.xt_val, p.xt_partial};
+    const int64_t tiles8 = ceil_div((int64_t)p.xt_ntiles, (int64_t)kNumXcd) * kNumXcd;
+    const size_t xlds = xtile_lds_bytes(sizeof(typename vtraits<V>::vec), sizeof(Acc), p.xt_tile);
+    hipLaunchKernelGGL((xtile_kernel<V, comp>), dim3((unsigned)(tiles8 * p.xt_groups)),
+                       dim3(kXTileThreads), xlds, s, a, t);
+    hipLaunchKernelGGL((xtile_combine_kernel<V, PRE, comp>),
+                       dim3((unsigned)ceil_div(p.xt_rows, kXCombRows)), dim3(256), 0, s, a, t);
+    MPH_HIP_CHECK(hipGetLastError());
+    return MPH_OK;
+  }
   hipLaunchKernelGGL((rowblock_spmv_kernel<V, PRE, comp>),
                      dim3((unsigned)A->rb.nblocks), dim3(kRowBlockThreads), lds,
                      s, a);

@@ -97,15 +97,15 @@ class DeviceMatrix:
     def analyse(self, stream=None):
         capi.call("mph_matrix_analyse", self.handle, stream_handle(stream))
 
-    def set_tuning(self, rows_chunk=0, lanes=0, coo_items=0, rows_per_thread=0):
-        t = capi.Tuning(rows_chunk, lanes, coo_items, rows_per_thread)
+    def set_tuning(self, rows_chunk=0, lanes=0, coo_items=0, rows_per_thread=0, xtile=0):
+        t = capi.Tuning(rows_chunk, lanes, coo_items, rows_per_thread, xtile)
         capi.call("mph_matrix_set_tuning", self.handle, C.byref(t))
 
     def get_tuning(self):
         t = capi.Tuning()
         capi.call("mph_matrix_get_tuning", self.handle, C.byref(t))
         return dict(rows_chunk=t.rows_chunk, lanes=t.lanes, coo_items=t.coo_items,
-                    rows_per_thread=t.rows_per_thread)
+                    rows_per_thread=t.rows_per_thread, xtile=t.xtile)
 
     # -------------------------------------------------------------- products
     def multiply(self, x, y, init=True, stream=None):

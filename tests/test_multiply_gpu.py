@@ -131,13 +131,86 @@ def test_deterministic_bitwise(gpu, powerlaw, name):
 @pytest.mark.parametrize("tuning", [dict(lanes=1), dict(lanes=8), dict(lanes=64),
                                     dict(rows_chunk=256), dict(rows_chunk=2048),
                                     dict(coo_items=4), dict(coo_items=16),
-                                    dict(rows_per_thread=2)])
+                                    dict(rows_per_thread=2), dict(xtile=1), dict(xtile=2),
+                                    dict(xtile=2, rows_chunk=256)])
 @pytest.mark.parametrize("name", ["CSR", "COO", "HYB", "HDC", "DIA", "ELL"])
 def test_tuning_variants(gpu, name, tuning):
     A = gen.powerlaw_rows(5000, 5000, 1.5, 10.0, 3000, seed=5) if name in ("CSR", "COO", "HYB", "HDC") \
         else gen.stencil3d_7(16)
     x = gen.random_vector(A.ncols, 3)
     check_fp64(A, as_format(A, name), x, tuning=tuning)
+
+
+# ------------------------------------------------------------ x-tiled long rows
+@pytest.fixture(scope="module")
+def wide():
+    """x of 3.2 MB (FP64): the long rows take the x-tiled pass by default."""
+    return gen.powerlaw_rows(20000, 400000, 1.5, 40.0, 30000, seed=31)
+
+
+@pytest.mark.parametrize("name", ["CSR", "HYB", "HDC"])
+@pytest.mark.parametrize("init", [True, False])
+def test_xtile_long_rows(gpu, wide, name, init):
+    A = wide
+    assert np.diff(A.row_offsets).max() > 2048
+    x = gen.random_vector(A.ncols, 4)
+    y0 = gen.random_vector(A.nrows, 5)
+    check_fp64(A, as_format(A, name), x, y0, init)
+
+
+@pytest.mark.parametrize("name", ["CSR", "HYB", "HDC"])
+def test_xtile_forced_and_deterministic(gpu, powerlaw, name):
+    Ah = as_format(powerlaw, name)
+    x = gen.random_vector(powerlaw.ncols, 6)
+    a = check_fp64(powerlaw, Ah, x, tuning=dict(xtile=2))
+    b = gpu_spmv(Ah, x, tuning=dict(xtile=2))
+    assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+@pytest.mark.parametrize("dtype,tol", [("f32", 1e-5), ("bf16", 1e-2)])
+def test_xtile_reduced_precision(gpu, wide, dtype, tol):
+    A = wide
+    x = gen.random_vector(A.ncols, 7)
+    y = gpu_spmv(A, x, dtype)
+    ref = oracle.spmv_compensated(A, x)
+    assert max_rel_err(y, ref, oracle.absrow(A, x)) <= tol
+
+
+def test_xtile_unsorted_columns(gpu):
+    """Long rows whose columns are not sorted keep the position chunks."""
+    A = gen.powerlaw_rows(4000, 300000, 1.5, 30.0, 6000, seed=8)
+    rng = np.random.default_rng(1)
+    ci = A.column_indices.copy()
+    v = A.values.copy()
+    for r in range(A.nrows):
+        a, b = A.row_offsets[r], A.row_offsets[r + 1]
+        perm = rng.permutation(b - a)
+        ci[a:b] = ci[a:b][perm]
+        v[a:b] = v[a:b][perm]
+    U = Csr(A.nrows, A.ncols, A.row_offsets, ci, v)
+    x = gen.random_vector(U.ncols, 2)
+    check_fp64(U, U, x, tuning=dict(xtile=2))
+
+
+def test_xtile_plan_keeps_spmm(gpu, wide):
+    """k > 1 products keep the position-chunk plan of the same handle."""
+    torch = _torch()
+    from morpheus_amd import device
+    A = wide
+    k = 3
+    X = np.stack([gen.random_vector(A.ncols, 10 + j) for j in range(k)], axis=1)
+    D = device.to_device(A, "f64")
+    Xt = torch.from_numpy(np.ascontiguousarray(X)).cuda()
+    Y = torch.full((A.nrows, k), float("nan"), dtype=torch.float64, device="cuda")
+    y1 = torch.full((A.nrows,), float("nan"), dtype=torch.float64, device="cuda")
+    D.multiply(Xt[:, 0].contiguous(), y1)
+    D.multiply_multi(Xt, Y)
+    torch.cuda.synchronize()
+    for j in range(k):
+        ref = oracle.spmv(A, X[:, j])
+        bound = fp64_bound(np.diff(A.row_offsets), oracle.absrow(A, X[:, j]))
+        assert np.all(np.abs(Y[:, j].cpu().numpy() - ref) <= bound)
+    D.close()
 
 
 # ------------------------------------------------------------ edge cases
