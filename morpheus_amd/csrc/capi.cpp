@@ -38,17 +38,13 @@ void release_plans(mph_matrix_impl* A) {
   if (p.partial) dfree(p.partial);
   if (p.tail_ro) dfree(p.tail_ro);
   if (p.sblocks) dfree(p.sblocks);
-  if (p.xt_row) dfree(p.xt_row);
-  if (p.xt_lro) dfree(p.xt_lro);
-  if (p.xt_sub) dfree(p.xt_sub);
-  if (p.xt_sub_start) dfree(p.xt_sub_start);
-  if (p.xt_col) dfree(p.xt_col);
-  if (p.xt_val) dfree(p.xt_val);
-  if (p.xt_partial) dfree(p.xt_partial);
+  xtile_release(p.xt);
   p = RowBlockPlan{};
   CooPlan& c = A->coo;
   if (c.first_val) dfree(c.first_val);
   if (c.last_val) dfree(c.last_val);
+  if (c.chunks) dfree(c.chunks);
+  xtile_release(c.xt);
   c = CooPlan{};
   A->analysed = false;
 }

@@ -40,6 +40,22 @@ constexpr int kXTileRows = 64;                 // rows of a sub-block
 constexpr int kXTileMaxGroupRows = 1024;       // rows of a group (run offsets in LDS)
 constexpr int64_t kXTileMinXBytes = 2 << 20;  // auto: x larger than this
 
+// The x-tiled pass of one matrix's long rows (see xtile_kernel).
+struct XTilePlan {
+  int32_t tile = 0;              // x entries per tile
+  int32_t ntiles = 0;
+  int32_t groups = 0;
+  int32_t gr = 0;                // long rows per group
+  int32_t rows = 0;              // tiled long rows (dealt to groups by length rank)
+  int32_t* row = nullptr;        // [rows] row ids
+  int32_t* lro = nullptr;        // [(groups*ntiles) * (gr+1)] item offsets of each row's run
+  XTSub* sub = nullptr;          // sub-blocks, a sentinel after each (group, tile)
+  int32_t* sub_start = nullptr;  // [groups*ntiles + 1] first sub-block of (group, tile)
+  uint16_t* col = nullptr;       // tile-major copy: column - tile * tile width
+  void* val = nullptr;           // tile-major copy: values
+  void* partial = nullptr;       // [ntiles * rows] (pairs in FP32), tile-major
+};
+
 // Analysis of a CSR-shaped item set (CSR itself, the CSR part of HDC, the
 // COO tail of HYB viewed through row offsets built here).
 struct RowBlockPlan {
@@ -49,25 +65,19 @@ struct RowBlockPlan {
   void* partial = nullptr;       // per-block accumulators (long-row chunks)
   int32_t* tail_ro = nullptr;    // HYB only: row offsets of the COO tail
   int32_t nlong = 0;             // rows split into chunks
-  // x-tiled long rows (SpMV); xt_rows == 0: long rows use position chunks
-  int32_t xt_tile = 0;           // x entries per tile
-  int32_t xt_ntiles = 0;
-  int32_t xt_groups = 0;
-  int32_t xt_gr = 0;             // long rows per group
-  int32_t xt_rows = 0;           // tiled long rows (longest first)
-  int32_t* xt_row = nullptr;     // [xt_rows] row ids
-  int32_t* xt_lro = nullptr;     // [(groups*ntiles) * (gr+1)] item offsets of each row's run
-  XTSub* xt_sub = nullptr;       // sub-blocks, a sentinel after each (group, tile)
-  int32_t* xt_sub_start = nullptr;  // [groups*ntiles + 1] first sub-block of (group, tile)
-  uint16_t* xt_col = nullptr;    // tile-major copy: column - tile * xt_tile
-  void* xt_val = nullptr;        // tile-major copy: values
-  void* xt_partial = nullptr;    // [ntiles * xt_rows] (pairs in FP32), tile-major
+  // x-tiled long rows (SpMV); xt.rows == 0: long rows use position chunks
+  XTilePlan xt;
   RowBlock* sblocks = nullptr;   // short blocks, {row, -2} per tiled row, chunks of the rest
   int32_t nsblocks = 0;
   int32_t nchunked = 0;          // long rows left to position chunks
 };
 
 // Analysis of a row-sorted COO matrix for the segmented reduction.
+// A COO wave chunk: entries [s, e).
+struct CooChunk {
+  int32_t s, e;
+};
+
 struct CooPlan {
   int items = 8;          // entries per lane
   int64_t nchunks = 0;    // wave chunks of 64*items entries
@@ -76,6 +86,10 @@ struct CooPlan {
   void* last_val = nullptr;    // nchunks accumulators
   bool sorted = true;          // rows non-decreasing
   bool prezero = false;        // y cleared by a fill pass instead of gap writes
+  // with x-tiled long rows, the segmented pass covers the other entries only,
+  // through an explicit chunk table
+  XTilePlan xt;
+  CooChunk* chunks = nullptr;  // [nchunks], nullptr: chunk c = [c*64*items, ...)
 };
 
 struct mph_matrix_impl {
@@ -144,6 +158,22 @@ int check_row_offsets(const int32_t* ro, int64_t nrows, int64_t nnz,
                       hipStream_t s, const char* what);
 int validate_rowblock_plan(const int32_t* d_ro, int64_t nrows, RowBlockPlan& p,
                            hipStream_t s);
+
+// x-tiled long rows.  xtile_build plans the candidate rows `lrows` (host row
+// offsets `ro`, device row offsets / columns / values) when the x-tile mode
+// and x size call for it; rows with unsorted columns cancel the plan and rows
+// with a run longer than half a sub-block in some tile are left out.
+// tiled[r] != 0 marks the rows taken.  xtile_multiply runs the pass of a
+// plan without DIA / ELL parts (COO) and writes y of the tiled rows.
+int xtile_build(mph_matrix_impl* A, const std::vector<int32_t>& ro, const int32_t* d_ro,
+                const int32_t* d_ci, const void* d_val, std::vector<int32_t> lrows,
+                XTilePlan& xp, std::vector<char>& tiled, hipStream_t s);
+void xtile_release(XTilePlan& xp);
+int xtile_multiply(mph_matrix_impl* A, const XTilePlan& xp, const void* x, void* y, int init,
+                   hipStream_t s);
+// HYB / COO: row offsets of a row-sorted COO entry list (ro has nrows + 1 entries)
+int coo_row_offsets(const int32_t* ri, int64_t nnz, int64_t nrows, int32_t* ro, hipStream_t s);
+int default_chunk(int vt);
 
 }  // namespace mph
 

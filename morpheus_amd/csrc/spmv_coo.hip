@@ -26,6 +26,9 @@
 //  * FP32 accumulation carries TwoSum-compensated (sum, error) pairs.
 // Unsorted COO (valid for the reference Serial backend) falls back to an
 // atomic scatter, which is correct but not bitwise reproducible.
+#include <algorithm>
+#include <vector>
+
 #include "mph_internal.hpp"
 
 namespace mph {
@@ -85,7 +88,21 @@ struct CooArgs {
   int vec_ok;  // row / column index arrays are 16-byte aligned
   typename vtraits<V>::acc* __restrict__ first_val;
   typename vtraits<V>::acc* __restrict__ last_val;
+  const CooChunk* __restrict__ chunks;  // nullptr: chunk c = [c*64*T, (c+1)*64*T)
 };
+
+template <typename V, int T>
+__device__ __forceinline__ void chunk_range(const CooArgs<V>& a, int64_t c, int64_t& cs,
+                                            int64_t& ce) {
+  if (a.chunks) {
+    const CooChunk k = a.chunks[c];
+    cs = k.s;
+    ce = k.e;
+  } else {
+    cs = c * (int64_t)(kWave * T);
+    ce = min(a.nnz, cs + (int64_t)(kWave * T));
+  }
+}
 
 template <typename Vec, typename A>
 __device__ __forceinline__ void put_y(Vec* y, int64_t r, A v, int init) {
@@ -131,9 +148,8 @@ template <typename V, int T, bool COMP>
 __device__ void coo_chunk(const CooArgs<V>& a, int64_t c, int lane) {
   using A = typename vtraits<V>::acc;
   using S = Sum<A, COMP>;
-  constexpr int CH = kWave * T;
-  const int64_t cs = c * CH;
-  const int64_t ce = min(a.nnz, cs + (int64_t)CH);
+  int64_t cs, ce;
+  chunk_range<V, T>(a, c, cs, ce);
   const int64_t i0 = cs + (int64_t)lane * T;
   const int n = (int)max<int64_t>(0, min<int64_t>(T, ce - i0));
   const bool active = n > 0;
@@ -234,10 +250,10 @@ template <typename V, int T, bool COMP>
 __global__ void coo_fixup_kernel(CooArgs<V> a) {
   using A = typename vtraits<V>::acc;
   using S = Sum<A, COMP>;
-  constexpr int CH = kWave * T;
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.nchunks) return;
-  const int64_t cs = c * CH, ce = min(a.nnz, cs + (int64_t)CH);
+  int64_t cs, ce;
+  chunk_range<V, T>(a, c, cs, ce);
   const int32_t fr = a.ri[cs], lr = a.ri[ce - 1];
   const bool sb = cs > 0 && a.ri[cs - 1] == fr;
   const bool ca = ce < a.nnz && a.ri[ce] == lr;
@@ -245,7 +261,8 @@ __global__ void coo_fixup_kernel(CooArgs<V> a) {
   S tot = S::load(a.last_val, c);
   for (int64_t cc = c + 1; cc < a.nchunks; cc++) {
     tot.merge(S::load(a.first_val, cc));
-    const int64_t s2 = cc * CH, e2 = min(a.nnz, s2 + (int64_t)CH);
+    int64_t s2, e2;
+    chunk_range<V, T>(a, cc, s2, e2);
     const bool whole = a.ri[s2] == a.ri[e2 - 1];
     const bool more = e2 < a.nnz && a.ri[e2] == lr;
     if (!(whole && more)) break;
@@ -292,6 +309,77 @@ namespace {
 constexpr int kGapFillMax = 256;
 }
 
+// Row-sorted COO with long rows and a large x: the long rows (contiguous,
+// column-sorted runs of entries) take the x-tiled pass, which needs only their
+// entry ranges; the segmented pass keeps every other entry through a table of
+// chunks that skip the tiled rows, and y is cleared first (init) since the
+// segmented pass no longer sees every gap between rows.
+int coo_xtile_analyse(mph_matrix_impl* A, hipStream_t s) {
+  CooPlan& p = A->coo;
+  const int64_t nrows = A->nrows, nnz = A->nnz;
+  const size_t vec_bytes = A->vt == MPH_F64 ? 8 : 4;
+  const int mode = A->tuning.xtile;
+  const bool on = mode == 2 || (mode == 0 && (int64_t)A->ncols * (int64_t)vec_bytes >= kXTileMinXBytes);
+  if (!on || nnz == 0 || nrows == 0 || A->ncols == 0) return MPH_OK;
+  const int64_t long_len = A->tuning.rows_chunk > 0 ? A->tuning.rows_chunk : default_chunk(A->vt);
+  int32_t* d_ro = nullptr;
+  MPH_HIP_CHECK(hipMalloc(&d_ro, sizeof(int32_t) * (nrows + 1)));
+  std::vector<int32_t> ro(nrows + 1);
+  int st = coo_row_offsets(A->coo_ri, nnz, nrows, d_ro, s);
+  if (st == MPH_OK &&
+      (hipMemcpyAsync(ro.data(), d_ro, sizeof(int32_t) * (nrows + 1), hipMemcpyDeviceToHost, s) !=
+           hipSuccess ||
+       hipStreamSynchronize(s) != hipSuccess)) {
+    set_error("COO row offsets copy failed");
+    st = MPH_ERR_HIP;
+  }
+  std::vector<int32_t> lrows;
+  std::vector<char> tiled;
+  if (st == MPH_OK) {
+    if (ro[0] != 0 || ro[nrows] != nnz) {
+      set_error("COO analysis check failed: row offsets");
+      st = MPH_ERR_RUNTIME;
+    }
+    for (int64_t r = 0; r < nrows && st == MPH_OK; r++) {
+      if (ro[r + 1] < ro[r]) {
+        set_error("COO analysis check failed: row offsets not monotone");
+        st = MPH_ERR_RUNTIME;
+      } else if (ro[r + 1] - ro[r] > long_len) {
+        lrows.push_back((int32_t)r);
+      }
+    }
+  }
+  if (st == MPH_OK && !lrows.empty())
+    st = xtile_build(A, ro, d_ro, A->coo_ci, A->coo_v, lrows, p.xt, tiled, s);
+  dfree(d_ro);
+  if (st != MPH_OK || p.xt.rows == 0) return st;
+
+  // chunks of the entries outside the tiled rows
+  const int64_t CH = (int64_t)kWave * p.items;
+  std::vector<CooChunk> ch;
+  auto cut = [&](int64_t a, int64_t b) {
+    for (int64_t c = a; c < b; c += CH) ch.push_back(CooChunk{(int32_t)c, (int32_t)std::min(c + CH, b)});
+  };
+  int64_t start = 0;
+  for (int64_t r = 0; r < nrows; r++) {
+    if (tiled[r]) {
+      cut(start, ro[r]);
+      start = ro[r + 1];
+    }
+  }
+  cut(start, nnz);
+  p.nchunks = (int64_t)ch.size();
+  p.nblocks = (int32_t)ceil_div(p.nchunks, (int64_t)(256 / kWave));
+  p.prezero = true;
+  if (!ch.empty()) {
+    MPH_HIP_CHECK(hipMalloc(&p.chunks, sizeof(CooChunk) * ch.size()));
+    MPH_HIP_CHECK(hipMemcpyAsync(p.chunks, ch.data(), sizeof(CooChunk) * ch.size(),
+                                 hipMemcpyHostToDevice, s));
+    MPH_HIP_CHECK(hipStreamSynchronize(s));
+  }
+  return MPH_OK;
+}
+
 int coo_analyse(mph_matrix_impl* A, hipStream_t s) {
   CooPlan& p = A->coo;
   p.items = A->tuning.coo_items > 0 ? A->tuning.coo_items : 8;
@@ -318,6 +406,7 @@ int coo_analyse(mph_matrix_impl* A, hipStream_t s) {
     dfree(g);
     if (st != MPH_OK) return st;
   }
+  if (sorted) MPH_CHECK_STATUS(coo_xtile_analyse(A, s));
   const size_t acc = A->vt == MPH_F64 ? 8 : 4;
   const size_t pair = A->vt == MPH_F64 ? 1 : 2;  // compensated pairs in FP32
   const size_t n = (size_t)std::max<int64_t>(p.nchunks, 1);
@@ -346,6 +435,7 @@ CooArgs<V> coo_args(mph_matrix_impl* A, const void* x, void* y, int init) {
              (reinterpret_cast<uintptr_t>(A->coo_ci) % 16 == 0);
   a.first_val = static_cast<typename vtraits<V>::acc*>(A->coo.first_val);
   a.last_val = static_cast<typename vtraits<V>::acc*>(A->coo.last_val);
+  a.chunks = A->coo.chunks;
   return a;
 }
 
@@ -369,6 +459,8 @@ int coo_run(mph_matrix_impl* A, const void* x, void* y, int init,
     MPH_HIP_CHECK(hipGetLastError());
     return MPH_OK;
   }
+  if (p.xt.rows > 0) MPH_CHECK_STATUS(xtile_multiply(A, p.xt, x, y, init, s));
+  if (p.nchunks == 0) return MPH_OK;
   const dim3 grid((unsigned)p.nblocks), block(256);
   const dim3 fgrid((unsigned)ceil_div(p.nchunks, 256));
   switch (p.items) {

@@ -351,6 +351,11 @@ struct XTArgs {
 
 // Items of one sub-block as loaded by a wave: lane l holds the kBatch
 // consecutive items l*kBatch .. l*kBatch + kBatch - 1.
+inline XTArgs xt_args(const XTilePlan& p) {
+  return XTArgs{p.tile, p.ntiles, p.groups, p.gr, p.rows, p.row, p.lro,
+                p.sub, p.sub_start, p.col, p.val, p.partial};
+}
+
 template <typename V>
 struct XTLoad {
   uint32_t c[kBatch];
@@ -868,11 +873,11 @@ __global__ void ell_tail_check_kernel(const int32_t* __restrict__ ci,
 }
 
 // --------------------------------------------------------------- host side
-namespace {
-
 int default_chunk(int vt) {
   return vt == MPH_F64 ? kDefaultChunkF64 : kDefaultChunkF32;
 }
+
+namespace {
 
 int read_flag(int32_t* dflag, hipStream_t s, int* out) {
   int32_t h = 0;
@@ -935,31 +940,18 @@ void launch_xtile_copy(const int32_t* bnd, int32_t nl, int32_t ntiles, int32_t g
                      static_cast<const V*>(val), col_out, static_cast<V*>(val_out));
 }
 
+}  // namespace
+
 // The x-tiled plan of the long rows (see xtile_kernel), built on the host
-// from the validated row blocks and the per-row tile bounds (computed on the
-// device and checked here), then the tile-major copy is made on the device.
-// Rows with unsorted columns leave the plan off; rows with a run longer than
-// kXTileCap / 2 items in some tile stay with the position chunks.
-int xtile_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
-  RowBlockPlan& p = A->rb;
+// from the per-row tile bounds (computed on the device and checked here),
+// then the tile-major copy is made on the device.
+int xtile_build(mph_matrix_impl* A, const std::vector<int32_t>& ro, const int32_t* d_ro,
+                const int32_t* d_ci, const void* d_val, std::vector<int32_t> lrows,
+                XTilePlan& xp, std::vector<char>& tiled, hipStream_t s) {
   const int64_t nrows = A->nrows;
   const size_t vec_bytes = A->vt == MPH_F64 ? 8 : 4;
-  const int mode = A->tuning.xtile;
-  const bool on = mode == 2 || (mode == 0 && (int64_t)A->ncols * (int64_t)vec_bytes >= kXTileMinXBytes);
-  if (!on || p.nlong == 0 || nrows == 0 || A->ncols == 0) return MPH_OK;
-  const int32_t* d_ci = A->format == MPH_HYB ? A->coo_ci : A->csr_ci;
-  const void* d_val = A->format == MPH_HYB ? A->coo_v : A->csr_v;
-
-  std::vector<int32_t> ro(nrows + 1);
-  std::vector<RowBlock> bl(p.nblocks + 1);
-  MPH_HIP_CHECK(hipMemcpyAsync(ro.data(), d_ro, sizeof(int32_t) * (nrows + 1),
-                               hipMemcpyDeviceToHost, s));
-  MPH_HIP_CHECK(hipMemcpyAsync(bl.data(), p.blocks, sizeof(RowBlock) * (p.nblocks + 1),
-                               hipMemcpyDeviceToHost, s));
-  MPH_HIP_CHECK(hipStreamSynchronize(s));
-  std::vector<int32_t> lrows;
-  for (int64_t b = 0; b < p.nblocks; b++)
-    if (bl[b].sub == 0) lrows.push_back(bl[b].row);
+  tiled.assign(nrows, 0);
+  if (lrows.empty() || nrows == 0 || A->ncols == 0) return MPH_OK;
   // longest first, so neighbouring rows of a sub-block have similar runs
   std::stable_sort(lrows.begin(), lrows.end(), [&](int32_t u, int32_t v) {
     return ro[u + 1] - ro[u] > ro[v + 1] - ro[v];
@@ -1019,7 +1011,6 @@ int xtile_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
   }
   // tiled rows: every run fits half a sub-block
   std::vector<int32_t> trow;     // indices into lrows
-  std::vector<char> tiled(nrows, 0);
   for (int32_t i = 0; i < nl; i++) {
     int32_t mx = 0;
     for (int32_t q = 0; q < ntiles; q++) mx = std::max(mx, B(q + 1, i) - B(q, i));
@@ -1102,22 +1093,6 @@ int xtile_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
   std::vector<int32_t> trow_id(nt);
   for (int32_t j = 0; j < nt; j++) trow_id[j] = lrows[trow[j]];
 
-  // short blocks with the tiled rows skipped; other long rows keep their chunks
-  std::vector<RowBlock> sb;
-  int32_t nchunked = 0;
-  for (int64_t b = 0; b < p.nblocks; b++) {
-    const RowBlock k = bl[b];
-    if (k.sub < 0) {
-      sb.push_back(k);
-    } else if (tiled[k.row]) {
-      if (k.sub == 0) sb.push_back(RowBlock{k.row, -2});
-    } else {
-      sb.push_back(k);
-      if (k.sub == 0) nchunked++;
-    }
-  }
-  sb.push_back(RowBlock{(int32_t)nrows, -1});
-
   // device side
   const size_t vsz = A->vt == MPH_F64 ? 8 : A->vt == MPH_F32 ? 4 : 2;
   const size_t acc = A->vt == MPH_F64 ? 8 : 4, pair = A->vt == MPH_F64 ? 1 : 2;
@@ -1125,7 +1100,6 @@ int xtile_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
   XTSub* d_sub = nullptr;
   uint16_t* d_col = nullptr;
   void *d_cv = nullptr, *d_part = nullptr;
-  RowBlock* d_sb = nullptr;
   auto release = [&]() {
     dfree(d_row);
     dfree(d_lro);
@@ -1136,7 +1110,6 @@ int xtile_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
     dfree(d_col);
     dfree(d_cv);
     dfree(d_part);
-    dfree(d_sb);
   };
   if (hipMalloc(&d_row, sizeof(int32_t) * nt) != hipSuccess ||
       hipMalloc(&d_lro, sizeof(int32_t) * lro.size()) != hipSuccess ||
@@ -1146,8 +1119,7 @@ int xtile_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
       hipMalloc(&d_sub, sizeof(XTSub) * subs.size()) != hipSuccess ||
       hipMalloc(&d_col, sizeof(uint16_t) * std::max<int64_t>(slots, 1)) != hipSuccess ||
       hipMalloc(&d_cv, vsz * std::max<int64_t>(slots, 1)) != hipSuccess ||
-      hipMalloc(&d_part, acc * pair * (size_t)ntiles * nt) != hipSuccess ||
-      hipMalloc(&d_sb, sizeof(RowBlock) * sb.size()) != hipSuccess) {
+      hipMalloc(&d_part, acc * pair * (size_t)ntiles * nt) != hipSuccess) {
     release();
     set_error("hipMalloc failed in the x-tile analysis");
     return MPH_ERR_OUT_OF_MEMORY;
@@ -1157,8 +1129,7 @@ int xtile_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
             hipMemcpyAsync(d_ss, sub_start.data(), sizeof(int32_t) * sub_start.size(), hipMemcpyHostToDevice, s) == hipSuccess &&
             hipMemcpyAsync(d_tb, tb.data(), sizeof(int32_t) * tb.size(), hipMemcpyHostToDevice, s) == hipSuccess &&
             hipMemcpyAsync(d_rs, run_sub.data(), sizeof(int32_t) * run_sub.size(), hipMemcpyHostToDevice, s) == hipSuccess &&
-            hipMemcpyAsync(d_sub, subs.data(), sizeof(XTSub) * subs.size(), hipMemcpyHostToDevice, s) == hipSuccess &&
-            hipMemcpyAsync(d_sb, sb.data(), sizeof(RowBlock) * sb.size(), hipMemcpyHostToDevice, s) == hipSuccess;
+            hipMemcpyAsync(d_sub, subs.data(), sizeof(XTSub) * subs.size(), hipMemcpyHostToDevice, s) == hipSuccess;
   if (ok) {
     if (A->vt == MPH_F64)
       launch_xtile_copy<double>(d_tb, nt, ntiles, gr, tile, d_lro, d_rs, d_sub, d_ci, d_val, d_col,
@@ -1197,21 +1168,92 @@ int xtile_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
       return MPH_ERR_HIP;
     }
   }
+  xp.row = d_row;
+  xp.lro = d_lro;
+  xp.sub = d_sub;
+  xp.sub_start = d_ss;
+  xp.col = d_col;
+  xp.val = d_cv;
+  xp.partial = d_part;
+  xp.tile = tile;
+  xp.ntiles = ntiles;
+  xp.groups = ngroups;
+  xp.gr = gr;
+  xp.rows = nt;
+  return MPH_OK;
+}
+
+void xtile_release(XTilePlan& xp) {
+  dfree(xp.row);
+  dfree(xp.lro);
+  dfree(xp.sub);
+  dfree(xp.sub_start);
+  dfree(xp.col);
+  dfree(xp.val);
+  dfree(xp.partial);
+  xp = XTilePlan{};
+}
+
+namespace {
+
+// CSR / HYB / HDC: the long rows of the row-block plan go to the x-tiled
+// pass; the short-block launch skips them ({row, -2}) and keeps the position
+// chunks of long rows the pass does not take.
+int xtile_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
+  RowBlockPlan& p = A->rb;
+  const int64_t nrows = A->nrows;
+  const size_t vec_bytes = A->vt == MPH_F64 ? 8 : 4;
+  const int mode = A->tuning.xtile;
+  const bool on = mode == 2 || (mode == 0 && (int64_t)A->ncols * (int64_t)vec_bytes >= kXTileMinXBytes);
+  if (!on || p.nlong == 0 || nrows == 0 || A->ncols == 0) return MPH_OK;
+  const int32_t* d_ci = A->format == MPH_HYB ? A->coo_ci : A->csr_ci;
+  const void* d_val = A->format == MPH_HYB ? A->coo_v : A->csr_v;
+
+  std::vector<int32_t> ro(nrows + 1);
+  std::vector<RowBlock> bl(p.nblocks + 1);
+  MPH_HIP_CHECK(hipMemcpyAsync(ro.data(), d_ro, sizeof(int32_t) * (nrows + 1),
+                               hipMemcpyDeviceToHost, s));
+  MPH_HIP_CHECK(hipMemcpyAsync(bl.data(), p.blocks, sizeof(RowBlock) * (p.nblocks + 1),
+                               hipMemcpyDeviceToHost, s));
+  MPH_HIP_CHECK(hipStreamSynchronize(s));
+  std::vector<int32_t> lrows;
+  for (int64_t b = 0; b < p.nblocks; b++)
+    if (bl[b].sub == 0) lrows.push_back(bl[b].row);
+  std::vector<char> tiled;
+  MPH_CHECK_STATUS(xtile_build(A, ro, d_ro, d_ci, d_val, lrows, p.xt, tiled, s));
+  if (p.xt.rows == 0) return MPH_OK;
+
+  std::vector<RowBlock> sb;
+  int32_t nchunked = 0;
+  for (int64_t b = 0; b < p.nblocks; b++) {
+    const RowBlock k = bl[b];
+    if (k.sub < 0) {
+      sb.push_back(k);
+    } else if (tiled[k.row]) {
+      if (k.sub == 0) sb.push_back(RowBlock{k.row, -2});
+    } else {
+      sb.push_back(k);
+      if (k.sub == 0) nchunked++;
+    }
+  }
+  sb.push_back(RowBlock{(int32_t)nrows, -1});
+  RowBlock* d_sb = nullptr;
+  if (hipMalloc(&d_sb, sizeof(RowBlock) * sb.size()) != hipSuccess) {
+    xtile_release(p.xt);
+    set_error("hipMalloc failed in the x-tile analysis");
+    return MPH_ERR_OUT_OF_MEMORY;
+  }
+  if (hipMemcpyAsync(d_sb, sb.data(), sizeof(RowBlock) * sb.size(), hipMemcpyHostToDevice, s) !=
+          hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    dfree(d_sb);
+    xtile_release(p.xt);
+    set_error("x-tile analysis: block upload failed");
+    return MPH_ERR_HIP;
+  }
   p.sblocks = d_sb;
   p.nsblocks = (int32_t)sb.size() - 1;
   p.nchunked = nchunked;
-  p.xt_row = d_row;
-  p.xt_lro = d_lro;
-  p.xt_sub = d_sub;
-  p.xt_sub_start = d_ss;
-  p.xt_col = d_col;
-  p.xt_val = d_cv;
-  p.xt_partial = d_part;
-  p.xt_tile = tile;
-  p.xt_ntiles = ntiles;
-  p.xt_groups = ngroups;
-  p.xt_gr = gr;
-  p.xt_rows = nt;
   return MPH_OK;
 }
 
@@ -1321,7 +1363,7 @@ int launch_spmv(mph_matrix_impl* A, const void* x, void* y, int init,
                      sizeof(int32_t) * (kRowBlockMaxRows + 1);
   constexpr bool comp = std::is_same<Acc, float>::value;
   const RowBlockPlan& p = A->rb;
-  if (p.xt_rows > 0) {
+  if (p.xt.rows > 0) {
     RBArgs<V> sa = a;
     sa.blocks = p.sblocks;
     sa.nblocks = p.nsblocks;
@@ -1331,14 +1373,13 @@ int launch_spmv(mph_matrix_impl* A, const void* x, void* y, int init,
     if (p.nchunked > 0)
       hipLaunchKernelGGL((rowblock_combine_kernel<V, PRE, comp>),
                          dim3((unsigned)ceil_div(p.nsblocks, 256)), dim3(256), 0, s, sa);
-    XTArgs t{p.xt_tile, p.xt_ntiles, p.xt_groups, p.xt_gr, p.xt_rows, p.xt_row, p.xt_lro,
-             p.xt_sub, p.xt_sub_start, p.xt_col, p.xt_val, p.xt_partial};
-    const int64_t tiles8 = ceil_div((int64_t)p.xt_ntiles, (int64_t)kNumXcd) * kNumXcd;
-    const size_t xlds = xtile_lds_bytes(sizeof(typename vtraits<V>::vec), sizeof(Acc), p.xt_tile);
-    hipLaunchKernelGGL((xtile_kernel<V, comp>), dim3((unsigned)(tiles8 * p.xt_groups)),
+    XTArgs t = xt_args(p.xt);
+    const int64_t tiles8 = ceil_div((int64_t)p.xt.ntiles, (int64_t)kNumXcd) * kNumXcd;
+    const size_t xlds = xtile_lds_bytes(sizeof(typename vtraits<V>::vec), sizeof(Acc), p.xt.tile);
+    hipLaunchKernelGGL((xtile_kernel<V, comp>), dim3((unsigned)(tiles8 * p.xt.groups)),
                        dim3(kXTileThreads), xlds, s, a, t);
     hipLaunchKernelGGL((xtile_combine_kernel<V, PRE, comp>),
-                       dim3((unsigned)ceil_div(p.xt_rows, kXCombRows)), dim3(256), 0, s, a, t);
+                       dim3((unsigned)ceil_div(p.xt.rows, kXCombRows)), dim3(256), 0, s, a, t);
     MPH_HIP_CHECK(hipGetLastError());
     return MPH_OK;
   }
@@ -1423,6 +1464,51 @@ int rowblock_multiply(mph_matrix_impl* A, int k, const void* x, void* y,
     case MPH_F64: return dispatch_pre<double>(A, k, x, y, init, s);
     case MPH_F32: return dispatch_pre<float>(A, k, x, y, init, s);
     case MPH_BF16: return dispatch_pre<bf16>(A, k, x, y, init, s);
+  }
+  return invalid("unknown value type");
+}
+
+int coo_row_offsets(const int32_t* ri, int64_t nnz, int64_t nrows, int32_t* ro, hipStream_t s) {
+  hipLaunchKernelGGL(tail_offsets_kernel, dim3((unsigned)ceil_div(nnz + 1, 256)), dim3(256), 0, s,
+                     ri, nnz, nrows, ro);
+  MPH_HIP_CHECK(hipGetLastError());
+  return MPH_OK;
+}
+
+namespace {
+
+template <typename V>
+int xtile_multiply_t(mph_matrix_impl* A, const XTilePlan& xp, const void* x, void* y, int init,
+                     hipStream_t s) {
+  using Acc = typename vtraits<V>::acc;
+  constexpr bool comp = std::is_same<Acc, float>::value;
+  RBArgs<V> a{};
+  a.nrows = A->nrows;
+  a.ncols = A->ncols;
+  a.x = static_cast<const typename vtraits<V>::vec*>(x);
+  a.y = static_cast<typename vtraits<V>::vec*>(y);
+  a.ld = 1;
+  a.init = init;
+  XTArgs t = xt_args(xp);
+  const int64_t tiles8 = ceil_div((int64_t)xp.ntiles, (int64_t)kNumXcd) * kNumXcd;
+  const size_t xlds = xtile_lds_bytes(sizeof(typename vtraits<V>::vec), sizeof(Acc), xp.tile);
+  hipLaunchKernelGGL((xtile_kernel<V, comp>), dim3((unsigned)(tiles8 * xp.groups)),
+                     dim3(kXTileThreads), xlds, s, a, t);
+  hipLaunchKernelGGL((xtile_combine_kernel<V, PRE_NONE, comp>),
+                     dim3((unsigned)ceil_div(xp.rows, kXCombRows)), dim3(256), 0, s, a, t);
+  MPH_HIP_CHECK(hipGetLastError());
+  return MPH_OK;
+}
+
+}  // namespace
+
+int xtile_multiply(mph_matrix_impl* A, const XTilePlan& xp, const void* x, void* y, int init,
+                   hipStream_t s) {
+  if (xp.rows == 0) return MPH_OK;
+  switch (A->vt) {
+    case MPH_F64: return xtile_multiply_t<double>(A, xp, x, y, init, s);
+    case MPH_F32: return xtile_multiply_t<float>(A, xp, x, y, init, s);
+    case MPH_BF16: return xtile_multiply_t<bf16>(A, xp, x, y, init, s);
   }
   return invalid("unknown value type");
 }

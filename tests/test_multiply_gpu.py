@@ -148,7 +148,7 @@ def wide():
     return gen.powerlaw_rows(20000, 400000, 1.5, 40.0, 30000, seed=31)
 
 
-@pytest.mark.parametrize("name", ["CSR", "HYB", "HDC"])
+@pytest.mark.parametrize("name", ["CSR", "COO", "HYB", "HDC"])
 @pytest.mark.parametrize("init", [True, False])
 def test_xtile_long_rows(gpu, wide, name, init):
     A = wide
@@ -158,7 +158,7 @@ def test_xtile_long_rows(gpu, wide, name, init):
     check_fp64(A, as_format(A, name), x, y0, init)
 
 
-@pytest.mark.parametrize("name", ["CSR", "HYB", "HDC"])
+@pytest.mark.parametrize("name", ["CSR", "COO", "HYB", "HDC"])
 def test_xtile_forced_and_deterministic(gpu, powerlaw, name):
     Ah = as_format(powerlaw, name)
     x = gen.random_vector(powerlaw.ncols, 6)
@@ -174,6 +174,26 @@ def test_xtile_reduced_precision(gpu, wide, dtype, tol):
     y = gpu_spmv(A, x, dtype)
     ref = oracle.spmv_compensated(A, x)
     assert max_rel_err(y, ref, oracle.absrow(A, x)) <= tol
+
+
+@pytest.mark.parametrize("name", ["CSR", "COO"])
+@pytest.mark.parametrize("init", [True, False])
+def test_xtile_long_rows_at_the_ends(gpu, name, init):
+    """Tiled long rows first and last, empty rows around them: the rows the
+    segmented / short-block passes skip are still cleared or accumulated."""
+    rng = np.random.default_rng(3)
+    n, m = 3000, 300000
+    lens = np.zeros(n, dtype=np.int64)
+    lens[[0, 1, 1500, n - 1]] = [5000, 3000, 4000, 2500]
+    lens[rng.choice(np.arange(10, n - 10), 800, replace=False)] = rng.integers(1, 20, 800)
+    lens[[5, 6, 7, 2000, 2001]] = 0
+    ro = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(lens, out=ro[1:])
+    ci = np.concatenate([np.sort(rng.choice(m, L, replace=False)) for L in lens if L > 0])
+    A = Csr(n, m, ro, ci, rng.standard_normal(ro[-1]))
+    x = gen.random_vector(m, 12)
+    y0 = gen.random_vector(n, 13)
+    check_fp64(A, as_format(A, name), x, y0, init, tuning=dict(xtile=2))
 
 
 def test_xtile_unsorted_columns(gpu):
