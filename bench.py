@@ -47,10 +47,13 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB device copy (PMC traffic calibration)")
+    ap.add_argument("--formats", default="",
+                    help="comma-separated subset of the step's formats (profiling one "
+                         "format at a time; the JSON line then covers that subset)")
     return ap.parse_args()
 
 
-def build_workload():
+def build_workload(only=None):
     import torch
     from morpheus_amd import capi, device
     from morpheus_amd import generators as gen
@@ -79,6 +82,8 @@ def build_workload():
     torch.cuda.synchronize()
     # interleave matrices so consecutive products never share matrix data
     order = ["CSR", "DIA", "COO", "ELL", "HYB", "HDC"]
+    if only:
+        order = [n for n in order if n in only]
     vecs = {}
     for name in order:
         D, host = mats[name]
@@ -293,7 +298,8 @@ def main():
     from morpheus_amd import capi
 
     capi.call("mph_initialize", 0)
-    mats, vecs, order, ell_z15, (z15, s7) = build_workload()
+    only = [f.strip().upper() for f in args.formats.split(",") if f.strip()]
+    mats, vecs, order, ell_z15, (z15, s7) = build_workload(only)
     total_ms, wall, per = run_gpu(mats, vecs, order, args.steps, args.warmup)
     ms_per_step = total_ms / args.steps
     flops_step = sum(2.0 * mats[n][1].nnz for n in order)
@@ -318,11 +324,17 @@ def main():
         log(f"{n} on {formats[n]['matrix']}: {t * 1e3:.1f} us  {formats[n]['gflops']} GFLOP/s  "
             f"{gbs:.0f} GB/s  ({100 * gbs / HBM_PEAK_GBS:.1f}% of 8 TB/s)")
     dom = max(order, key=lambda n: formats[n]["ms"])
+    # kernels one product launches per format on these matrices (Z15's long
+    # rows take the x-tiled pass; S7's HDC has an empty CSR part)
     kernel_names = {
-        "CSR": ["rowblock_spmv_kernel<double, 0, false>", "rowblock_combine_kernel<double, 0, false>"],
-        "COO": ["coo_segmented_kernel<double, 8, false>", "coo_fixup_kernel<double, 8, false>"],
-        "HYB": ["rowblock_spmv_kernel<double, 2, false>", "rowblock_combine_kernel<double, 2, false>"],
-        "HDC": ["dia_kernel<double, 1, 1>"],  # S7's HDC has an empty CSR part
+        "CSR": ["rowblock_spmv_kernel<double, 0, false>", "xtile_kernel<double, false>",
+                "xtile_combine_kernel<double, 0, false>"],
+        "COO": ["coo_zero_kernel<double>", "xtile_kernel<double, false>",
+                "xtile_combine_kernel<double, 0, false>", "coo_segmented_kernel<double, 8, false>",
+                "coo_fixup_kernel<double, 8, false>"],
+        "HYB": ["rowblock_spmv_kernel<double, 2, false>", "xtile_kernel<double, false>",
+                "xtile_combine_kernel<double, 2, false>"],
+        "HDC": ["dia_kernel<double, 1, 1>"],
         "DIA": ["dia_kernel<double, 1, 1>"],
         "ELL": ["ell_kernel<double, 1, 1>"],
     }
@@ -344,11 +356,11 @@ def main():
                          "spmv_f64_six_formats_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            tk = json.load(f)["kernels"]
-        if all(k in tk for k in kernel_names[dom]):
-            roof["traffic"] = int(sum(tk[k]["fetch_bytes"] + tk[k]["write_bytes"]
-                                      for k in kernel_names[dom]))
-            roof["traffic_unit"] = "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE)"
+            tf = json.load(f).get("formats", {})
+        if dom in tf:
+            roof["traffic"] = int(tf[dom]["fetch_bytes"] + tf[dom]["write_bytes"])
+            roof["traffic_unit"] = ("bytes per product (rocprofv3 FETCH_SIZE + WRITE_SIZE over "
+                                    "the format's kernels)")
 
     # practical streaming ceiling on this device: an elementwise kernel that
     # reads 1 GiB and writes 1 GiB (context for the roofline fraction)
