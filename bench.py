@@ -47,13 +47,17 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB device copy (PMC traffic calibration)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the library's default launch shapes (no autotuning)")
+    ap.add_argument("--no-sweep", action="store_true",
+                    help="skip the FP32 / BF16 precision sweep (S7, Z15, B9)")
     ap.add_argument("--formats", default="",
                     help="comma-separated subset of the step's formats (profiling one "
                          "format at a time; the JSON line then covers that subset)")
     return ap.parse_args()
 
 
-def build_workload(only=None):
+def build_workload(only=None, tune=True):
     import torch
     from morpheus_amd import capi, device
     from morpheus_amd import generators as gen
@@ -85,14 +89,20 @@ def build_workload(only=None):
     if only:
         order = [n for n in order if n in only]
     vecs = {}
+    tuned = {}
     for name in order:
         D, host = mats[name]
         x = torch.empty(D.ncols, dtype=torch.float64, device="cuda").uniform_(-1.0, 1.0)
         y = torch.empty(D.nrows, dtype=torch.float64, device="cuda")
         D.analyse()
+        # the launch shape of each handle is tuned once, by timed trials, as
+        # part of its analysis (mph_matrix_autotune)
+        if tune:
+            device.autotune_launch(D, 1, 10)
+            tuned[name] = {k: v for k, v in D.get_tuning().items() if v}
         vecs[name] = (x, y)
     torch.cuda.synchronize()
-    return mats, vecs, order, ell_z15, (z15, s7)
+    return mats, vecs, order, ell_z15, (z15, s7), tuned
 
 
 def run_gpu(mats, vecs, order, steps, warmup):
@@ -276,6 +286,72 @@ def run_batch(steps):
     return res
 
 
+def run_sweep(z15, s7, steps):
+    """Precision sweep: S7, Z15 and B9 (4M rows, bandwidth 9, 2 % off-band)
+    stored in FP32 and BF16 (int32 indices, FP32 accumulation, TwoSum on rows
+    longer than 1k), every format the reference admits for the matrix; per
+    (matrix, format, value type) the product time and the max-row relative
+    error max_i |y_i - y_ref_i| / sum_j |a_ij x_j| against the compensated
+    FP64 oracle on the original FP64 matrix and x."""
+    import numpy as np
+    import torch
+    import oracle  # the checker: FP64 reference products and row scales
+    from morpheus_amd import capi, device
+    from morpheus_amd import generators as gen
+    from morpheus_amd.formats import FORMAT_NAMES
+
+    t0 = time.time()
+    b9 = gen.B9()
+    log(f"sweep: generated B9 (nnz {b9.nnz}) in {time.time() - t0:.1f}s")
+    stream = torch.cuda.current_stream()
+    out = {}
+    for mname, A in (("S7", s7), ("Z15", z15), ("B9", b9)):
+        x = gen.random_vector(A.ncols, 21)
+        yref = oracle.spmv_compensated(A, x)
+        scale = oracle.absrow(A, x)
+        mask = scale > 0
+        out[mname] = {}
+        for dtype in ("f32", "bf16"):
+            base = device.to_device(A, dtype)
+            vdt = device.vector_dtype(dtype)
+            xt = torch.from_numpy(x).cuda().to(vdt)
+            y = torch.empty(A.nrows, dtype=vdt, device="cuda")
+            for f, fname in enumerate(FORMAT_NAMES):
+                try:
+                    D = base if f == base.format else device.convert(base, f)
+                except capi.FormatConversionException:
+                    out[mname].setdefault(fname, {})[dtype] = "FormatConversionException"
+                    continue
+                D.analyse()
+                D.multiply(xt, y, True, stream)
+                torch.cuda.synchronize()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                for _ in range(steps):
+                    D.multiply(xt, y, True, stream)
+                e1.record(stream)
+                torch.cuda.synchronize()
+                ms = e0.elapsed_time(e1) / steps
+                yh = y.double().cpu().numpy()
+                err = float(np.max(np.abs(yh[mask] - yref[mask]) / scale[mask])) if mask.any() else 0.0
+                out[mname].setdefault(fname, {})[dtype] = {
+                    "ms": round(ms, 5),
+                    "gflops": round(2.0 * A.nnz / (ms * 1e-3) / 1e9, 2),
+                    "max_rel_err": err,
+                }
+                if D is not base:
+                    D.close()
+            base.close()
+        log("sweep " + mname + ": " + ", ".join(
+            f"{f} " + "/".join(f"{v['max_rel_err']:.1e}" if isinstance(v, dict) else "n/a"
+                               for v in d.values()) for f, d in out[mname].items()))
+    out["tolerance"] = {"f32": 1e-5, "bf16": 1e-2}
+    out["max_rel_err"] = {dt: max(v[dt]["max_rel_err"] for m in ("S7", "Z15", "B9")
+                                  for v in out[m].values() if isinstance(v.get(dt), dict))
+                          for dt in ("f32", "bf16")}
+    return out
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -299,7 +375,7 @@ def main():
 
     capi.call("mph_initialize", 0)
     only = [f.strip().upper() for f in args.formats.split(",") if f.strip()]
-    mats, vecs, order, ell_z15, (z15, s7) = build_workload(only)
+    mats, vecs, order, ell_z15, (z15, s7), tuned = build_workload(only, not args.no_tune)
     total_ms, wall, per = run_gpu(mats, vecs, order, args.steps, args.warmup)
     ms_per_step = total_ms / args.steps
     flops_step = sum(2.0 * mats[n][1].nnz for n in order)
@@ -324,14 +400,16 @@ def main():
         log(f"{n} on {formats[n]['matrix']}: {t * 1e3:.1f} us  {formats[n]['gflops']} GFLOP/s  "
             f"{gbs:.0f} GB/s  ({100 * gbs / HBM_PEAK_GBS:.1f}% of 8 TB/s)")
     dom = max(order, key=lambda n: formats[n]["ms"])
+    coo_items = tuned.get("COO", {}).get("coo_items", 8)
     # kernels one product launches per format on these matrices (Z15's long
     # rows take the x-tiled pass; S7's HDC has an empty CSR part)
     kernel_names = {
         "CSR": ["rowblock_spmv_kernel<double, 0, false>", "xtile_kernel<double, false>",
                 "xtile_combine_kernel<double, 0, false>"],
         "COO": ["coo_zero_kernel<double>", "xtile_kernel<double, false>",
-                "xtile_combine_kernel<double, 0, false>", "coo_segmented_kernel<double, 8, false>",
-                "coo_fixup_kernel<double, 8, false>"],
+                "xtile_combine_kernel<double, 0, false>",
+                f"coo_segmented_kernel<double, {coo_items}, false>",
+                f"coo_fixup_kernel<double, {coo_items}, false>"],
         "HYB": ["rowblock_spmv_kernel<double, 2, false>", "xtile_kernel<double, false>",
                 "xtile_combine_kernel<double, 2, false>"],
         "HDC": ["dia_kernel<double, 1, 1>"],
@@ -379,6 +457,7 @@ def main():
     log(f"elementwise 1 GiB read + 1 GiB write: {stream_gbs:.0f} GB/s")
 
     batch = None if args.no_batch else run_batch(args.batch_steps)
+    sweep = None if (args.no_sweep or only) else run_sweep(z15, s7, 20)
 
     cpu = None
     if not args.no_cpu:
@@ -424,12 +503,14 @@ def main():
             "index": "int32",
             "ell_on_z15": ell_z15,
             "stream_copy_gbs": round(stream_gbs, 1),
+            "tuning": tuned,
             "wall_s": round(wall, 4),
         },
         "roofline": roof,
         "cpu_baseline": cpu,
         "formats": formats,
         "batched_spmm": batch,
+        "precision_sweep": sweep,
     }
     print(json.dumps(line), flush=True)
 

@@ -140,7 +140,6 @@ int mph_autotune_format(int64_t nrows, int64_t ncols, int64_t nnz, const int32_t
   const int32_t* Ri = (const int32_t*)cri.p;
   const int32_t* Ci = (const int32_t*)cci.p;
   for (int f = 0; f < MPH_NFORMATS; f++) {
-    if (f == MPH_COO && k != 1) continue;  // COO products take one right-hand side
     DevBuf a0, a1, a2, a3, a4, a5;
     mph_matrix_t A = nullptr;
     int st = MPH_OK;

@@ -160,10 +160,9 @@ int multiply_impl(mph_matrix_t A, int k, const void* x, void* y, int init,
   if (st == MPH_ERR_NOT_IMPLEMENTED && A->format == MPH_HYB) {
     // unsorted tail: two passes
     MPH_CHECK_STATUS(ell_multiply(A, k, x, y, init, s));
-    if (k != 1) return MPH_ERR_NOT_IMPLEMENTED;
     mph_matrix_impl t = hyb_tail_view(A);
     st = coo_analyse(&t, s);
-    if (st == MPH_OK) st = coo_multiply(&t, 1, x, y, 0, s);
+    if (st == MPH_OK) st = coo_multiply(&t, k, x, y, 0, s);
     hipStreamSynchronize(s);
     release_plans(&t);
     return st;
@@ -171,8 +170,7 @@ int multiply_impl(mph_matrix_t A, int k, const void* x, void* y, int init,
   if (st != MPH_OK) return st;
   switch (A->format) {
     case MPH_COO:
-      if (k == 1) return coo_multiply(A, 1, x, y, init, s);
-      return MPH_ERR_NOT_IMPLEMENTED;
+      return coo_multiply(A, k, x, y, init, s);
     case MPH_HDC:
       // an HDC whose CSR part is empty is its DIA part
       if (A->csr_nnz == 0) return dia_multiply(A, k, x, y, init, s);

@@ -164,13 +164,14 @@ int validate_rowblock_plan(const int32_t* d_ro, int64_t nrows, RowBlockPlan& p,
 // and x size call for it; rows with unsorted columns cancel the plan and rows
 // with a run longer than half a sub-block in some tile are left out.
 // tiled[r] != 0 marks the rows taken.  xtile_multiply runs the pass of a
-// plan without DIA / ELL parts (COO) and writes y of the tiled rows.
+// plan without DIA / ELL parts (COO) and writes y of the tiled rows (x and y
+// read with element stride ld).
 int xtile_build(mph_matrix_impl* A, const std::vector<int32_t>& ro, const int32_t* d_ro,
                 const int32_t* d_ci, const void* d_val, std::vector<int32_t> lrows,
                 XTilePlan& xp, std::vector<char>& tiled, hipStream_t s);
 void xtile_release(XTilePlan& xp);
 int xtile_multiply(mph_matrix_impl* A, const XTilePlan& xp, const void* x, void* y, int init,
-                   hipStream_t s);
+                   int64_t ld, hipStream_t s);
 // HYB / COO: row offsets of a row-sorted COO entry list (ro has nrows + 1 entries)
 int coo_row_offsets(const int32_t* ri, int64_t nnz, int64_t nrows, int32_t* ro, hipStream_t s);
 int default_chunk(int vt);

@@ -89,6 +89,7 @@ struct CooArgs {
   typename vtraits<V>::acc* __restrict__ first_val;
   typename vtraits<V>::acc* __restrict__ last_val;
   const CooChunk* __restrict__ chunks;  // nullptr: chunk c = [c*64*T, (c+1)*64*T)
+  int64_t ld;                           // element stride of x and y (one column of X / Y)
 };
 
 template <typename V, int T>
@@ -109,11 +110,11 @@ __device__ __forceinline__ void put_y(Vec* y, int64_t r, A v, int init) {
   y[r] = init ? (Vec)v : (Vec)((A)y[r] + v);
 }
 
-__device__ __forceinline__ void zero_rows(double* y, int64_t a, int64_t b) {
-  for (int64_t r = a; r < b; r++) y[r] = 0.0;
+__device__ __forceinline__ void zero_rows(double* y, int64_t a, int64_t b, int64_t ld) {
+  for (int64_t r = a; r < b; r++) y[r * ld] = 0.0;
 }
-__device__ __forceinline__ void zero_rows(float* y, int64_t a, int64_t b) {
-  for (int64_t r = a; r < b; r++) y[r] = 0.0f;
+__device__ __forceinline__ void zero_rows(float* y, int64_t a, int64_t b, int64_t ld) {
+  for (int64_t r = a; r < b; r++) y[r * ld] = 0.0f;
 }
 
 typedef int v4i __attribute__((ext_vector_type(4)));
@@ -167,7 +168,7 @@ __device__ void coo_chunk(const CooArgs<V>& a, int64_t c, int lane) {
     } else if (row == last_row && continues_after) {
       v.store(a.last_val, c);
     } else {
-      put_y(a.y, row, v.value(), a.init);
+      put_y(a.y, row * a.ld, v.value(), a.init);
     }
   };
 
@@ -180,7 +181,7 @@ __device__ void coo_chunk(const CooArgs<V>& a, int64_t c, int lane) {
   // all gathers of the lane are issued before the serial pass consumes them
   A prod[T];
 #pragma unroll
-  for (int k = 0; k < T; k++) prod[k] = k < n ? vals[k] * (A)a.x[cols[k]] : (A)0;
+  for (int k = 0; k < T; k++) prod[k] = k < n ? vals[k] * (A)a.x[(int64_t)cols[k] * a.ld] : (A)0;
 
   // serial pass: head segment, complete middle segments, tail segment
   int32_t h = rows[0], cur = rows[0];
@@ -198,7 +199,7 @@ __device__ void coo_chunk(const CooArgs<V>& a, int64_t c, int lane) {
         } else {
           write_seg(cur, run);  // a middle segment: complete
         }
-        if (fill) zero_rows(a.y, (int64_t)cur + 1, rows[k]);
+        if (fill) zero_rows(a.y, (int64_t)cur + 1, rows[k], a.ld);
         cur = rows[k];
         run = S::zero();
       }
@@ -232,15 +233,15 @@ __device__ void coo_chunk(const CooArgs<V>& a, int64_t c, int lane) {
   // the tail segment ends at this lane's boundary
   if (lane == last_lane) {
     write_seg(t, acc);
-    if (fill && ce == a.nnz) zero_rows(a.y, (int64_t)t + 1, a.nrows);
+    if (fill && ce == a.nnz) zero_rows(a.y, (int64_t)t + 1, a.nrows, a.ld);
   } else if (h_next != t) {
     write_seg(t, acc);
-    if (fill) zero_rows(a.y, (int64_t)t + 1, h_next);
+    if (fill) zero_rows(a.y, (int64_t)t + 1, h_next, a.ld);
   }
   // gap before the chunk's first row
   if (lane == 0 && fill) {
     const int64_t prev_row = cs == 0 ? -1 : a.ri[cs - 1];
-    zero_rows(a.y, prev_row + 1, first_row);
+    zero_rows(a.y, prev_row + 1, first_row, a.ld);
   }
 }
 
@@ -267,7 +268,7 @@ __global__ void coo_fixup_kernel(CooArgs<V> a) {
     const bool more = e2 < a.nnz && a.ri[e2] == lr;
     if (!(whole && more)) break;
   }
-  put_y(a.y, lr, tot.value(), a.init);
+  put_y(a.y, lr * a.ld, tot.value(), a.init);
 }
 
 template <typename V, int T, bool COMP>
@@ -284,14 +285,14 @@ __global__ void coo_atomic_kernel(CooArgs<V> a) {
   using A = typename vtraits<V>::acc;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.nnz) return;
-  const A p = vtraits<V>::widen(a.val[i]) * (A)a.x[a.ci[i]];
-  atomicAdd(&a.y[a.ri[i]], p);
+  const A p = vtraits<V>::widen(a.val[i]) * (A)a.x[(int64_t)a.ci[i] * a.ld];
+  atomicAdd(&a.y[(int64_t)a.ri[i] * a.ld], p);
 }
 
 template <typename V>
-__global__ void coo_zero_kernel(typename vtraits<V>::vec* y, int64_t n) {
+__global__ void coo_zero_kernel(typename vtraits<V>::vec* y, int64_t n, int64_t ld) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) y[i] = 0;
+  if (i < n) y[i * ld] = 0;
 }
 
 // max row gap (for the gap-fill decision)
@@ -419,7 +420,7 @@ int coo_analyse(mph_matrix_impl* A, hipStream_t s) {
 namespace {
 
 template <typename V>
-CooArgs<V> coo_args(mph_matrix_impl* A, const void* x, void* y, int init) {
+CooArgs<V> coo_args(mph_matrix_impl* A, const void* x, void* y, int init, int64_t ld) {
   CooArgs<V> a{};
   a.nrows = A->nrows;
   a.nnz = A->nnz;
@@ -436,20 +437,21 @@ CooArgs<V> coo_args(mph_matrix_impl* A, const void* x, void* y, int init) {
   a.first_val = static_cast<typename vtraits<V>::acc*>(A->coo.first_val);
   a.last_val = static_cast<typename vtraits<V>::acc*>(A->coo.last_val);
   a.chunks = A->coo.chunks;
+  a.ld = ld;
   return a;
 }
 
 template <typename V>
-int coo_run(mph_matrix_impl* A, const void* x, void* y, int init,
+int coo_run(mph_matrix_impl* A, const void* x, void* y, int init, int64_t ld,
             hipStream_t s) {
   using Acc = typename vtraits<V>::acc;
   constexpr bool comp = std::is_same<Acc, float>::value;
-  CooArgs<V> a = coo_args<V>(A, x, y, init);
+  CooArgs<V> a = coo_args<V>(A, x, y, init, ld);
   const CooPlan& p = A->coo;
   if (init && (p.prezero || !p.sorted || A->nnz == 0)) {
     if (A->nrows > 0)
       hipLaunchKernelGGL(coo_zero_kernel<V>, dim3((unsigned)ceil_div(A->nrows, 256)),
-                         dim3(256), 0, s, a.y, A->nrows);
+                         dim3(256), 0, s, a.y, A->nrows, ld);
     MPH_HIP_CHECK(hipGetLastError());
   }
   if (A->nnz == 0) return MPH_OK;
@@ -459,7 +461,7 @@ int coo_run(mph_matrix_impl* A, const void* x, void* y, int init,
     MPH_HIP_CHECK(hipGetLastError());
     return MPH_OK;
   }
-  if (p.xt.rows > 0) MPH_CHECK_STATUS(xtile_multiply(A, p.xt, x, y, init, s));
+  if (p.xt.rows > 0) MPH_CHECK_STATUS(xtile_multiply(A, p.xt, x, y, init, ld, s));
   if (p.nchunks == 0) return MPH_OK;
   const dim3 grid((unsigned)p.nblocks), block(256);
   const dim3 fgrid((unsigned)ceil_div(p.nchunks, 256));
@@ -483,15 +485,24 @@ int coo_run(mph_matrix_impl* A, const void* x, void* y, int init,
 
 }  // namespace
 
+// k right-hand sides (row-major X / Y): one segmented pass per column, each
+// reading x and y with element stride k.
 int coo_multiply(mph_matrix_impl* A, int k, const void* x, void* y, int init,
                  hipStream_t s) {
-  if (k != 1) return MPH_ERR_NOT_IMPLEMENTED;  // SpMM on COO goes through CSR
-  switch (A->vt) {
-    case MPH_F64: return coo_run<double>(A, x, y, init, s);
-    case MPH_F32: return coo_run<float>(A, x, y, init, s);
-    case MPH_BF16: return coo_run<bf16>(A, x, y, init, s);
+  const size_t vec = A->vt == MPH_F64 ? 8 : 4;
+  for (int j = 0; j < k; j++) {
+    const void* xj = static_cast<const char*>(x) + vec * j;
+    void* yj = static_cast<char*>(y) + vec * j;
+    int st;
+    switch (A->vt) {
+      case MPH_F64: st = coo_run<double>(A, xj, yj, init, k, s); break;
+      case MPH_F32: st = coo_run<float>(A, xj, yj, init, k, s); break;
+      case MPH_BF16: st = coo_run<bf16>(A, xj, yj, init, k, s); break;
+      default: return invalid("unknown value type");
+    }
+    if (st != MPH_OK) return st;
   }
-  return invalid("unknown value type");
+  return MPH_OK;
 }
 
 }  // namespace mph

@@ -447,7 +447,7 @@ __global__ __launch_bounds__(kXTileThreads) void xtile_kernel(RBArgs<V> a, XTArg
 #pragma unroll
       for (int q = 0; q < kPer; q++) {
         const int32_t j = j0 + q * kXTileThreads;
-        xv[q] = j < nx ? a.x[c0 + j] : (Vec)0;
+        xv[q] = j < nx ? a.x[(c0 + j) * a.ld] : (Vec)0;
       }
 #pragma unroll
       for (int q = 0; q < kPer; q++) {
@@ -639,11 +639,11 @@ __global__ __launch_bounds__(256) void xtile_combine_kernel(RBArgs<V> a, XTArgs 
   if constexpr (COMP) {
     csum<Acc> tot{pre[0], 0};
     for (int q = 0; q < kXCombRanges; q++) tot.combine(s_sum[q][ri], s_err[q][ri]);
-    store_y(a.y, r, tot.value(), a.init);
+    store_y(a.y, r * a.ld, tot.value(), a.init);
   } else {
     Acc tot = 0;
     for (int q = 0; q < kXCombRanges; q++) tot += s_sum[q][ri];
-    store_y(a.y, r, tot + pre[0], a.init);
+    store_y(a.y, r * a.ld, tot + pre[0], a.init);
   }
 }
 
@@ -1479,7 +1479,7 @@ namespace {
 
 template <typename V>
 int xtile_multiply_t(mph_matrix_impl* A, const XTilePlan& xp, const void* x, void* y, int init,
-                     hipStream_t s) {
+                     int64_t ld, hipStream_t s) {
   using Acc = typename vtraits<V>::acc;
   constexpr bool comp = std::is_same<Acc, float>::value;
   RBArgs<V> a{};
@@ -1487,7 +1487,7 @@ int xtile_multiply_t(mph_matrix_impl* A, const XTilePlan& xp, const void* x, voi
   a.ncols = A->ncols;
   a.x = static_cast<const typename vtraits<V>::vec*>(x);
   a.y = static_cast<typename vtraits<V>::vec*>(y);
-  a.ld = 1;
+  a.ld = (int32_t)ld;
   a.init = init;
   XTArgs t = xt_args(xp);
   const int64_t tiles8 = ceil_div((int64_t)xp.ntiles, (int64_t)kNumXcd) * kNumXcd;
@@ -1503,12 +1503,12 @@ int xtile_multiply_t(mph_matrix_impl* A, const XTilePlan& xp, const void* x, voi
 }  // namespace
 
 int xtile_multiply(mph_matrix_impl* A, const XTilePlan& xp, const void* x, void* y, int init,
-                   hipStream_t s) {
+                   int64_t ld, hipStream_t s) {
   if (xp.rows == 0) return MPH_OK;
   switch (A->vt) {
-    case MPH_F64: return xtile_multiply_t<double>(A, xp, x, y, init, s);
-    case MPH_F32: return xtile_multiply_t<float>(A, xp, x, y, init, s);
-    case MPH_BF16: return xtile_multiply_t<bf16>(A, xp, x, y, init, s);
+    case MPH_F64: return xtile_multiply_t<double>(A, xp, x, y, init, ld, s);
+    case MPH_F32: return xtile_multiply_t<float>(A, xp, x, y, init, ld, s);
+    case MPH_BF16: return xtile_multiply_t<bf16>(A, xp, x, y, init, ld, s);
   }
   return invalid("unknown value type");
 }

@@ -293,7 +293,7 @@ def test_reduced_precision_error(gpu, powerlaw, dtype, tol, name):
 
 
 # ------------------------------------------------------------ SpMM
-@pytest.mark.parametrize("name", ["CSR", "DIA", "ELL", "HYB", "HDC"])
+@pytest.mark.parametrize("name", ["CSR", "COO", "DIA", "ELL", "HYB", "HDC"])
 @pytest.mark.parametrize("k", [2, 8, 11])
 def test_spmm_matches_columnwise(gpu, name, k):
     A = gen.powerlaw_rows(4000, 4000, 1.5, 8.0, 2500, seed=8) if name not in ("DIA", "ELL") \
