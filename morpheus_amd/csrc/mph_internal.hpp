@@ -39,6 +39,7 @@ constexpr int kXTileCap = 512;                 // items of a (one-wave) sub-bloc
 constexpr int kXTileRows = 64;                 // rows of a sub-block
 constexpr int kXTileMaxGroupRows = 1024;       // rows of a group (run offsets in LDS)
 constexpr int64_t kXTileMinXBytes = 2 << 20;  // auto: x larger than this
+constexpr int64_t kXTileMinRowLen = 512;       // rows longer than this are candidates
 
 // The x-tiled pass of one matrix's long rows (see xtile_kernel).
 struct XTilePlan {

@@ -310,8 +310,9 @@ namespace {
 constexpr int kGapFillMax = 256;
 }
 
-// Row-sorted COO with long rows and a large x: the long rows (contiguous,
-// column-sorted runs of entries) take the x-tiled pass, which needs only their
+// Row-sorted COO with long rows and a large x: rows longer than
+// kXTileMinRowLen (contiguous, column-sorted runs of entries) take the x-tiled
+// pass, which needs only their
 // entry ranges; the segmented pass keeps every other entry through a table of
 // chunks that skip the tiled rows, and y is cleared first (init) since the
 // segmented pass no longer sees every gap between rows.
@@ -322,7 +323,6 @@ int coo_xtile_analyse(mph_matrix_impl* A, hipStream_t s) {
   const int mode = A->tuning.xtile;
   const bool on = mode == 2 || (mode == 0 && (int64_t)A->ncols * (int64_t)vec_bytes >= kXTileMinXBytes);
   if (!on || nnz == 0 || nrows == 0 || A->ncols == 0) return MPH_OK;
-  const int64_t long_len = A->tuning.rows_chunk > 0 ? A->tuning.rows_chunk : default_chunk(A->vt);
   int32_t* d_ro = nullptr;
   MPH_HIP_CHECK(hipMalloc(&d_ro, sizeof(int32_t) * (nrows + 1)));
   std::vector<int32_t> ro(nrows + 1);
@@ -345,7 +345,7 @@ int coo_xtile_analyse(mph_matrix_impl* A, hipStream_t s) {
       if (ro[r + 1] < ro[r]) {
         set_error("COO analysis check failed: row offsets not monotone");
         st = MPH_ERR_RUNTIME;
-      } else if (ro[r + 1] - ro[r] > long_len) {
+      } else if ((int64_t)ro[r + 1] - ro[r] > kXTileMinRowLen) {
         lrows.push_back((int32_t)r);
       }
     }

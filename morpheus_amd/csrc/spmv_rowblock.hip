@@ -1196,49 +1196,98 @@ void xtile_release(XTilePlan& xp) {
 
 namespace {
 
-// CSR / HYB / HDC: the long rows of the row-block plan go to the x-tiled
-// pass; the short-block launch skips them ({row, -2}) and keeps the position
-// chunks of long rows the pass does not take.
+// CSR / HYB / HDC: rows longer than kXTileMinRowLen go to the x-tiled pass.
+// The short-block list of the SpMV launch is rebuilt around them on the host:
+// a tiled row becomes a {row, -2} record the kernel skips, a row longer than
+// the chunk that the pass does not take keeps its position chunks, and the
+// other rows form short blocks with the usual limits; the list is checked
+// before upload.
 int xtile_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
   RowBlockPlan& p = A->rb;
   const int64_t nrows = A->nrows;
   const size_t vec_bytes = A->vt == MPH_F64 ? 8 : 4;
   const int mode = A->tuning.xtile;
   const bool on = mode == 2 || (mode == 0 && (int64_t)A->ncols * (int64_t)vec_bytes >= kXTileMinXBytes);
-  if (!on || p.nlong == 0 || nrows == 0 || A->ncols == 0) return MPH_OK;
+  if (!on || nrows == 0 || A->ncols == 0) return MPH_OK;
   const int32_t* d_ci = A->format == MPH_HYB ? A->coo_ci : A->csr_ci;
   const void* d_val = A->format == MPH_HYB ? A->coo_v : A->csr_v;
 
   std::vector<int32_t> ro(nrows + 1);
-  std::vector<RowBlock> bl(p.nblocks + 1);
   MPH_HIP_CHECK(hipMemcpyAsync(ro.data(), d_ro, sizeof(int32_t) * (nrows + 1),
-                               hipMemcpyDeviceToHost, s));
-  MPH_HIP_CHECK(hipMemcpyAsync(bl.data(), p.blocks, sizeof(RowBlock) * (p.nblocks + 1),
                                hipMemcpyDeviceToHost, s));
   MPH_HIP_CHECK(hipStreamSynchronize(s));
   std::vector<int32_t> lrows;
-  for (int64_t b = 0; b < p.nblocks; b++)
-    if (bl[b].sub == 0) lrows.push_back(bl[b].row);
+  for (int64_t r = 0; r < nrows; r++)
+    if ((int64_t)ro[r + 1] - ro[r] > kXTileMinRowLen) lrows.push_back((int32_t)r);
+  if (lrows.empty()) return MPH_OK;
   std::vector<char> tiled;
   MPH_CHECK_STATUS(xtile_build(A, ro, d_ro, d_ci, d_val, lrows, p.xt, tiled, s));
   if (p.xt.rows == 0) return MPH_OK;
 
+  const int64_t chunk = p.chunk;
   std::vector<RowBlock> sb;
   int32_t nchunked = 0;
-  for (int64_t b = 0; b < p.nblocks; b++) {
-    const RowBlock k = bl[b];
-    if (k.sub < 0) {
-      sb.push_back(k);
-    } else if (tiled[k.row]) {
-      if (k.sub == 0) sb.push_back(RowBlock{k.row, -2});
+  for (int64_t r = 0; r < nrows;) {
+    const int64_t len = (int64_t)ro[r + 1] - ro[r];
+    if (tiled[r]) {
+      sb.push_back(RowBlock{(int32_t)r, -2});
+      r++;
+    } else if (len > chunk) {
+      for (int64_t q = 0; q * chunk < len; q++) sb.push_back(RowBlock{(int32_t)r, (int32_t)q});
+      nchunked++;
+      r++;
     } else {
-      sb.push_back(k);
-      if (k.sub == 0) nchunked++;
+      const int64_t r0 = r;
+      int64_t items = 0;
+      while (r < nrows && !tiled[r] && (int64_t)ro[r + 1] - ro[r] <= chunk &&
+             r - r0 < kRowBlockMaxRows && items + ((int64_t)ro[r + 1] - ro[r]) < 2 * chunk) {
+        items += (int64_t)ro[r + 1] - ro[r];
+        r++;
+      }
+      sb.push_back(RowBlock{(int32_t)r0, -1});
     }
   }
   sb.push_back(RowBlock{(int32_t)nrows, -1});
+  // check: every row once, block limits as the kernels assume
+  {
+    int64_t next = 0;
+    for (size_t b = 0; b + 1 < sb.size(); b++) {
+      const RowBlock k = sb[b];
+      bool ok = k.row == next || (k.sub > 0 && k.row == next - 1);
+      if (k.sub == -1) {
+        const int64_t r1 = sb[b + 1].row;
+        ok = ok && r1 > k.row && r1 - k.row <= kRowBlockMaxRows &&
+             (int64_t)ro[r1] - ro[k.row] < 2 * chunk;
+        for (int64_t r = k.row; ok && r < r1; r++) ok = !tiled[r] && (int64_t)ro[r + 1] - ro[r] <= chunk;
+        next = r1;
+      } else if (k.sub == -2) {
+        ok = ok && tiled[k.row];
+        next = k.row + 1;
+      } else {
+        ok = ok && !tiled[k.row] && (int64_t)k.sub * chunk < (int64_t)ro[k.row + 1] - ro[k.row] &&
+             (k.sub == 0 || (sb[b - 1].row == k.row && sb[b - 1].sub == k.sub - 1));
+        next = k.row + 1;
+      }
+      if (!ok) {
+        xtile_release(p.xt);
+        set_error("x-tile analysis check failed: short-block list");
+        return MPH_ERR_RUNTIME;
+      }
+    }
+    if (next != nrows) {
+      xtile_release(p.xt);
+      set_error("x-tile analysis check failed: short blocks do not cover the rows");
+      return MPH_ERR_RUNTIME;
+    }
+  }
   RowBlock* d_sb = nullptr;
-  if (hipMalloc(&d_sb, sizeof(RowBlock) * sb.size()) != hipSuccess) {
+  void* d_part = nullptr;
+  const int accs = (A->vt == MPH_F64 ? 8 : 4);
+  const bool grow = (int64_t)sb.size() > (int64_t)p.nblocks + 1;
+  if (hipMalloc(&d_sb, sizeof(RowBlock) * sb.size()) != hipSuccess ||
+      (grow && hipMalloc(&d_part, (size_t)accs * 16 * sb.size()) != hipSuccess)) {
+    dfree(d_sb);
+    dfree(d_part);
     xtile_release(p.xt);
     set_error("hipMalloc failed in the x-tile analysis");
     return MPH_ERR_OUT_OF_MEMORY;
@@ -1247,9 +1296,15 @@ int xtile_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
           hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess) {
     dfree(d_sb);
+    dfree(d_part);
     xtile_release(p.xt);
     set_error("x-tile analysis: block upload failed");
     return MPH_ERR_HIP;
+  }
+  if (grow) {
+    // the chunk partials are indexed by the record position in either list
+    dfree(p.partial);
+    p.partial = d_part;
   }
   p.sblocks = d_sb;
   p.nsblocks = (int32_t)sb.size() - 1;
