@@ -38,6 +38,8 @@ void release_plans(mph_matrix_impl* A) {
   if (p.partial) dfree(p.partial);
   if (p.tail_ro) dfree(p.tail_ro);
   if (p.sblocks) dfree(p.sblocks);
+  if (p.smap) dfree(p.smap);
+  if (p.sro) dfree(p.sro);
   xtile_release(p.xt);
   p = RowBlockPlan{};
   CooPlan& c = A->coo;

@@ -40,6 +40,9 @@ constexpr int kXTileRows = 64;                 // rows of a sub-block
 constexpr int kXTileMaxGroupRows = 1024;       // rows of a group (run offsets in LDS)
 constexpr int64_t kXTileMinXBytes = 2 << 20;  // auto: x larger than this
 constexpr int64_t kXTileMinRowLen = 512;       // rows longer than this are candidates
+// HYB / HDC with at least this many rows run the ELL / DIA part as its own
+// coalesced pass and the tail / CSR part over the rows that have one.
+constexpr int64_t kSplitMinRows = 1 << 16;
 
 // The x-tiled pass of one matrix's long rows (see xtile_kernel).
 struct XTilePlan {
@@ -68,9 +71,14 @@ struct RowBlockPlan {
   int32_t nlong = 0;             // rows split into chunks
   // x-tiled long rows (SpMV); xt.rows == 0: long rows use position chunks
   XTilePlan xt;
-  RowBlock* sblocks = nullptr;   // short blocks, {row, -2} per tiled row, chunks of the rest
+  // split SpMV plan (x-tiled rows, and / or HYB / HDC in two passes):
+  // records over the list rows, {row, -2} per tiled row, chunks of the rest
+  RowBlock* sblocks = nullptr;
   int32_t nsblocks = 0;
   int32_t nchunked = 0;          // long rows left to position chunks
+  bool split = false;            // HYB / HDC: ELL / DIA pass, then the list
+  int32_t* smap = nullptr;       // split: list row -> matrix row
+  int32_t* sro = nullptr;        // split: item offsets of the list rows (+1)
 };
 
 // Analysis of a row-sorted COO matrix for the segmented reduction.

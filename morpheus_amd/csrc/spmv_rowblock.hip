@@ -63,7 +63,15 @@ struct RBArgs {
   int init;
   typename vtraits<V>::acc* __restrict__ partial;
   int32_t nblocks;
+  // split plan: block rows are list rows, list row i is matrix row rmap[i]
+  // (nullptr: the identity); used with PRE_NONE only
+  const int32_t* __restrict__ rmap;
 };
+
+template <typename V>
+__device__ __forceinline__ int64_t out_row(const RBArgs<V>& a, int64_t i) {
+  return a.rmap ? (int64_t)a.rmap[i] : i;
+}
 
 // --------------------------------------------------------------- prefixes
 template <typename V, int PRE, int KR>
@@ -245,11 +253,11 @@ __global__ void rowblock_combine_kernel(RBArgs<V> a) {
     csum<Acc> cs{pre[0], 0};
     for (int64_t q = 0; q < nsub; q++)
       cs.combine(a.partial[2 * (b + q)], a.partial[2 * (b + q) + 1]);
-    store_y(a.y, r, cs.value(), a.init);
+    store_y(a.y, out_row(a, r), cs.value(), a.init);
   } else {
     Acc t = 0;
     for (int64_t q = 0; q < nsub; q++) t += a.partial[b + q];
-    store_y(a.y, r, t + pre[0], a.init);
+    store_y(a.y, out_row(a, r), t + pre[0], a.init);
   }
 }
 
@@ -261,7 +269,7 @@ __global__ __launch_bounds__(kRowBlockThreads) void rowblock_spmv_kernel(
   __shared__ Acc s_red[2 * kRowBlockThreads / kWave];
   const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
   const RowBlock blk = a.blocks[b];
-  if (blk.sub < -1) return;  // a long row handled by the sliced pass
+  if (blk.sub < -1) return;  // a long row handled by the x-tiled pass
   if (blk.sub >= 0) {
     long_chunk_spmv<V, COMP>(a, b, blk, s_red);
     return;
@@ -311,7 +319,7 @@ __global__ __launch_bounds__(kRowBlockThreads) void rowblock_spmv_kernel(
       if constexpr (PRE != PRE_NONE) t += s_pre[r];
       out = t;
     }
-    if (l == 0) store_y(a.y, row0 + r, out, a.init);
+    if (l == 0) store_y(a.y, out_row(a, row0 + r), out, a.init);
   }
 }
 
@@ -1196,19 +1204,27 @@ void xtile_release(XTilePlan& xp) {
 
 namespace {
 
-// CSR / HYB / HDC: rows longer than kXTileMinRowLen go to the x-tiled pass.
-// The short-block list of the SpMV launch is rebuilt around them on the host:
-// a tiled row becomes a {row, -2} record the kernel skips, a row longer than
-// the chunk that the pass does not take keeps its position chunks, and the
-// other rows form short blocks with the usual limits; the list is checked
-// before upload.
-int xtile_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
+// The split SpMV plan of CSR / HYB / HDC, built on the host around the
+// x-tiled rows:
+// * x-tiling on: rows longer than kXTileMinRowLen go to the x-tiled pass;
+// * HYB / HDC with x-tiling on or at least kSplitMinRows rows: the ELL / DIA
+//   part of every row is computed first by its own coalesced pass, and the
+//   rows whose tail / CSR part is non-empty form a compacted row list (list
+//   row i is matrix row map[i]; its items start at offset lro[i], and
+//   lro[i + 1] ends them, since the rows left out hold no items).
+// Over the list (all rows for CSR), a tiled row becomes a {row, -2} record
+// the kernel skips, a row longer than the chunk that the pass does not take
+// keeps its position chunks, and the other rows form short blocks with the
+// usual limits; the records are checked before upload.
+int split_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
   RowBlockPlan& p = A->rb;
   const int64_t nrows = A->nrows;
   const size_t vec_bytes = A->vt == MPH_F64 ? 8 : 4;
   const int mode = A->tuning.xtile;
-  const bool on = mode == 2 || (mode == 0 && (int64_t)A->ncols * (int64_t)vec_bytes >= kXTileMinXBytes);
-  if (!on || nrows == 0 || A->ncols == 0) return MPH_OK;
+  const bool xt_on = mode == 2 || (mode == 0 && (int64_t)A->ncols * (int64_t)vec_bytes >= kXTileMinXBytes);
+  const bool prefixed = A->format == MPH_HYB || A->format == MPH_HDC;
+  const bool split = prefixed && (xt_on || nrows >= kSplitMinRows);
+  if ((!xt_on && !split) || nrows == 0 || A->ncols == 0) return MPH_OK;
   const int32_t* d_ci = A->format == MPH_HYB ? A->coo_ci : A->csr_ci;
   const void* d_val = A->format == MPH_HYB ? A->coo_v : A->csr_v;
 
@@ -1216,89 +1232,119 @@ int xtile_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
   MPH_HIP_CHECK(hipMemcpyAsync(ro.data(), d_ro, sizeof(int32_t) * (nrows + 1),
                                hipMemcpyDeviceToHost, s));
   MPH_HIP_CHECK(hipStreamSynchronize(s));
-  std::vector<int32_t> lrows;
-  for (int64_t r = 0; r < nrows; r++)
-    if ((int64_t)ro[r + 1] - ro[r] > kXTileMinRowLen) lrows.push_back((int32_t)r);
-  if (lrows.empty()) return MPH_OK;
-  std::vector<char> tiled;
-  MPH_CHECK_STATUS(xtile_build(A, ro, d_ro, d_ci, d_val, lrows, p.xt, tiled, s));
-  if (p.xt.rows == 0) return MPH_OK;
+  std::vector<char> tiled(nrows, 0);
+  if (xt_on) {
+    std::vector<int32_t> lrows;
+    for (int64_t r = 0; r < nrows; r++)
+      if ((int64_t)ro[r + 1] - ro[r] > kXTileMinRowLen) lrows.push_back((int32_t)r);
+    if (!lrows.empty()) MPH_CHECK_STATUS(xtile_build(A, ro, d_ro, d_ci, d_val, lrows, p.xt, tiled, s));
+  }
+  if (p.xt.rows == 0 && !split) return MPH_OK;
+
+  // the list rows: every row (CSR), or the rows with tail / CSR items
+  std::vector<int32_t> map, lro;
+  if (split) {
+    for (int64_t r = 0; r < nrows; r++)
+      if (ro[r + 1] > ro[r]) {
+        map.push_back((int32_t)r);
+        lro.push_back(ro[r]);
+      }
+    lro.push_back(ro[nrows]);
+  }
+  const std::vector<int32_t>& L = split ? lro : ro;
+  const int64_t m = split ? (int64_t)map.size() : nrows;
+  auto is_tiled = [&](int64_t i) { return tiled[split ? map[i] : i] != 0; };
+  auto len = [&](int64_t i) { return (int64_t)L[i + 1] - L[i]; };
 
   const int64_t chunk = p.chunk;
   std::vector<RowBlock> sb;
   int32_t nchunked = 0;
-  for (int64_t r = 0; r < nrows;) {
-    const int64_t len = (int64_t)ro[r + 1] - ro[r];
-    if (tiled[r]) {
-      sb.push_back(RowBlock{(int32_t)r, -2});
-      r++;
-    } else if (len > chunk) {
-      for (int64_t q = 0; q * chunk < len; q++) sb.push_back(RowBlock{(int32_t)r, (int32_t)q});
+  for (int64_t i = 0; i < m;) {
+    if (is_tiled(i)) {
+      sb.push_back(RowBlock{(int32_t)i, -2});
+      i++;
+    } else if (len(i) > chunk) {
+      for (int64_t q = 0; q * chunk < len(i); q++) sb.push_back(RowBlock{(int32_t)i, (int32_t)q});
       nchunked++;
-      r++;
+      i++;
     } else {
-      const int64_t r0 = r;
+      const int64_t i0 = i;
       int64_t items = 0;
-      while (r < nrows && !tiled[r] && (int64_t)ro[r + 1] - ro[r] <= chunk &&
-             r - r0 < kRowBlockMaxRows && items + ((int64_t)ro[r + 1] - ro[r]) < 2 * chunk) {
-        items += (int64_t)ro[r + 1] - ro[r];
-        r++;
+      while (i < m && !is_tiled(i) && len(i) <= chunk && i - i0 < kRowBlockMaxRows &&
+             items + len(i) < 2 * chunk) {
+        items += len(i);
+        i++;
       }
-      sb.push_back(RowBlock{(int32_t)r0, -1});
+      sb.push_back(RowBlock{(int32_t)i0, -1});
     }
   }
-  sb.push_back(RowBlock{(int32_t)nrows, -1});
-  // check: every row once, block limits as the kernels assume
+  sb.push_back(RowBlock{(int32_t)m, -1});
+  // check: every list row once, block limits as the kernels assume, the
+  // map strictly increasing over rows with items and lro its offsets
   {
+    bool ok = true;
+    for (int64_t i = 0; ok && split && i < m; i++)
+      ok = map[i] >= 0 && map[i] < nrows && (i == 0 || map[i] > map[i - 1]) &&
+           L[i] == ro[map[i]] && L[i + 1] == ro[map[i] + 1];
     int64_t next = 0;
-    for (size_t b = 0; b + 1 < sb.size(); b++) {
+    for (size_t b = 0; ok && b + 1 < sb.size(); b++) {
       const RowBlock k = sb[b];
-      bool ok = k.row == next || (k.sub > 0 && k.row == next - 1);
+      ok = k.row == next || (k.sub > 0 && k.row == next - 1);
       if (k.sub == -1) {
-        const int64_t r1 = sb[b + 1].row;
-        ok = ok && r1 > k.row && r1 - k.row <= kRowBlockMaxRows &&
-             (int64_t)ro[r1] - ro[k.row] < 2 * chunk;
-        for (int64_t r = k.row; ok && r < r1; r++) ok = !tiled[r] && (int64_t)ro[r + 1] - ro[r] <= chunk;
-        next = r1;
+        const int64_t i1 = sb[b + 1].row;
+        ok = ok && i1 > k.row && i1 - k.row <= kRowBlockMaxRows && (int64_t)L[i1] - L[k.row] < 2 * chunk;
+        for (int64_t i = k.row; ok && i < i1; i++) ok = !is_tiled(i) && len(i) <= chunk;
+        next = i1;
       } else if (k.sub == -2) {
-        ok = ok && tiled[k.row];
+        ok = ok && is_tiled(k.row);
         next = k.row + 1;
       } else {
-        ok = ok && !tiled[k.row] && (int64_t)k.sub * chunk < (int64_t)ro[k.row + 1] - ro[k.row] &&
+        ok = ok && !is_tiled(k.row) && (int64_t)k.sub * chunk < len(k.row) &&
              (k.sub == 0 || (sb[b - 1].row == k.row && sb[b - 1].sub == k.sub - 1));
         next = k.row + 1;
       }
-      if (!ok) {
-        xtile_release(p.xt);
-        set_error("x-tile analysis check failed: short-block list");
-        return MPH_ERR_RUNTIME;
-      }
     }
-    if (next != nrows) {
+    if (!ok || next != m) {
       xtile_release(p.xt);
-      set_error("x-tile analysis check failed: short blocks do not cover the rows");
+      set_error("split SpMV analysis check failed: short-block list");
       return MPH_ERR_RUNTIME;
     }
   }
   RowBlock* d_sb = nullptr;
   void* d_part = nullptr;
+  int32_t* d_map = nullptr;
+  int32_t* d_lro = nullptr;
   const int accs = (A->vt == MPH_F64 ? 8 : 4);
   const bool grow = (int64_t)sb.size() > (int64_t)p.nblocks + 1;
-  if (hipMalloc(&d_sb, sizeof(RowBlock) * sb.size()) != hipSuccess ||
-      (grow && hipMalloc(&d_part, (size_t)accs * 16 * sb.size()) != hipSuccess)) {
+  bool fail = hipMalloc(&d_sb, sizeof(RowBlock) * sb.size()) != hipSuccess ||
+              (grow && hipMalloc(&d_part, (size_t)accs * 16 * sb.size()) != hipSuccess);
+  if (!fail && split)
+    fail = hipMalloc(&d_map, sizeof(int32_t) * (size_t)std::max<int64_t>(m, 1)) != hipSuccess ||
+           hipMalloc(&d_lro, sizeof(int32_t) * (size_t)(m + 1)) != hipSuccess;
+  if (fail) {
     dfree(d_sb);
     dfree(d_part);
+    dfree(d_map);
+    dfree(d_lro);
     xtile_release(p.xt);
-    set_error("hipMalloc failed in the x-tile analysis");
+    set_error("hipMalloc failed in the split SpMV analysis");
     return MPH_ERR_OUT_OF_MEMORY;
   }
-  if (hipMemcpyAsync(d_sb, sb.data(), sizeof(RowBlock) * sb.size(), hipMemcpyHostToDevice, s) !=
-          hipSuccess ||
-      hipStreamSynchronize(s) != hipSuccess) {
+  fail = hipMemcpyAsync(d_sb, sb.data(), sizeof(RowBlock) * sb.size(), hipMemcpyHostToDevice, s) !=
+         hipSuccess;
+  if (!fail && split && m > 0)
+    fail = hipMemcpyAsync(d_map, map.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, s) !=
+           hipSuccess;
+  if (!fail && split)
+    fail = hipMemcpyAsync(d_lro, lro.data(), sizeof(int32_t) * (m + 1), hipMemcpyHostToDevice, s) !=
+           hipSuccess;
+  if (fail || hipStreamSynchronize(s) != hipSuccess) {
     dfree(d_sb);
     dfree(d_part);
+    dfree(d_map);
+    dfree(d_lro);
     xtile_release(p.xt);
-    set_error("x-tile analysis: block upload failed");
+    set_error("split SpMV analysis: upload failed");
     return MPH_ERR_HIP;
   }
   if (grow) {
@@ -1309,6 +1355,9 @@ int xtile_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
   p.sblocks = d_sb;
   p.nsblocks = (int32_t)sb.size() - 1;
   p.nchunked = nchunked;
+  p.split = split;
+  p.smap = d_map;
+  p.sro = d_lro;
   return MPH_OK;
 }
 
@@ -1368,7 +1417,7 @@ int rowblock_analyse(mph_matrix_impl* A, hipStream_t s) {
   dfree(cnt);
   dfree(first);
   dfree(tmp);
-  if (st == MPH_OK) st = xtile_analyse(A, ro, s);
+  if (st == MPH_OK) st = split_analyse(A, ro, s);
   return st;
 }
 
@@ -1409,6 +1458,42 @@ RBArgs<V> make_args(mph_matrix_impl* A, const void* x, void* y, int ld,
   return a;
 }
 
+// The split plan: short blocks over the list rows (tiled rows skipped,
+// untiled long rows as chunks), the x-tiled pass, and the combines.
+template <typename V, int PRE>
+int launch_split_spmv(mph_matrix_impl* A, const void* x, void* y, int init, hipStream_t s) {
+  using Acc = typename vtraits<V>::acc;
+  constexpr bool comp = std::is_same<Acc, float>::value;
+  RBArgs<V> a = make_args<V>(A, x, y, 1, init);
+  const RowBlockPlan& p = A->rb;
+  const size_t lds = sizeof(Acc) * (2 * (size_t)a.chunk + (PRE ? kRowBlockMaxRows : 0)) +
+                     sizeof(int32_t) * (kRowBlockMaxRows + 1);
+  RBArgs<V> sa = a;
+  sa.blocks = p.sblocks;
+  sa.nblocks = p.nsblocks;
+  if (p.split) {
+    sa.ro = p.sro;
+    sa.rmap = p.smap;
+  }
+  if (p.nsblocks > 0)
+    hipLaunchKernelGGL((rowblock_spmv_kernel<V, PRE, comp>), dim3((unsigned)p.nsblocks),
+                       dim3(kRowBlockThreads), lds, s, sa);
+  if (p.nchunked > 0)
+    hipLaunchKernelGGL((rowblock_combine_kernel<V, PRE, comp>),
+                       dim3((unsigned)ceil_div(p.nsblocks, 256)), dim3(256), 0, s, sa);
+  if (p.xt.rows > 0) {
+    XTArgs t = xt_args(p.xt);
+    const int64_t tiles8 = ceil_div((int64_t)p.xt.ntiles, (int64_t)kNumXcd) * kNumXcd;
+    const size_t xlds = xtile_lds_bytes(sizeof(typename vtraits<V>::vec), sizeof(Acc), p.xt.tile);
+    hipLaunchKernelGGL((xtile_kernel<V, comp>), dim3((unsigned)(tiles8 * p.xt.groups)),
+                       dim3(kXTileThreads), xlds, s, a, t);
+    hipLaunchKernelGGL((xtile_combine_kernel<V, PRE, comp>),
+                       dim3((unsigned)ceil_div(p.xt.rows, kXCombRows)), dim3(256), 0, s, a, t);
+  }
+  MPH_HIP_CHECK(hipGetLastError());
+  return MPH_OK;
+}
+
 template <typename V, int PRE>
 int launch_spmv(mph_matrix_impl* A, const void* x, void* y, int init,
                 hipStream_t s) {
@@ -1417,26 +1502,16 @@ int launch_spmv(mph_matrix_impl* A, const void* x, void* y, int init,
   const size_t lds = sizeof(Acc) * (2 * (size_t)a.chunk + (PRE ? kRowBlockMaxRows : 0)) +
                      sizeof(int32_t) * (kRowBlockMaxRows + 1);
   constexpr bool comp = std::is_same<Acc, float>::value;
-  const RowBlockPlan& p = A->rb;
-  if (p.xt.rows > 0) {
-    RBArgs<V> sa = a;
-    sa.blocks = p.sblocks;
-    sa.nblocks = p.nsblocks;
-    if (p.nsblocks > 0)
-      hipLaunchKernelGGL((rowblock_spmv_kernel<V, PRE, comp>), dim3((unsigned)p.nsblocks),
-                         dim3(kRowBlockThreads), lds, s, sa);
-    if (p.nchunked > 0)
-      hipLaunchKernelGGL((rowblock_combine_kernel<V, PRE, comp>),
-                         dim3((unsigned)ceil_div(p.nsblocks, 256)), dim3(256), 0, s, sa);
-    XTArgs t = xt_args(p.xt);
-    const int64_t tiles8 = ceil_div((int64_t)p.xt.ntiles, (int64_t)kNumXcd) * kNumXcd;
-    const size_t xlds = xtile_lds_bytes(sizeof(typename vtraits<V>::vec), sizeof(Acc), p.xt.tile);
-    hipLaunchKernelGGL((xtile_kernel<V, comp>), dim3((unsigned)(tiles8 * p.xt.groups)),
-                       dim3(kXTileThreads), xlds, s, a, t);
-    hipLaunchKernelGGL((xtile_combine_kernel<V, PRE, comp>),
-                       dim3((unsigned)ceil_div(p.xt.rows, kXCombRows)), dim3(256), 0, s, a, t);
-    MPH_HIP_CHECK(hipGetLastError());
-    return MPH_OK;
+  if (A->rb.sblocks) {
+    if (!A->rb.split) return launch_split_spmv<V, PRE_NONE>(A, x, y, init, s);
+    // HYB / HDC: the ELL / DIA part of every row in one coalesced pass that
+    // writes y (zeros when that part is empty; nothing to do then when
+    // accumulating), then the tail / CSR part of the rows that have one is
+    // added (short blocks and the x-tiled pass accumulate)
+    if (PRE != PRE_NONE || init)
+      MPH_CHECK_STATUS(A->format == MPH_HYB ? ell_multiply(A, 1, x, y, init, s)
+                                            : dia_multiply(A, 1, x, y, init, s));
+    return launch_split_spmv<V, PRE_NONE>(A, x, y, 0, s);
   }
   hipLaunchKernelGGL((rowblock_spmv_kernel<V, PRE, comp>),
                      dim3((unsigned)A->rb.nblocks), dim3(kRowBlockThreads), lds,

@@ -176,7 +176,7 @@ def test_xtile_reduced_precision(gpu, wide, dtype, tol):
     assert max_rel_err(y, ref, oracle.absrow(A, x)) <= tol
 
 
-@pytest.mark.parametrize("name", ["CSR", "COO"])
+@pytest.mark.parametrize("name", ["CSR", "COO", "HYB", "HDC"])
 @pytest.mark.parametrize("init", [True, False])
 def test_xtile_long_rows_at_the_ends(gpu, name, init):
     """Tiled long rows first and last, empty rows around them: the rows the
@@ -231,6 +231,36 @@ def test_xtile_plan_keeps_spmm(gpu, wide):
         bound = fp64_bound(np.diff(A.row_offsets), oracle.absrow(A, X[:, j]))
         assert np.all(np.abs(Y[:, j].cpu().numpy() - ref) <= bound)
     D.close()
+
+
+# ------------------------------------------------------------ split two-pass HYB / HDC
+@pytest.fixture(scope="module")
+def tall():
+    """100k rows: HYB / HDC run the ELL / DIA part as its own pass and then
+    the rows that have a tail / CSR part (a compacted row list), with the
+    long rows x-tiled or not."""
+    mats = {"powerlaw": gen.powerlaw_rows(100000, 100000, 1.5, 8.0, 3000, seed=41),
+            "banded": gen.banded(100000, 4, 0.02, seed=42)}
+    return {(m, f): (A, as_format(A, f)) for m, A in mats.items() for f in ("HYB", "HDC")}
+
+
+@pytest.mark.parametrize("mat", ["powerlaw", "banded"])
+@pytest.mark.parametrize("name", ["HYB", "HDC"])
+@pytest.mark.parametrize("xtile", [0, 1, 2])
+@pytest.mark.parametrize("init", [True, False])
+def test_split_two_pass(gpu, tall, mat, name, xtile, init):
+    A, Ah = tall[(mat, name)]
+    x = gen.random_vector(A.ncols, 14)
+    y0 = gen.random_vector(A.nrows, 15)
+    check_fp64(A, Ah, x, y0, init, tuning=dict(xtile=xtile))
+
+
+def test_split_two_pass_deterministic(gpu, tall):
+    _, Ah = tall[("powerlaw", "HYB")]
+    x = gen.random_vector(Ah.ncols, 16)
+    a = gpu_spmv(Ah, x, tuning=dict(xtile=2))
+    b = gpu_spmv(Ah, x, tuning=dict(xtile=2))
+    assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
 
 
 # ------------------------------------------------------------ edge cases
