@@ -88,6 +88,40 @@ __device__ __forceinline__ T shfl(T v, int src) {
   return __shfl(v, src, kWave);
 }
 
+// DPP lane moves of the GFX9 wavefront (no LDS, no address registers).
+// Controls: row_shr:n = 0x110 + n (within rows of 16 lanes), row_bcast:15 =
+// 0x142 (lane 15 of a row to the next row; with row mask 0xA rows 1 and 3
+// are written), row_bcast:31 = 0x143 (lane 31 to rows 2 and 3, row mask
+// 0xC), wave_shr:1 = 0x138 (lane i gets lane i - 1), wave_shl:1 = 0x130
+// (lane i gets lane i + 1).  Lanes without a source, or in rows the mask
+// leaves out, read 0.
+enum : int {
+  kDppRowShr1 = 0x111,
+  kDppRowShr2 = 0x112,
+  kDppRowShr4 = 0x114,
+  kDppRowShr8 = 0x118,
+  kDppRowBcast15 = 0x142,
+  kDppRowBcast31 = 0x143,
+  kDppWaveShr1 = 0x138,
+  kDppWaveShl1 = 0x130,
+};
+
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ int32_t dpp_i32(int32_t v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ float dpp_val(float v) {
+  return __builtin_bit_cast(float, dpp_i32<CTRL, ROWS>(__builtin_bit_cast(int32_t, v)));
+}
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ double dpp_val(double v) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)dpp_i32<CTRL, ROWS>((int32_t)(uint32_t)u);
+  const uint32_t hi = (uint32_t)dpp_i32<CTRL, ROWS>((int32_t)(uint32_t)(u >> 32));
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
 // Sum over aligned groups of L lanes (L a power of two <= 64).
 template <typename A>
 __device__ __forceinline__ A group_sum(A v, int L) {

@@ -55,6 +55,7 @@ struct XTilePlan {
   int32_t* lro = nullptr;        // [(groups*ntiles) * (gr+1)] item offsets of each row's run
   XTSub* sub = nullptr;          // sub-blocks, a sentinel after each (group, tile)
   int32_t* sub_start = nullptr;  // [groups*ntiles + 1] first sub-block of (group, tile)
+  uint8_t* head = nullptr;       // [sub-blocks * 64] run of each lane's first item
   uint16_t* col = nullptr;       // tile-major copy: column - tile * tile width
   void* val = nullptr;           // tile-major copy: values
   void* partial = nullptr;       // [ntiles * rows] (pairs in FP32), tile-major

@@ -352,6 +352,7 @@ struct XTArgs {
   const int32_t* __restrict__ lro;
   const XTSub* __restrict__ sub;
   const int32_t* __restrict__ sub_start;
+  const uint8_t* __restrict__ head;
   const uint16_t* __restrict__ col;
   const void* __restrict__ val;
   void* __restrict__ partial;
@@ -361,20 +362,29 @@ struct XTArgs {
 // consecutive items l*kBatch .. l*kBatch + kBatch - 1.
 inline XTArgs xt_args(const XTilePlan& p) {
   return XTArgs{p.tile, p.ntiles, p.groups, p.gr, p.rows, p.row, p.lro,
-                p.sub, p.sub_start, p.col, p.val, p.partial};
+                p.sub, p.sub_start, p.head, p.col, p.val, p.partial};
+}
+
+// A sub-block record is the same for every lane of the wave that reads it:
+// kept in scalar registers.
+__device__ __forceinline__ XTSub uniform_sub(const XTSub& v) {
+  return XTSub{__builtin_amdgcn_readfirstlane(v.i0), __builtin_amdgcn_readfirstlane(v.e0),
+               __builtin_amdgcn_readfirstlane(v.p0), 0};
 }
 
 template <typename V>
 struct XTLoad {
   uint32_t c[kBatch];
   typename vtraits<V>::acc v[kBatch];
+  uint32_t h;  // run of the lane's first item
 };
 
 template <typename V>
 __device__ __forceinline__ void xtile_load(const XTArgs& t, const V* cval, const XTSub& s0,
-                                           int32_t n, int lane, XTLoad<V>& d) {
+                                           int32_t n, int32_t sidx, int lane, XTLoad<V>& d) {
   const int32_t jb = lane * kBatch;
   const int32_t S = (n + kBatch - 1) / kBatch;
+  d.h = jb < n ? (uint32_t)t.head[(int64_t)sidx * kWave + lane] : 0u;
 #pragma unroll
   for (int q = 0; q < kBatch; q++) {
     const int32_t j = jb + q;
@@ -409,8 +419,22 @@ struct XSum {
       s += o.s;
     }
   }
-  __device__ XSum up(int d) const { return XSum{shfl_up(s, d), C ? shfl_up(c, d) : (A)0}; }
+  // the running sum of another lane, moved by a DPP control
+  template <int CTRL, int ROWS>
+  __device__ XSum mv() const {
+    return XSum{dpp_val<CTRL, ROWS>(s), C ? dpp_val<CTRL, ROWS>(c) : (A)0};
+  }
 };
+
+// One step of the wave's segmented inclusive scan: a lane whose segment did
+// not start at or after the source lane adds the source's running sum.
+template <int CTRL, int ROWS, typename S>
+__device__ __forceinline__ void xscan_step(S& acc, int& flag, bool valid) {
+  const S o = acc.template mv<CTRL, ROWS>();
+  const int of = dpp_i32<CTRL, ROWS>(flag);
+  if (valid && !flag) acc.merge(o);
+  if (valid) flag |= of;
+}
 
 template <typename V, bool COMP>
 __global__ __launch_bounds__(kXTileThreads) void xtile_kernel(RBArgs<V> a, XTArgs t) {
@@ -444,9 +468,9 @@ __global__ __launch_bounds__(kXTileThreads) void xtile_kernel(RBArgs<V> a, XTArg
   XTSub s0{0, 0}, s1{0, 0};
   XTLoad<V> cur;
   if (wave < nsub) {
-    s0 = t.sub[sb0 + wave];
-    s1 = t.sub[sb0 + wave + 1];
-    xtile_load<V>(t, cval, s0, s1.e0 - s0.e0, lane, cur);
+    s0 = uniform_sub(t.sub[sb0 + wave]);
+    s1 = uniform_sub(t.sub[sb0 + wave + 1]);
+    xtile_load<V>(t, cval, s0, s1.e0 - s0.e0, sb0 + wave, lane, cur);
   }
   {
     constexpr int kPer = 8;  // x loads in flight per thread
@@ -480,11 +504,7 @@ __global__ __launch_bounds__(kXTileThreads) void xtile_kernel(RBArgs<V> a, XTArg
       // the copy holds columns inside the tile; the clamp keeps a corrupted
       // entry inside the LDS tile
       p[b] = cur.v[b] * (Acc)xs[min(cur.c[b], (uint32_t)(nx - 1))];
-    if (q + kWaves < nsub) {
-      s0 = s_sub[q + kWaves];
-      s1 = s_sub[q + kWaves + 1];
-      xtile_load<V>(t, cval, s0, s1.e0 - s0.e0, lane, cur);
-    }
+    const int32_t hd = (int32_t)min(cur.h, (uint32_t)(nr - 1));
     auto put = [&](int32_t r, const S& v) {
       const int64_t o = (int64_t)tile * t.rows + i0 + r;
       if constexpr (COMP) {
@@ -497,29 +517,21 @@ __global__ __launch_bounds__(kXTileThreads) void xtile_kernel(RBArgs<V> a, XTArg
     const int32_t jb = lane * kBatch;
     const int cnt = (int)max(0, min(kBatch, n - jb));
     const bool active = cnt > 0;
-    // run of the lane's first item: the last r with ro[r] <= jb
-    int32_t r = 0;
-    if (active) {
-      int32_t lo = 0, hi = nr - 1;
-      while (lo < hi) {
-        const int32_t mid = (lo + hi + 1) >> 1;
-        if (ro[mid] - e0 <= jb)
-          lo = mid;
-        else
-          hi = mid - 1;
-      }
-      r = lo;
-    }
+    // run of the lane's first item (the last r with ro[r] <= jb), from the
+    // analysis
+    const int32_t r = active ? hd : 0;
     const int32_t h = r;
     int32_t cur_r = r;
     S head = S::zero(), run = S::zero();
     bool single = true;
-    int32_t next_start = active ? ro[cur_r + 1] - e0 : 0;
+    // first item of the next run, relative to the lane's first item (the
+    // loop compares it with the constant b)
+    const int32_t base = e0 + jb;
+    int32_t next_rel = active ? ro[cur_r + 1] - base : 0;
 #pragma unroll
     for (int b = 0; b < kBatch; b++) {
       if (b < cnt) {
-        const int32_t j = jb + b;
-        if (j >= next_start) {
+        if (b >= next_rel) {
           // a run ends here: the lane's head run, or a run complete in the lane
           if (single) {
             head = run;
@@ -529,27 +541,37 @@ __global__ __launch_bounds__(kXTileThreads) void xtile_kernel(RBArgs<V> a, XTArg
           }
           do {
             cur_r++;
-            next_start = ro[cur_r + 1] - e0;
-          } while (j >= next_start);
+            next_rel = ro[cur_r + 1] - base;
+          } while (b >= next_rel);
           run = S::zero();
         }
         run.add(p[b]);
       }
     }
     const int32_t tl = cur_r;
+    // the next sub-block's items are requested once this one's products are
+    // folded into the lane runs (fewer registers live across the scan)
+    if (q + kWaves < nsub) {
+      s0 = uniform_sub(s_sub[q + kWaves]);
+      s1 = uniform_sub(s_sub[q + kWaves + 1]);
+      xtile_load<V>(t, cval, s0, s1.e0 - s0.e0, sb0 + q + kWaves, lane, cur);
+    }
     // segmented inclusive scan of the lane tails across the wave
-    const int32_t t_prev = shfl_up(tl, 1);
+    // (Hillis-Steele within rows of 16 lanes, then across rows; all 64
+    // lanes take part)
+    const int32_t t_prev = dpp_i32<kDppWaveShr1>(tl);
     const bool cont = active && lane > 0 && single && h == t_prev;
     S acc = active ? run : S::zero();
     int flag = cont ? 0 : 1;
-    for (int d = 1; d < kWave; d <<= 1) {
-      const S o = acc.up(d);
-      const int of = shfl_up(flag, d);
-      if (lane >= d && !flag) acc.merge(o);
-      if (lane >= d) flag |= of;
-    }
-    const S prev_acc = acc.up(1);
-    const int32_t h_next = shfl_down(h, 1);
+    const int rl = lane & 15;
+    xscan_step<kDppRowShr1, 0xf>(acc, flag, rl >= 1);
+    xscan_step<kDppRowShr2, 0xf>(acc, flag, rl >= 2);
+    xscan_step<kDppRowShr4, 0xf>(acc, flag, rl >= 4);
+    xscan_step<kDppRowShr8, 0xf>(acc, flag, rl >= 8);
+    xscan_step<kDppRowBcast15, 0xa>(acc, flag, (lane & 16) != 0);
+    xscan_step<kDppRowBcast31, 0xc>(acc, flag, lane >= 32);
+    const S prev_acc = acc.template mv<kDppWaveShr1, 0xf>();
+    const int32_t h_next = dpp_i32<kDppWaveShl1>(h);
     const int last_lane = (n - 1) / kBatch;
     if (active) {
       if (!single) {
@@ -1098,6 +1120,30 @@ int xtile_build(mph_matrix_impl* A, const std::vector<int32_t>& ro, const int32_
       return MPH_ERR_RUNTIME;
     }
   }
+  // per sub-block, the run of each lane's first item 8 * lane: the last run
+  // r with lro[r] - e0 <= 8 * lane (0 for lanes past the items)
+  std::vector<uint8_t> head(subs.size() * kWave, 0);
+  for (int32_t g = 0; g < ngroups; g++) {
+    const int32_t j0 = g * gr;
+    for (int32_t q = 0; q < ntiles; q++) {
+      const int64_t k = (int64_t)g * ntiles + q;
+      const int32_t* L = &lro[(size_t)k * (gr + 1)];
+      for (int32_t si = sub_start[k]; si + 1 < sub_start[k + 1]; si++) {
+        const XTSub a = subs[si], b = subs[si + 1];
+        const int32_t nr = b.i0 - a.i0, n = b.e0 - a.e0;
+        if (nr <= 0 || nr > kXTileRows || n > kXTileCap || a.i0 < j0 || b.i0 > j0 + gr) {
+          set_error("x-tile analysis check failed: sub-block shape");
+          return MPH_ERR_RUNTIME;
+        }
+        int32_t r = 0;
+        for (int32_t l = 0; l < kWave && l * kBatch < n; l++) {
+          const int32_t jb = l * kBatch;
+          while (r + 1 < nr && L[a.i0 - j0 + r + 1] - a.e0 <= jb) r++;
+          head[(size_t)si * kWave + l] = (uint8_t)r;
+        }
+      }
+    }
+  }
   std::vector<int32_t> trow_id(nt);
   for (int32_t j = 0; j < nt; j++) trow_id[j] = lrows[trow[j]];
 
@@ -1106,6 +1152,7 @@ int xtile_build(mph_matrix_impl* A, const std::vector<int32_t>& ro, const int32_
   const size_t acc = A->vt == MPH_F64 ? 8 : 4, pair = A->vt == MPH_F64 ? 1 : 2;
   int32_t *d_row = nullptr, *d_lro = nullptr, *d_ss = nullptr, *d_tb = nullptr, *d_rs = nullptr;
   XTSub* d_sub = nullptr;
+  uint8_t* d_head = nullptr;
   uint16_t* d_col = nullptr;
   void *d_cv = nullptr, *d_part = nullptr;
   auto release = [&]() {
@@ -1115,6 +1162,7 @@ int xtile_build(mph_matrix_impl* A, const std::vector<int32_t>& ro, const int32_
     dfree(d_tb);
     dfree(d_rs);
     dfree(d_sub);
+    dfree(d_head);
     dfree(d_col);
     dfree(d_cv);
     dfree(d_part);
@@ -1125,6 +1173,7 @@ int xtile_build(mph_matrix_impl* A, const std::vector<int32_t>& ro, const int32_
       hipMalloc(&d_tb, sizeof(int32_t) * tb.size()) != hipSuccess ||
       hipMalloc(&d_rs, sizeof(int32_t) * run_sub.size()) != hipSuccess ||
       hipMalloc(&d_sub, sizeof(XTSub) * subs.size()) != hipSuccess ||
+      hipMalloc(&d_head, head.size()) != hipSuccess ||
       hipMalloc(&d_col, sizeof(uint16_t) * std::max<int64_t>(slots, 1)) != hipSuccess ||
       hipMalloc(&d_cv, vsz * std::max<int64_t>(slots, 1)) != hipSuccess ||
       hipMalloc(&d_part, acc * pair * (size_t)ntiles * nt) != hipSuccess) {
@@ -1137,7 +1186,8 @@ int xtile_build(mph_matrix_impl* A, const std::vector<int32_t>& ro, const int32_
             hipMemcpyAsync(d_ss, sub_start.data(), sizeof(int32_t) * sub_start.size(), hipMemcpyHostToDevice, s) == hipSuccess &&
             hipMemcpyAsync(d_tb, tb.data(), sizeof(int32_t) * tb.size(), hipMemcpyHostToDevice, s) == hipSuccess &&
             hipMemcpyAsync(d_rs, run_sub.data(), sizeof(int32_t) * run_sub.size(), hipMemcpyHostToDevice, s) == hipSuccess &&
-            hipMemcpyAsync(d_sub, subs.data(), sizeof(XTSub) * subs.size(), hipMemcpyHostToDevice, s) == hipSuccess;
+            hipMemcpyAsync(d_sub, subs.data(), sizeof(XTSub) * subs.size(), hipMemcpyHostToDevice, s) == hipSuccess &&
+            hipMemcpyAsync(d_head, head.data(), head.size(), hipMemcpyHostToDevice, s) == hipSuccess;
   if (ok) {
     if (A->vt == MPH_F64)
       launch_xtile_copy<double>(d_tb, nt, ntiles, gr, tile, d_lro, d_rs, d_sub, d_ci, d_val, d_col,
@@ -1180,6 +1230,7 @@ int xtile_build(mph_matrix_impl* A, const std::vector<int32_t>& ro, const int32_
   xp.lro = d_lro;
   xp.sub = d_sub;
   xp.sub_start = d_ss;
+  xp.head = d_head;
   xp.col = d_col;
   xp.val = d_cv;
   xp.partial = d_part;
@@ -1196,6 +1247,7 @@ void xtile_release(XTilePlan& xp) {
   dfree(xp.lro);
   dfree(xp.sub);
   dfree(xp.sub_start);
+  dfree(xp.head);
   dfree(xp.col);
   dfree(xp.val);
   dfree(xp.partial);
