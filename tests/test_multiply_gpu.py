@@ -263,6 +263,43 @@ def test_split_two_pass_deterministic(gpu, tall):
     assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
 
 
+# ------------------------------------------------------------ hipGraph capture
+@pytest.mark.parametrize("name", ALL)
+@pytest.mark.parametrize("xtile", [1, 2])
+def test_graph_replays_match_eager(gpu, name, xtile):
+    """mph_multiply captured in a hipGraph: four replays, with x rewritten
+    eagerly between them, each bit-identical to an eager product on the
+    same x (the captured launches hold no memset nodes)."""
+    torch = _torch()
+    from morpheus_amd import device
+    A = gen.powerlaw_rows(70000, 300000, 1.5, 8.0, 4000, seed=9) \
+        if name in ("CSR", "COO", "HYB", "HDC") else gen.stencil3d_7(24)
+    D = device.to_device(as_format(A, name), "f64")
+    D.set_tuning(xtile=xtile)
+    D.analyse()
+    x = torch.zeros(A.ncols, dtype=torch.float64, device="cuda")
+    y = torch.full((A.nrows,), float("nan"), dtype=torch.float64, device="cuda")
+    ye = torch.empty_like(y)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        D.multiply(x, y, True)
+    for rep in range(4):
+        xh = gen.random_vector(A.ncols, 40 + rep)
+        x.copy_(torch.from_numpy(xh))
+        g.replay()
+        torch.cuda.synchronize()
+        got = y.cpu().numpy().copy()
+        D.multiply(x, ye, True)
+        torch.cuda.synchronize()
+        assert np.array_equal(got.view(np.uint64), ye.cpu().numpy().view(np.uint64)), f"replay {rep}"
+        if rep == 0:
+            ref = oracle.spmv(A, xh)
+            bound = fp64_bound(np.diff(A.row_offsets), oracle.absrow(A, xh))
+            assert np.all(np.abs(got - ref) <= bound)
+    D.close()
+
+
 # ------------------------------------------------------------ edge cases
 def _with_empty_rows(n=5000, seed=4):
     rng = np.random.default_rng(seed)
