@@ -295,6 +295,37 @@ int mph_coo_count_nnz_per_diagonal(int64_t nrows, int64_t ncols, int64_t nnz,
 int mph_exceeds_tolerance(int64_t num_rows, int64_t num_entries,
                           int64_t num_columns);
 
+/* ------------------------------------------------------------ analytics */
+/* Morpheus::count_nnz_per_row<Exec>(A, nnz_per_row, init)
+ * (core/src/Morpheus_MatrixAnalytics.hpp:112-116) for a handle of any
+ * format, counting as core/src/impl/{Coo,Csr,Dia,Ell,Hyb,Hdc}/Serial/
+ * Morpheus_NonZeros_Per_Row_Impl.hpp do: CSR / COO stored entries, DIA
+ * in-range slots, ELL slots whose index is not -1, HYB = ELL + COO,
+ * HDC = DIA + CSR.  nnz_per_row: nrows int32 device counters; init != 0
+ * zeroes them first, otherwise the counts are added. */
+int mph_matrix_count_nnz_per_row(mph_matrix_t A, int32_t* nnz_per_row,
+                                 int32_t init, mph_stream_t stream);
+/* Morpheus::count_nnz_per_diagonal<Exec>(A, nnz_per_diagonal, init)
+ * (core/src/Morpheus_MatrixAnalytics.hpp:206-211) for a handle of any
+ * format, counting as core/src/impl/{Coo,Csr,Dia,Ell}/Kokkos/
+ * Morpheus_NonZeros_Per_Diagonal_Impl.hpp do (DIA: non-zero values over the
+ * pitch rows; ELL: among a row's first c slots, c being its count of slots
+ * with a valid index and a non-zero value).  nnz_per_diagonal: nrows +
+ * ncols - 1 int32 device counters, entry col - row + nrows - 1.  The
+ * indices are checked first (synchronises). */
+int mph_matrix_count_nnz_per_diagonal(mph_matrix_t A, int32_t* nnz_per_diagonal,
+                                      int32_t init, mph_stream_t stream);
+/* Morpheus::max / min / std / count_nnz of an int32 device vector, in one
+ * pass (core/src/impl/DenseVector/Kokkos/Morpheus_VectorAnalytics_Impl.hpp
+ * :50-127): std = sqrt(sq_dev / n), sq_dev summed in a fixed order;
+ * count = entries > threshold and != 0.  Synchronises. */
+typedef struct {
+  int64_t max, min, count;
+  double sq_dev;
+} mph_i32_stats_t;
+int mph_i32_stats(const int32_t* v, int64_t n, double mean, int32_t threshold,
+                  mph_i32_stats_t* out, mph_stream_t stream);
+
 /* ------------------------------------------------------------ batched */
 /* Many matrices of uneven size and structure, each with k right-hand sides,
  * in one persistent launch over a work queue of (matrix, row block) tiles.

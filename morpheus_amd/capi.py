@@ -66,6 +66,10 @@ pi64 = C.POINTER(C.c_int64)
 pI = C.POINTER(C.c_int)
 
 
+class I32Stats(C.Structure):
+    _fields_ = [("max", i64), ("min", i64), ("count", i64), ("sq_dev", C.c_double)]
+
+
 class Tuning(C.Structure):
     _fields_ = [("rows_chunk", i32), ("lanes", i32), ("coo_items", i32),
                 ("rows_per_thread", i32), ("xtile", i32)]
@@ -113,6 +117,9 @@ _SIGS = {
     "mph_coo_sort_by_row_and_column": (I, [i64, p, p, p, I, p]),
     "mph_coo_is_sorted": (I, [i64, p, p, pI, p]),
     "mph_coo_count_nnz_per_diagonal": (I, [i64, i64, i64, p, p, p, I, p]),
+    "mph_matrix_count_nnz_per_row": (I, [p, p, i32, p]),
+    "mph_matrix_count_nnz_per_diagonal": (I, [p, p, i32, p]),
+    "mph_i32_stats": (I, [p, i64, C.c_double, i32, p, p]),
     "mph_exceeds_tolerance": (I, [i64, i64, i64]),
     "mph_batch_create": (I, [C.POINTER(p), i32, C.POINTER(p), i32, C.POINTER(p),
                              C.POINTER(p), p]),

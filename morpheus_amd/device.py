@@ -344,6 +344,63 @@ def from_coo(A: DeviceMatrix, fmt: int, stream=None) -> DeviceMatrix:
     raise capi.InvalidInputException(f"unknown format {fmt}")
 
 
+# ------------------------------------------------------------------ analytics
+def count_nnz_per_row(A: DeviceMatrix, out=None, init=True, stream=None):
+    """Morpheus::count_nnz_per_row<HIP>(A, out, init) into an int32 device
+    vector (a new one when out is None)."""
+    if out is None:
+        out = torch.zeros(A.nrows, dtype=torch.int32, device="cuda")
+    capi.call("mph_matrix_count_nnz_per_row", A.handle, _p(out), int(bool(init)),
+              stream_handle(stream))
+    return out
+
+
+def count_nnz_per_diagonal(A: DeviceMatrix, out=None, init=True, stream=None):
+    """Morpheus::count_nnz_per_diagonal<HIP>(A, out, init): nrows + ncols - 1
+    int32 counters, entry col - row + nrows - 1."""
+    if out is None:
+        out = torch.zeros(A.nrows + A.ncols - 1, dtype=torch.int32, device="cuda")
+    capi.call("mph_matrix_count_nnz_per_diagonal", A.handle, _p(out), int(bool(init)),
+              stream_handle(stream))
+    return out
+
+
+def i32_stats(v, mean=0.0, threshold=0, stream=None):
+    """(max, min, sum of squared deviations from mean, count of entries >
+    threshold and != 0) of an int32 device vector."""
+    st = capi.I32Stats()
+    capi.call("mph_i32_stats", _p(v), int(v.numel()), float(mean), int(threshold),
+              C.byref(st), stream_handle(stream))
+    return st.max, st.min, st.sq_dev, st.count
+
+
+def max_nnnz(A: DeviceMatrix) -> int:
+    """core/src/Morpheus_MatrixAnalytics.hpp:126-140."""
+    return i32_stats(count_nnz_per_row(A))[0]
+
+
+def min_nnnz(A: DeviceMatrix) -> int:
+    """core/src/Morpheus_MatrixAnalytics.hpp:151-165."""
+    return i32_stats(count_nnz_per_row(A))[1]
+
+
+def std_nnnz(A: DeviceMatrix) -> int:
+    """core/src/Morpheus_MatrixAnalytics.hpp:177-191: the deviation around the
+    integer mean nnnz / nrows, truncated to the size type."""
+    sq = i32_stats(count_nnz_per_row(A), float(A.nnz // A.nrows))[2]
+    return int((sq / A.nrows) ** 0.5)
+
+
+def count_diagonals(A: DeviceMatrix) -> int:
+    """core/src/Morpheus_MatrixAnalytics.hpp:220-234."""
+    return i32_stats(count_nnz_per_diagonal(A))[3]
+
+
+def count_true_diagonals(A: DeviceMatrix, threshold: int) -> int:
+    """core/src/Morpheus_MatrixAnalytics.hpp:245-259."""
+    return i32_stats(count_nnz_per_diagonal(A), 0.0, threshold)[3]
+
+
 def convert(A: DeviceMatrix, fmt: int, stream=None) -> DeviceMatrix:
     """Morpheus::convert<Morpheus::HIP>(src, dst): via a temporary COO when
     neither side is COO (core/src/impl/Morpheus_Convert_Impl.hpp:83-106)."""

@@ -58,6 +58,15 @@ _SIGS = {
     "orc_coo_is_sorted": (I, [i64, p, p]),
     "orc_coo_sort_by_row_and_column": (I, [i64, p, p, p]),
     "orc_count_nnz_per_diagonal": (None, [i64, i64, i64, p, p, p]),
+    "orc_csr_nnz_per_row": (None, [i64, p, p, I]),
+    "orc_coo_nnz_per_row": (None, [i64, i64, p, p, I]),
+    "orc_dia_nnz_per_row": (None, [i64, i64, i64, p, p, I]),
+    "orc_ell_nnz_per_row": (None, [i64, i64, i64, p, p, I]),
+    "orc_csr_nnz_per_diagonal": (None, [i64, i64, p, p, p, I]),
+    "orc_dia_nnz_per_diagonal": (None, [i64, i64, i64, i64, p, p, p, I]),
+    "orc_ell_nnz_per_diagonal": (None, [i64, i64, i64, i64, p, p, p, I]),
+    "orc_i32_stats": (None, [i64, p, C.c_double, C.c_int32, pi64, pi64,
+                             C.POINTER(C.c_double), pi64]),
     "orc_csr_to_coo": (None, [i64, i64, p, p, p, p, p, p]),
     "orc_coo_to_csr": (None, [i64, i64, p, p, p, p, p, p]),
     "orc_coo_dia_ndiag": (i64, [i64, p, p]),
@@ -190,6 +199,97 @@ def count_nnz_per_diagonal(A: Coo) -> np.ndarray:
     lib().orc_count_nnz_per_diagonal(A.nrows, A.ncols, A.nnz, _ptr(A.row_indices),
                                      _ptr(A.column_indices), _ptr(out))
     return out
+
+
+# ---------------------------------------------------------------- analytics
+def nnz_per_row(A, out=None, init=True) -> np.ndarray:
+    """Impl::count_nnz_per_row of any host container (Serial semantics;
+    core/src/impl/*/Serial/Morpheus_NonZeros_Per_Row_Impl.hpp)."""
+    L = lib()
+    out = _i32(A.nrows) if out is None else np.array(out, dtype=np.int32, copy=True)
+    if isinstance(A, Csr):
+        L.orc_csr_nnz_per_row(A.nrows, _ptr(A.row_offsets), _ptr(out), int(init))
+    elif isinstance(A, Coo):
+        L.orc_coo_nnz_per_row(A.nrows, A.nnz, _ptr(A.row_indices), _ptr(out), int(init))
+    elif isinstance(A, Dia):
+        L.orc_dia_nnz_per_row(A.nrows, A.ncols, A.ndiag, _ptr(A.offsets), _ptr(out), int(init))
+    elif isinstance(A, Ell):
+        L.orc_ell_nnz_per_row(A.nrows, A.width, A.pitch, _ptr(A.column_indices), _ptr(out),
+                              int(init))
+    elif isinstance(A, Hyb):
+        # core/src/impl/Hyb/Serial/Morpheus_NonZeros_Per_Row_Impl.hpp: ELL, then COO
+        out = nnz_per_row(A.coo, nnz_per_row(A.ell, out, init), False)
+    elif isinstance(A, Hdc):
+        # core/src/impl/Hdc/Serial/Morpheus_NonZeros_Per_Row_Impl.hpp: DIA, then CSR
+        out = nnz_per_row(A.csr, nnz_per_row(A.dia, out, init), False)
+    else:
+        raise TypeError(type(A))
+    return out
+
+
+def nnz_per_diagonal(A, out=None, init=True) -> np.ndarray:
+    """Impl::count_nnz_per_diagonal of any host container; entry d is
+    diagonal d - (nrows - 1)."""
+    L = lib()
+    n = A.nrows + A.ncols - 1
+    out = _i32(n) if out is None else np.array(out, dtype=np.int32, copy=True)
+    if isinstance(A, Csr):
+        L.orc_csr_nnz_per_diagonal(A.nrows, A.ncols, _ptr(A.row_offsets),
+                                   _ptr(A.column_indices), _ptr(out), int(init))
+    elif isinstance(A, Coo):
+        if init:
+            out[:] = 0
+        L.orc_count_nnz_per_diagonal(A.nrows, A.ncols, A.nnz, _ptr(A.row_indices),
+                                     _ptr(A.column_indices), _ptr(out))
+    elif isinstance(A, Dia):
+        L.orc_dia_nnz_per_diagonal(A.nrows, A.ncols, A.ndiag, A.pitch, _ptr(A.offsets),
+                                   _ptr(A.values), _ptr(out), int(init))
+    elif isinstance(A, Ell):
+        L.orc_ell_nnz_per_diagonal(A.nrows, A.ncols, A.width, A.pitch,
+                                   _ptr(A.column_indices), _ptr(A.values), _ptr(out), int(init))
+    elif isinstance(A, Hyb):
+        out = nnz_per_diagonal(A.coo, nnz_per_diagonal(A.ell, out, init), False)
+    elif isinstance(A, Hdc):
+        out = nnz_per_diagonal(A.csr, nnz_per_diagonal(A.dia, out, init), False)
+    else:
+        raise TypeError(type(A))
+    return out
+
+
+def i32_stats(v, mean=0.0, threshold=0):
+    """(max, min, sum of squared deviations from mean, count of entries >
+    threshold and != 0) of an int32 vector."""
+    v = np.ascontiguousarray(v, dtype=np.int32)
+    mx, mn, c = C.c_int64(), C.c_int64(), C.c_int64()
+    sq = C.c_double()
+    lib().orc_i32_stats(v.size, _ptr(v), float(mean), int(threshold), C.byref(mx),
+                        C.byref(mn), C.byref(sq), C.byref(c))
+    return mx.value, mn.value, sq.value, c.value
+
+
+def nnz_of(A) -> int:
+    """A.nnnz() of the reference containers."""
+    if isinstance(A, Hyb):
+        return A.ell.nnz + A.coo.nnz
+    if isinstance(A, Hdc):
+        return A.dia.nnz + A.csr.nnz
+    return int(A.nnz) if not isinstance(A, Csr) else int(A.row_offsets[-1])
+
+
+def row_stats(A):
+    """max_nnnz, min_nnnz and std_nnnz as core/src/Morpheus_MatrixAnalytics.hpp
+    :126-204 computes them: std around the integer mean nnnz / nrows,
+    truncated to the size type."""
+    counts = nnz_per_row(A)
+    mean = float(nnz_of(A) // A.nrows)
+    mx, mn, sq, _ = i32_stats(counts, mean)
+    return mx, mn, int(np.sqrt(sq / A.nrows))
+
+
+def count_diagonals(A, threshold=0) -> int:
+    """core/src/Morpheus_MatrixAnalytics.hpp:220-259 (count_diagonals with
+    threshold 0, count_true_diagonals otherwise)."""
+    return i32_stats(nnz_per_diagonal(A), 0.0, threshold)[3]
 
 
 # ------------------------------------------------------------------ convert

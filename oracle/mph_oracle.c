@@ -202,6 +202,111 @@ void orc_count_nnz_per_diagonal(int64_t nrows, int64_t ncols, int64_t nnz,
 }
 
 /* ========================================================================
+ * Matrix analytics (Serial backend)
+ * ===================================================================== */
+
+/* core/src/impl/Csr/Serial/Morpheus_NonZeros_Per_Row_Impl.hpp:53-61 */
+void orc_csr_nnz_per_row(int64_t nrows, const int32_t* ro, int32_t* out, int init) {
+  for (int64_t i = 0; i < nrows; i++) {
+    int32_t nz = init ? 0 : out[i];
+    for (int32_t jj = ro[i]; jj < ro[i + 1]; jj++) nz++;
+    out[i] = nz;
+  }
+}
+
+/* core/src/impl/Coo/Serial/Morpheus_NonZeros_Per_Row_Impl.hpp:51-60 */
+void orc_coo_nnz_per_row(int64_t nrows, int64_t nnz, const int32_t* ri, int32_t* out,
+                         int init) {
+  if (init)
+    for (int64_t n = 0; n < nrows; n++) out[n] = 0;
+  for (int64_t n = 0; n < nnz; n++) out[ri[n]]++;
+}
+
+/* core/src/impl/Dia/Serial/Morpheus_NonZeros_Per_Row_Impl.hpp:53-66: every
+ * in-range slot counts, stored zeros included */
+void orc_dia_nnz_per_row(int64_t nrows, int64_t ncols, int64_t ndiag, const int32_t* off,
+                         int32_t* out, int init) {
+  for (int64_t row = 0; row < nrows; row++) {
+    int32_t nz = init ? 0 : out[row];
+    for (int64_t n = 0; n < ndiag; n++) {
+      const int64_t col = row + off[n];
+      if (col >= 0 && col < ncols) nz++;
+    }
+    out[row] = nz;
+  }
+}
+
+/* core/src/impl/Ell/Serial/Morpheus_NonZeros_Per_Row_Impl.hpp:54-69: every
+ * slot of the stored width whose index is not the padding -1 */
+void orc_ell_nnz_per_row(int64_t nrows, int64_t width, int64_t pitch, const int32_t* ci,
+                         int32_t* out, int init) {
+  if (init)
+    for (int64_t n = 0; n < nrows; n++) out[n] = 0;
+  for (int64_t i = 0; i < nrows; i++)
+    for (int64_t n = 0; n < width; n++)
+      if (ci[i + pitch * n] != -1) out[i]++;
+}
+
+/* core/src/impl/Csr/Kokkos/Morpheus_NonZeros_Per_Diagonal_Impl.hpp:59-76
+ * (the Serial entry point forwards there): every stored entry */
+void orc_csr_nnz_per_diagonal(int64_t nrows, int64_t ncols, const int32_t* ro,
+                              const int32_t* ci, int32_t* out, int init) {
+  if (init)
+    for (int64_t n = 0; n < nrows + ncols - 1; n++) out[n] = 0;
+  for (int64_t i = 0; i < nrows; i++)
+    for (int32_t jj = ro[i]; jj < ro[i + 1]; jj++) out[(int64_t)ci[jj] - i + nrows - 1]++;
+}
+
+/* core/src/impl/Dia/Kokkos/Morpheus_NonZeros_Per_Diagonal_Impl.hpp:61-76:
+ * the non-zero values of each stored diagonal, over all pitch rows */
+void orc_dia_nnz_per_diagonal(int64_t nrows, int64_t ncols, int64_t ndiag, int64_t pitch,
+                              const int32_t* off, const double* v, int32_t* out, int init) {
+  if (init)
+    for (int64_t n = 0; n < nrows + ncols - 1; n++) out[n] = 0;
+  for (int64_t i = 0; i < ndiag; i++)
+    for (int64_t j = 0; j < pitch; j++)
+      if (v[j + pitch * i] != 0) out[(int64_t)off[i] + nrows - 1]++;
+}
+
+/* core/src/impl/Ell/Kokkos/Morpheus_NonZeros_Per_Diagonal_Impl.hpp:63-122: a
+ * row's count c is its slots with a valid index and a non-zero value; the
+ * second pass then looks at the row's FIRST c slots only (so a zero value
+ * ahead of a non-zero one hides the latter) */
+void orc_ell_nnz_per_diagonal(int64_t nrows, int64_t ncols, int64_t width, int64_t pitch,
+                              const int32_t* ci, const double* v, int32_t* out, int init) {
+  if (init)
+    for (int64_t n = 0; n < nrows + ncols - 1; n++) out[n] = 0;
+  for (int64_t i = 0; i < nrows; i++) {
+    int64_t c = 0;
+    for (int64_t n = 0; n < width; n++)
+      if (ci[i + pitch * n] != -1 && v[i + pitch * n] != 0) c++;
+    for (int64_t n = 0; n < c; n++) {
+      const int32_t col = ci[i + pitch * n];
+      if (col != -1 && v[i + pitch * n] != 0) out[(int64_t)col - i + nrows - 1]++;
+    }
+  }
+}
+
+/* core/src/impl/DenseVector/Kokkos/Morpheus_VectorAnalytics_Impl.hpp:50-127:
+ * max, min, the sum of squared deviations from `mean` (std is
+ * sqrt(sum / n)), and the count of entries > threshold and != 0 */
+void orc_i32_stats(int64_t n, const int32_t* v, double mean, int32_t threshold,
+                   int64_t* mx, int64_t* mn, double* sqdev, int64_t* count) {
+  int64_t a = n > 0 ? v[0] : 0, b = n > 0 ? v[0] : 0, c = 0;
+  double s = 0;
+  for (int64_t i = 0; i < n; i++) {
+    if (v[i] > a) a = v[i];
+    if (v[i] < b) b = v[i];
+    s += (v[i] - mean) * (v[i] - mean);
+    if (v[i] > threshold && v[i] != 0) c++;
+  }
+  *mx = a;
+  *mn = b;
+  *sqdev = s;
+  *count = c;
+}
+
+/* ========================================================================
  * Convert (Serial backend)
  * ===================================================================== */
 

@@ -129,3 +129,40 @@ def max_rel_err(y, yref, absrow):
     if not mask.any():
         return 0.0
     return float(np.max(np.abs(y[mask] - yref[mask]) / s[mask]))
+
+
+def anti_diagonal_matrix(n: int, m: int, diags, seed: int) -> Csr:
+    """The reference's AntiDiagonalMatrixGenerator
+    (core/tests/utils/MatrixGenerator.hpp:147-160): A(i, m - j - 1) is set
+    wherever |j - i| equals |d| for a listed d, with values U(-2, 2) (drawn
+    here from a seeded generator)."""
+    rng = np.random.default_rng(seed)
+    dense = np.zeros((n, m))
+    i = np.arange(n)[:, None]
+    j = np.arange(m)[None, :]
+    mask = np.zeros((n, m), dtype=bool)
+    for d in diags:
+        mask |= np.abs(j - i) == abs(int(d))
+    # column j of the generator's loop lands at m - j - 1
+    mask = mask[:, ::-1]
+    vals = rng.uniform(-2.0, 2.0, size=int(mask.sum()))
+    vals[vals == 0] = 1.0
+    dense[mask] = vals
+    rows, cols = np.nonzero(dense)
+    counts = np.bincount(rows, minlength=n)
+    ro = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(counts, out=ro[1:])
+    return Csr(n, m, ro, cols, dense[rows, cols])
+
+
+def reference_diags(n: int, freq: int = 5):
+    """The diagonal list of the reference's analytics tests
+    (core/tests/Test_MatrixAnalytics.hpp:145-160): 0 and +-k*freq."""
+    return [0] + [s * d for d in range(freq, n, freq) for s in (1, -1)]
+
+
+def dense_of(A: Csr) -> np.ndarray:
+    D = np.zeros((A.nrows, A.ncols))
+    r = np.repeat(np.arange(A.nrows), np.diff(A.row_offsets))
+    D[r, A.column_indices] = A.values
+    return D
