@@ -22,6 +22,8 @@
 // All device work is ordered on the default stream, as in the reference.
 #pragma once
 
+#include <cmath>
+
 #include "Morpheus_AMD_Containers.hpp"
 
 namespace Morpheus {
@@ -633,37 +635,182 @@ bool is_sorted(Matrix& mat) {
   return sorted != 0;
 }
 
-// Non-zeros per diagonal of a COO matrix into a DenseVector of
-// nrows + ncols - 1 counters, diagonal d = col - row + nrows - 1
-// (core/src/Morpheus_MatrixAnalytics.hpp:206-210,
-// core/src/impl/Coo/Kokkos/Morpheus_NonZeros_Per_Diagonal_Impl.hpp:38-74).
+// ============================================================ analytics
+// core/src/Morpheus_MatrixAnalytics.hpp:49-104
+template <typename Matrix>
+typename Matrix::size_type number_of_rows(const Matrix& A) {
+  return A.nrows();
+}
+template <typename Matrix>
+typename Matrix::size_type number_of_columns(const Matrix& A) {
+  return A.ncols();
+}
+template <typename Matrix>
+typename Matrix::size_type number_of_nnz(const Matrix& A) {
+  return A.nnnz();
+}
+template <typename Matrix>
+double average_nnnz(const Matrix& A) {
+  return A.nnnz() / (double)A.nrows();
+}
+template <typename Matrix>
+double density(const Matrix& A) {
+  return A.nnnz() / (double)(A.nrows() * A.ncols());
+}
+
+namespace Impl {
+// a concrete container of A (the active one of a DynamicMatrix)
+template <class M, class Fn>
+inline void with_concrete(const M& A, Fn&& fn) {
+  if constexpr (is_dynamic_matrix_format_container_v<M>)
+    std::visit([&](const auto& a) { fn(a); }, A.const_formats());
+  else
+    fn(A);
+}
+
+// fn(int* device counters) over an int DenseVector in either space: host
+// vectors travel through a device copy (in when `in`, always back out)
+template <class Vec, class Fn>
+inline void with_device_ints(Vec& v, bool in, Fn&& fn) {
+  static_assert(std::is_same_v<typename Vec::non_const_value_type, int>,
+                "the analytics count into a DenseVector<int>");
+  if constexpr (Vec::memory_space::is_device) {
+    fn(v.data());
+  } else {
+    DenseVector<int, HIPSpace> d(v.size());
+    if (in) copy_array(v.const_view(), d.view(), "counters");
+    fn(d.data());
+    copy_array(d.const_view(), v.view(), "counters");
+  }
+}
+
+template <class Vec>
+inline mph_i32_stats_t int_stats(const Vec& v, size_t n, double mean, int threshold) {
+  static_assert(std::is_same_v<typename Vec::non_const_value_type, int>,
+                "the vector analytics take a DenseVector<int>");
+  mph_i32_stats_t st{};
+  if constexpr (Vec::memory_space::is_device) {
+    check(mph_i32_stats(v.data(), (int64_t)n, mean, threshold, &st, nullptr), "Morpheus::max/min/std");
+  } else {
+    DenseVector<int, HIPSpace> d(v.size());
+    copy_array(v.const_view(), d.view(), "vector");
+    check(mph_i32_stats(d.data(), (int64_t)n, mean, threshold, &st, nullptr), "Morpheus::max/min/std");
+  }
+  return st;
+}
+
+template <class Exec, class M>
+inline DenseVector<int, HIPSpace> rows_counted(const M& A) {
+  DenseVector<int, HIPSpace> c(A.nrows());
+  with_concrete(A, [&](const auto& a) {
+    check_access<Exec, std::decay_t<decltype(a)>>();
+    check(mph_matrix_count_nnz_per_row(matrix_handle(a), c.data(), 1, nullptr),
+          "Morpheus::count_nnz_per_row");
+  });
+  return c;
+}
+
+template <class Exec, class M>
+inline DenseVector<int, HIPSpace> diagonals_counted(const M& A) {
+  DenseVector<int, HIPSpace> c(A.nrows() + A.ncols() - 1);
+  with_concrete(A, [&](const auto& a) {
+    check_access<Exec, std::decay_t<decltype(a)>>();
+    check(mph_matrix_count_nnz_per_diagonal(matrix_handle(a), c.data(), 1, nullptr),
+          "Morpheus::count_nnz_per_diagonal");
+  });
+  return c;
+}
+}  // namespace Impl
+
+// Non-zeros per row into a DenseVector<int> of nrows counters, per format as
+// core/src/impl/<Fmt>/Serial/Morpheus_NonZeros_Per_Row_Impl.hpp counts them
+// (core/src/Morpheus_MatrixAnalytics.hpp:112-116).  init = false adds.
+template <typename ExecSpace, typename Matrix, typename Vector>
+void count_nnz_per_row(const Matrix& A, Vector& nnz_per_row, const bool init = true) {
+  if (nnz_per_row.size() != A.nrows())
+    throw InvalidInputException(
+        "count_nnz_per_row: destination vector must have equal size to the source matrix "
+        "number of rows");
+  Impl::with_concrete(A, [&](const auto& a) {
+    Impl::check_access<ExecSpace, std::decay_t<decltype(a)>>();
+    Impl::with_device_ints(nnz_per_row, !init, [&](int* out) {
+      Impl::check(mph_matrix_count_nnz_per_row(Impl::matrix_handle(a), out, init ? 1 : 0, nullptr),
+                  "Morpheus::count_nnz_per_row");
+    });
+  });
+}
+
+// Non-zeros per diagonal into a DenseVector<int> of nrows + ncols - 1
+// counters, diagonal d = col - row + nrows - 1
+// (core/src/Morpheus_MatrixAnalytics.hpp:206-211; per format as
+// core/src/impl/<Fmt>/Kokkos/Morpheus_NonZeros_Per_Diagonal_Impl.hpp).
 template <typename ExecSpace, typename Matrix, typename Vector>
 void count_nnz_per_diagonal(const Matrix& A, Vector& nnz_per_diagonal, const bool init = true) {
-  static_assert(Matrix::format_id == COO_FORMAT, "count_nnz_per_diagonal takes a CooMatrix");
-  static_assert(std::is_same_v<typename Vector::non_const_value_type, int>,
-                "count_nnz_per_diagonal counts into a DenseVector<int>");
-  Impl::check_access<ExecSpace, Matrix>();
-  const size_t nd = A.nrows() + A.ncols() - 1;
-  if (nnz_per_diagonal.size() != nd)
-    throw InvalidInputException("count_nnz_per_diagonal: vector must hold nrows + ncols - 1 entries");
-  auto run = [&](const auto& d, int* out) {
-    Impl::check(mph_coo_count_nnz_per_diagonal((int64_t)d.nrows(), (int64_t)d.ncols(),
-                                               (int64_t)d.nnnz(), d.crow_indices().data(),
-                                               d.ccolumn_indices().data(), out, init ? 1 : 0,
-                                               nullptr),
-                "Morpheus::count_nnz_per_diagonal");
-  };
-  if constexpr (Matrix::memory_space::is_device) {
-    run(A, nnz_per_diagonal.data());
-  } else {
-    Impl::device_type_of<Matrix> d;
-    d.allocate(A);
-    Impl::copy_sparse(A, d);
-    DenseVector<int, HIPSpace> cnt(nd);
-    Impl::copy_array(nnz_per_diagonal.const_view(), cnt.view(), "nnz_per_diagonal");
-    run(d, cnt.data());
-    Impl::copy_array(cnt.const_view(), nnz_per_diagonal.view(), "nnz_per_diagonal");
-  }
+  if (nnz_per_diagonal.size() != A.nrows() + A.ncols() - 1)
+    throw InvalidInputException(
+        "count_nnz_per_diagonal: destination vector must have equal size to the source matrix "
+        "number of diagonals (i.e NROWS + NCOLS - 1)");
+  Impl::with_concrete(A, [&](const auto& a) {
+    Impl::check_access<ExecSpace, std::decay_t<decltype(a)>>();
+    Impl::with_device_ints(nnz_per_diagonal, !init, [&](int* out) {
+      Impl::check(mph_matrix_count_nnz_per_diagonal(Impl::matrix_handle(a), out, init ? 1 : 0,
+                                                    nullptr),
+                  "Morpheus::count_nnz_per_diagonal");
+    });
+  });
+}
+
+// core/src/Morpheus_MatrixAnalytics.hpp:126-140
+template <typename ExecSpace, typename Matrix>
+typename Matrix::size_type max_nnnz(const Matrix& A) {
+  auto c = Impl::rows_counted<ExecSpace>(A);
+  return (typename Matrix::size_type)Impl::int_stats(c, c.size(), 0.0, 0).max;
+}
+// core/src/Morpheus_MatrixAnalytics.hpp:151-165
+template <typename ExecSpace, typename Matrix>
+typename Matrix::size_type min_nnnz(const Matrix& A) {
+  auto c = Impl::rows_counted<ExecSpace>(A);
+  return (typename Matrix::size_type)Impl::int_stats(c, c.size(), 0.0, 0).min;
+}
+// core/src/Morpheus_MatrixAnalytics.hpp:177-191: the deviation around the
+// integer mean nnnz / nrows, returned in the size type (truncated)
+template <typename ExecSpace, typename Matrix>
+typename Matrix::size_type std_nnnz(const Matrix& A) {
+  auto c = Impl::rows_counted<ExecSpace>(A);
+  const double mean = (double)(A.nnnz() / A.nrows());
+  const auto st = Impl::int_stats(c, c.size(), mean, 0);
+  return (typename Matrix::size_type)std::sqrt(st.sq_dev / (double)c.size());
+}
+// core/src/Morpheus_MatrixAnalytics.hpp:220-234
+template <typename ExecSpace, typename Matrix>
+typename Matrix::size_type count_diagonals(const Matrix& A) {
+  auto c = Impl::diagonals_counted<ExecSpace>(A);
+  return (typename Matrix::size_type)Impl::int_stats(c, c.size(), 0.0, 0).count;
+}
+// core/src/Morpheus_MatrixAnalytics.hpp:245-259
+template <typename ExecSpace, typename Matrix>
+typename Matrix::size_type count_true_diagonals(const Matrix& A,
+                                                typename Matrix::index_type threshold) {
+  auto c = Impl::diagonals_counted<ExecSpace>(A);
+  return (typename Matrix::size_type)Impl::int_stats(c, c.size(), 0.0, (int)threshold).count;
+}
+
+// Vector analytics over a DenseVector<int> (core/src/Morpheus_VectorAnalytics.hpp:48-130)
+template <typename ExecSpace, typename Vector>
+typename Vector::value_type max(const Vector& vec, typename Vector::size_type size) {
+  return (typename Vector::value_type)Impl::int_stats(vec, size, 0.0, 0).max;
+}
+template <typename ExecSpace, typename Vector>
+typename Vector::value_type min(const Vector& vec, typename Vector::size_type size) {
+  return (typename Vector::value_type)Impl::int_stats(vec, size, 0.0, 0).min;
+}
+template <typename ExecSpace, typename Vector>
+double std(const Vector& vec, typename Vector::size_type size, double mean) {
+  return std::sqrt(Impl::int_stats(vec, size, mean, 0).sq_dev / (double)size);
+}
+template <typename ExecSpace, typename Vector>
+typename Vector::size_type count_nnz(const Vector& vec, typename Vector::value_type threshold = 0) {
+  return (typename Vector::size_type)Impl::int_stats(vec, vec.size(), 0.0, (int)threshold).count;
 }
 
 // ============================================================ extensions

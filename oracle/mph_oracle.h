@@ -70,6 +70,21 @@ int orc_coo_sort_by_row_and_column(int64_t n, int32_t* ri, int32_t* ci,
 void orc_count_nnz_per_diagonal(int64_t nrows, int64_t ncols, int64_t nnz,
                                 const int32_t* ri, const int32_t* ci,
                                 int32_t* out /* nrows+ncols-1, accumulated */);
+void orc_csr_nnz_per_row(int64_t nrows, const int32_t* ro, int32_t* out, int init);
+void orc_coo_nnz_per_row(int64_t nrows, int64_t nnz, const int32_t* ri, int32_t* out,
+                         int init);
+void orc_dia_nnz_per_row(int64_t nrows, int64_t ncols, int64_t ndiag, const int32_t* off,
+                         int32_t* out, int init);
+void orc_ell_nnz_per_row(int64_t nrows, int64_t width, int64_t pitch, const int32_t* ci,
+                         int32_t* out, int init);
+void orc_csr_nnz_per_diagonal(int64_t nrows, int64_t ncols, const int32_t* ro,
+                              const int32_t* ci, int32_t* out, int init);
+void orc_dia_nnz_per_diagonal(int64_t nrows, int64_t ncols, int64_t ndiag, int64_t pitch,
+                              const int32_t* off, const double* v, int32_t* out, int init);
+void orc_ell_nnz_per_diagonal(int64_t nrows, int64_t ncols, int64_t width, int64_t pitch,
+                              const int32_t* ci, const double* v, int32_t* out, int init);
+void orc_i32_stats(int64_t n, const int32_t* v, double mean, int32_t threshold,
+                   int64_t* mx, int64_t* mn, double* sqdev, int64_t* count);
 
 /* ----------------------------------------------------------------- convert */
 void orc_csr_to_coo(int64_t nrows, int64_t nnz, const int32_t* ro,

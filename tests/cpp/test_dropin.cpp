@@ -594,6 +594,114 @@ TEST(Convert_Tolerance) {
   EXPECT_EQ(dyn.active_index(), 0);
 }
 
+// ============================================================ analytics
+// core/tests/Test_MatrixAnalytics.hpp: non-zeros per row on the diagonal
+// test matrix and per diagonal on the anti-diagonal one, counted on the
+// dense matrix; init and accumulate; min / max / std; count_diagonals; each
+// format, on HIPSpace through HIP and on HostSpace through Serial, and
+// through a DynamicMatrix.
+static Coo<Host> analytics_matrix(int n, bool anti) {
+  std::vector<int> diags{0};
+  for (int d = 5; d < n; d += 5) {
+    diags.push_back(d);
+    diags.push_back(-d);
+  }
+  std::vector<double> dense((size_t)n * n, 0.0);
+  std::mt19937_64 rng(n + (anti ? 1 : 0));
+  std::uniform_real_distribution<double> u(0.5, 2.0);
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++)
+      for (int d : diags)
+        if (std::abs(j - i) == std::abs(d)) dense[(size_t)i * n + (anti ? n - j - 1 : j)] = u(rng);
+  size_t nnz = 0;
+  for (double v : dense) nnz += v != 0;
+  Coo<Host> A(n, n, nnz);
+  size_t k = 0;
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++)
+      if (dense[(size_t)i * n + j] != 0) {
+        A.row_indices(k) = i;
+        A.column_indices(k) = j;
+        A.values(k) = dense[(size_t)i * n + j];
+        k++;
+      }
+  return A;
+}
+
+template <int F>
+static void analytics_format(int n) {
+  using IVecH = Morpheus::DenseVector<int, Host>;
+  using IVecD = Morpheus::DenseVector<int, Dev>;
+  const Coo<Host> R = analytics_matrix(n, false), Q = analytics_matrix(n, true);
+  std::vector<int> rows(n, 0), diag(2 * n - 1, 0);
+  for (size_t k = 0; k < R.nnnz(); k++) rows[R.crow_indices(k)]++;
+  for (size_t k = 0; k < Q.nnnz(); k++) diag[Q.ccolumn_indices(k) - Q.crow_indices(k) + n - 1]++;
+  const int mx = *std::max_element(rows.begin(), rows.end());
+  const int mn = *std::min_element(rows.begin(), rows.end());
+  double sq = 0;
+  const double mean = (double)(R.nnnz() / (size_t)n);
+  for (int r : rows) sq += (r - mean) * (r - mean);
+  const size_t sd = (size_t)std::sqrt(sq / n);
+  size_t ndiag = 0, ntrue = 0;
+  for (int d : diag) {
+    ndiag += d > 0;
+    ntrue += d > 3;
+  }
+
+  using M = std::decay_t<decltype(small_of<F>())>;
+  M Rh, Qh;
+  Morpheus::convert<Serial>(R, Rh);
+  Morpheus::convert<Serial>(Q, Qh);
+  auto Rd = Morpheus::create_mirror_container<Dev>(Rh);
+  auto Qd = Morpheus::create_mirror_container<Dev>(Qh);
+  Morpheus::copy(Rh, Rd);
+  Morpheus::copy(Qh, Qd);
+
+  IVecD cd(n, 1);
+  Morpheus::count_nnz_per_row<HIPExec>(Rd, cd);
+  IVecH ch(n, 0);
+  Morpheus::copy(cd, ch);
+  bool ok = true;
+  for (int i = 0; i < n; i++) ok = ok && ch(i) == rows[i];
+  EXPECT_TRUE(ok);
+  IVecH acc(n, 1);
+  Morpheus::count_nnz_per_row<Serial>(Rh, acc, false);
+  ok = true;
+  for (int i = 0; i < n; i++) ok = ok && acc(i) == rows[i] + 1;
+  EXPECT_TRUE(ok);
+  EXPECT_EQ(Morpheus::max_nnnz<HIPExec>(Rd), (size_t)mx);
+  EXPECT_EQ(Morpheus::min_nnnz<Serial>(Rh), (size_t)mn);
+  EXPECT_EQ(Morpheus::std_nnnz<HIPExec>(Rd), sd);
+  EXPECT_EQ(Morpheus::max<HIPExec>(cd, cd.size()), mx);
+
+  IVecD dd(2 * n - 1, 0);
+  Morpheus::count_nnz_per_diagonal<HIPExec>(Qd, dd);
+  IVecH dh(2 * n - 1, 0);
+  Morpheus::copy(dd, dh);
+  ok = true;
+  for (int i = 0; i < 2 * n - 1; i++) ok = ok && dh(i) == diag[i];
+  EXPECT_TRUE(ok);
+  EXPECT_EQ(Morpheus::count_diagonals<HIPExec>(Qd), ndiag);
+  EXPECT_EQ(Morpheus::count_true_diagonals<Serial>(Qh, 3), ntrue);
+  EXPECT_EQ(Morpheus::count_nnz<HIPExec>(dd), ndiag);
+
+  Dyn<Dev> dyn(Qd);
+  EXPECT_EQ(Morpheus::count_diagonals<HIPExec>(dyn), ndiag);
+  EXPECT_EQ(Morpheus::number_of_nnz(Rh), R.nnnz());
+  EXPECT_TRUE(std::abs(Morpheus::average_nnnz(Rd) - R.nnnz() / (double)n) < 1e-12);
+}
+
+TEST(Analytics_AllFormats) {
+  for (int n : {10, 100, 1000}) {
+    analytics_format<0>(n);
+    analytics_format<1>(n);
+    analytics_format<2>(n);
+    analytics_format<3>(n);
+    analytics_format<4>(n);
+    analytics_format<5>(n);
+  }
+}
+
 // ============================================================ larger cases vs the oracle
 // A random matrix with ragged rows (one long row), converted on the device
 // from CSR to every format and multiplied: conversions are compared with the
