@@ -49,6 +49,11 @@ class HipError(MorpheusError):
     status = ERR_HIP
 
 
+class IOException(MorpheusError):
+    """Morpheus::IOException (core/src/Morpheus_Exceptions.hpp): MatrixMarket
+    files that cannot be opened or parsed."""
+
+
 _EXC = {
     ERR_NOT_IMPLEMENTED: NotImplementedException,
     ERR_INVALID_INPUT: InvalidInputException,

@@ -1,6 +1,7 @@
 // Morpheus_AMD.hpp -- umbrella header of the MI355X drop-in for the Morpheus
 // SpMV path (the counterpart of core/src/Morpheus.hpp): spaces, containers,
-// copy / mirrors, multiply, convert and the DynamicMatrix switch, all served
+// copy / mirrors, multiply, convert, the DynamicMatrix switch, the matrix
+// analytics and MatrixMarket I/O, the device work served
 // by the HIP kernels of libmorpheus_amd.so through include/morpheus_amd.h.
 //
 // Build: any C++17 compiler with the HIP runtime headers, e.g.
@@ -12,3 +13,4 @@
 #include "Morpheus_AMD_Spaces.hpp"
 #include "Morpheus_AMD_Containers.hpp"
 #include "Morpheus_AMD_Algorithms.hpp"
+#include "Morpheus_AMD_MatrixMarket.hpp"

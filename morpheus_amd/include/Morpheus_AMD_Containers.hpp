@@ -1227,5 +1227,26 @@ inline constexpr bool is_dense_vector_format_container_v =
     is_dense_vector_format_container<T>::value;
 template <class T>
 inline constexpr bool is_sparse_matrix_container_v = is_sparse_matrix_container<T>::value;
+template <class T>
+inline constexpr bool is_dense_matrix_format_container_v =
+    is_dense_matrix_format_container<T>::value;
+// a concrete sparse container of one format (core/src/Morpheus_FormatTraits.hpp)
+template <class T, int F, class = void>
+struct has_format_id : std::false_type {};
+template <class T, int F>
+struct has_format_id<T, F, std::void_t<decltype(T::format_id)>>
+    : std::bool_constant<is_sparse_matrix_container<T>::value && T::format_id == F> {};
+template <class T>
+inline constexpr bool is_coo_matrix_format_container_v = has_format_id<T, COO_FORMAT>::value;
+template <class T>
+inline constexpr bool is_csr_matrix_format_container_v = has_format_id<T, CSR_FORMAT>::value;
+template <class T>
+inline constexpr bool is_dia_matrix_format_container_v = has_format_id<T, DIA_FORMAT>::value;
+template <class T>
+inline constexpr bool is_ell_matrix_format_container_v = has_format_id<T, ELL_FORMAT>::value;
+template <class T>
+inline constexpr bool is_hyb_matrix_format_container_v = has_format_id<T, HYB_FORMAT>::value;
+template <class T>
+inline constexpr bool is_hdc_matrix_format_container_v = has_format_id<T, HDC_FORMAT>::value;
 
 }  // namespace Morpheus
