@@ -295,6 +295,39 @@ int mph_coo_count_nnz_per_diagonal(int64_t nrows, int64_t ncols, int64_t nnz,
 int mph_exceeds_tolerance(int64_t num_rows, int64_t num_entries,
                           int64_t num_columns);
 
+/* ------------------------------------------------------------ vectors */
+/* Dense-vector algorithms on device vectors of `value_type` (MPH_F64:
+ * double; MPH_F32 / MPH_BF16: float), n entries.  Sums are formed in FP64
+ * per block and added in a fixed order, so each call gives the same bits. */
+/* Morpheus::dot<HIP>(n, x, y) (core/src/Morpheus_Dot.hpp:45-57;
+ * core/src/impl/DenseVector/HIP/Morpheus_Dot_Impl.hpp).  Synchronises. */
+int mph_dot(int64_t n, const void* x, const void* y, int value_type,
+            double* result, mph_stream_t stream);
+/* Morpheus::waxpby<HIP>(n, alpha, x, beta, y, w): w = alpha x + beta y, each
+ * product and sum rounded on its own, as the Serial loops
+ * (core/src/Morpheus_WAXPBY.hpp:48-65). */
+int mph_waxpby(int64_t n, double alpha, const void* x, double beta, const void* y,
+               void* w, int value_type, mph_stream_t stream);
+/* Morpheus::reduce<HIP>(in, n): the sum of the entries, TwoSum-compensated
+ * (core/src/Morpheus_Reduction.hpp:44-52).  Synchronises. */
+int mph_reduce(int64_t n, const void* x, int value_type, double* result,
+               mph_stream_t stream);
+
+/* Conjugate gradients on a square matrix handle (no reference counterpart:
+ * the consumer of the SpMV path).  Vectors are device vectors of the
+ * handle's vector type.  mph_cg_start(b, x0) forms r = b - A x0 and p = r;
+ * mph_cg_iterate enqueues `iters` iterations, each the handle's SpMV and
+ * three fused vector launches, with every scalar kept on the device (no host
+ * synchronisation, no memset: a hipGraph may capture it, and replays of a
+ * captured iteration continue the iteration); mph_cg_residual returns r.r
+ * (synchronises).  Calls on one handle must be ordered. */
+typedef struct mph_cg_s* mph_cg_t;
+int mph_cg_create(mph_cg_t* out, mph_matrix_t A);
+int mph_cg_destroy(mph_cg_t cg);
+int mph_cg_start(mph_cg_t cg, const void* b, void* x, mph_stream_t stream);
+int mph_cg_iterate(mph_cg_t cg, void* x, int32_t iters, mph_stream_t stream);
+int mph_cg_residual(mph_cg_t cg, double* rr, mph_stream_t stream);
+
 /* ------------------------------------------------------------ analytics */
 /* Morpheus::count_nnz_per_row<Exec>(A, nnz_per_row, init)
  * (core/src/Morpheus_MatrixAnalytics.hpp:112-116) for a handle of any

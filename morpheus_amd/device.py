@@ -344,6 +344,65 @@ def from_coo(A: DeviceMatrix, fmt: int, stream=None) -> DeviceMatrix:
     raise capi.InvalidInputException(f"unknown format {fmt}")
 
 
+# ------------------------------------------------------------ vector algorithms
+def _vt_of(v) -> int:
+    return capi.F64 if v.dtype == torch.float64 else capi.F32
+
+
+def dot(x, y, stream=None) -> float:
+    """Morpheus::dot<HIP>(n, x, y) of two device vectors."""
+    out = C.c_double()
+    capi.call("mph_dot", int(x.numel()), _p(x), _p(y), _vt_of(x), C.byref(out),
+              stream_handle(stream))
+    return out.value
+
+
+def waxpby(alpha, x, beta, y, w, stream=None):
+    """Morpheus::waxpby<HIP>(n, alpha, x, beta, y, w): w = alpha x + beta y."""
+    capi.call("mph_waxpby", int(x.numel()), float(alpha), _p(x), float(beta), _p(y), _p(w),
+              _vt_of(x), stream_handle(stream))
+    return w
+
+
+def reduce(x, stream=None) -> float:
+    """Morpheus::reduce<HIP>(x, n)."""
+    out = C.c_double()
+    capi.call("mph_reduce", int(x.numel()), _p(x), _vt_of(x), C.byref(out), stream_handle(stream))
+    return out.value
+
+
+class CG:
+    """Conjugate gradients on a square DeviceMatrix, one SpMV and three fused
+    vector launches per iteration, scalars on the device."""
+
+    def __init__(self, A: DeviceMatrix):
+        self.A = A
+        self.handle = C.c_void_p()
+        capi.call("mph_cg_create", C.byref(self.handle), A.handle)
+
+    def start(self, b, x, stream=None):
+        capi.call("mph_cg_start", self.handle, _p(b), _p(x), stream_handle(stream))
+
+    def iterate(self, x, iters, stream=None):
+        capi.call("mph_cg_iterate", self.handle, _p(x), int(iters), stream_handle(stream))
+
+    def residual(self, stream=None) -> float:
+        rr = C.c_double()
+        capi.call("mph_cg_residual", self.handle, C.byref(rr), stream_handle(stream))
+        return rr.value
+
+    def close(self):
+        if self.handle:
+            capi.lib().mph_cg_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # ------------------------------------------------------------------ analytics
 def count_nnz_per_row(A: DeviceMatrix, out=None, init=True, stream=None):
     """Morpheus::count_nnz_per_row<HIP>(A, out, init) into an int32 device

@@ -635,6 +635,92 @@ bool is_sorted(Matrix& mat) {
   return sorted != 0;
 }
 
+// ============================================================ dense vectors
+namespace Impl {
+template <class Vec>
+constexpr int vector_code() {
+  using V = typename Vec::non_const_value_type;
+  static_assert(std::is_same_v<V, double> || std::is_same_v<V, float>,
+                "the vector algorithms take double or float vectors");
+  return std::is_same_v<V, double> ? MPH_F64 : MPH_F32;
+}
+// fn(device pointer) for a vector in either space (host vectors are staged;
+// written back when `out`)
+template <class Vec, class Fn>
+inline void with_device_vector(Vec& v, bool in, bool out, Fn&& fn) {
+  if constexpr (Vec::memory_space::is_device) {
+    fn(v.data());
+  } else {
+    DenseVector<typename Vec::non_const_value_type, HIPSpace> d(v.size());
+    if (in) copy_array(v.const_view(), d.view(), "vector");
+    fn(d.data());
+    if (out) copy_array(d.const_view(), v.view(), "vector");
+  }
+}
+}  // namespace Impl
+
+// core/src/Morpheus_Dot.hpp:45-57: the sum of x_i y_i over the first n
+// entries (formed in FP64 and added in a fixed order)
+template <typename ExecSpace, typename Vector1, typename Vector2>
+typename Vector2::value_type dot(typename Vector1::size_type n, const Vector1& x,
+                                 const Vector2& y) {
+  static_assert(is_dense_vector_format_container_v<Vector1> &&
+                    is_dense_vector_format_container_v<Vector2>,
+                "x and y must be DenseVector containers");
+  Impl::check_access<ExecSpace, Vector1>();
+  if (x.size() < n || y.size() < n) throw InvalidInputException("Morpheus::dot: vectors too short");
+  double r = 0;
+  auto& xm = const_cast<Vector1&>(x);
+  auto& ym = const_cast<Vector2&>(y);
+  Impl::with_device_vector(xm, true, false, [&](auto* px) {
+    Impl::with_device_vector(ym, true, false, [&](auto* py) {
+      Impl::check(mph_dot((int64_t)n, px, py, Impl::vector_code<Vector1>(), &r, nullptr),
+                  "Morpheus::dot");
+    });
+  });
+  return (typename Vector2::value_type)r;
+}
+
+// core/src/Morpheus_WAXPBY.hpp:48-65: w = alpha x + beta y on the first n
+// entries, each product and sum rounded on its own
+template <typename ExecSpace, typename Vector1, typename Vector2, typename Vector3>
+void waxpby(const size_t n, const typename Vector1::value_type alpha, const Vector1& x,
+            const typename Vector2::value_type beta, const Vector2& y, Vector3& w) {
+  static_assert(is_dense_vector_format_container_v<Vector1> &&
+                    is_dense_vector_format_container_v<Vector2> &&
+                    is_dense_vector_format_container_v<Vector3>,
+                "x, y and w must be DenseVector containers");
+  Impl::check_access<ExecSpace, Vector1>();
+  if (x.size() < n || y.size() < n || w.size() < n)
+    throw InvalidInputException("Morpheus::waxpby: vectors too short");
+  auto& xm = const_cast<Vector1&>(x);
+  auto& ym = const_cast<Vector2&>(y);
+  Impl::with_device_vector(xm, true, false, [&](auto* px) {
+    Impl::with_device_vector(ym, true, false, [&](auto* py) {
+      Impl::with_device_vector(w, false, true, [&](auto* pw) {
+        Impl::check(mph_waxpby((int64_t)n, (double)alpha, px, (double)beta, py, pw,
+                               Impl::vector_code<Vector1>(), nullptr),
+                    "Morpheus::waxpby");
+      });
+    });
+  });
+}
+
+// core/src/Morpheus_Reduction.hpp:44-52: the sum of the first `size` entries
+template <typename ExecSpace, typename Vector>
+typename Vector::value_type reduce(const Vector& in, typename Vector::size_type size) {
+  static_assert(is_dense_vector_format_container_v<Vector>, "in must be a DenseVector container");
+  Impl::check_access<ExecSpace, Vector>();
+  if (in.size() < size) throw InvalidInputException("Morpheus::reduce: vector too short");
+  double r = 0;
+  auto& vm = const_cast<Vector&>(in);
+  Impl::with_device_vector(vm, true, false, [&](auto* p) {
+    Impl::check(mph_reduce((int64_t)size, p, Impl::vector_code<Vector>(), &r, nullptr),
+                "Morpheus::reduce");
+  });
+  return (typename Vector::value_type)r;
+}
+
 // ============================================================ analytics
 // core/src/Morpheus_MatrixAnalytics.hpp:49-104
 template <typename Matrix>

@@ -58,6 +58,10 @@ _SIGS = {
     "orc_coo_is_sorted": (I, [i64, p, p]),
     "orc_coo_sort_by_row_and_column": (I, [i64, p, p, p]),
     "orc_count_nnz_per_diagonal": (None, [i64, i64, i64, p, p, p]),
+    "orc_dot": (C.c_double, [i64, p, p]),
+    "orc_waxpby": (None, [i64, C.c_double, p, C.c_double, p, p]),
+    "orc_reduce": (C.c_double, [i64, p]),
+    "orc_cg_csr": (C.c_double, [i64, p, p, p, p, p, i64]),
     "orc_csr_nnz_per_row": (None, [i64, p, p, I]),
     "orc_coo_nnz_per_row": (None, [i64, i64, p, p, I]),
     "orc_dia_nnz_per_row": (None, [i64, i64, i64, p, p, I]),
@@ -199,6 +203,37 @@ def count_nnz_per_diagonal(A: Coo) -> np.ndarray:
     lib().orc_count_nnz_per_diagonal(A.nrows, A.ncols, A.nnz, _ptr(A.row_indices),
                                      _ptr(A.column_indices), _ptr(out))
     return out
+
+
+# ------------------------------------------------------------ vector algorithms
+def dot(x, y=None) -> float:
+    """Morpheus::dot (Serial): sequential sum of x_i y_i (x_i^2 when y is x)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = x if y is None else np.ascontiguousarray(y, dtype=np.float64)
+    return lib().orc_dot(x.size, _ptr(x), _ptr(y))
+
+
+def waxpby(alpha, x, beta, y) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    w = _f64(x.size)
+    lib().orc_waxpby(x.size, float(alpha), _ptr(x), float(beta), _ptr(y), _ptr(w))
+    return w
+
+
+def reduce(v) -> float:
+    """Morpheus::reduce (Serial): Kahan summation."""
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    return lib().orc_reduce(v.size, _ptr(v))
+
+
+def cg(A: Csr, b, x0, iters):
+    """`iters` textbook CG iterations from x0; returns (x, r.r)."""
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.array(x0, dtype=np.float64, copy=True)
+    rr = lib().orc_cg_csr(A.nrows, _ptr(A.row_offsets), _ptr(A.column_indices),
+                          _ptr(A.values), _ptr(b), _ptr(x), int(iters))
+    return x, rr
 
 
 # ---------------------------------------------------------------- analytics

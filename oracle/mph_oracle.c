@@ -307,6 +307,78 @@ void orc_i32_stats(int64_t n, const int32_t* v, double mean, int32_t threshold,
 }
 
 /* ========================================================================
+ * Dense vector algorithms (Serial backend)
+ * ===================================================================== */
+
+/* core/src/impl/DenseVector/Serial/Morpheus_Dot_Impl.hpp:48-60 */
+double orc_dot(int64_t n, const double* x, const double* y) {
+  double r = 0;
+  if (x == y)
+    for (int64_t i = 0; i < n; i++) r += x[i] * x[i];
+  else
+    for (int64_t i = 0; i < n; i++) r += x[i] * y[i];
+  return r;
+}
+
+/* core/src/impl/DenseVector/Serial/Morpheus_WAXPBY_Impl.hpp:49-64 (built
+ * with -ffp-contract=off: every product and sum rounded on its own) */
+void orc_waxpby(int64_t n, double alpha, const double* x, double beta, const double* y,
+                double* w) {
+  if (alpha == 1.0)
+    for (int64_t i = 0; i < n; i++) w[i] = x[i] + beta * y[i];
+  else if (beta == 1.0)
+    for (int64_t i = 0; i < n; i++) w[i] = alpha * x[i] + y[i];
+  else
+    for (int64_t i = 0; i < n; i++) w[i] = alpha * x[i] + beta * y[i];
+}
+
+/* core/src/impl/DenseVector/Serial/Morpheus_Reduction_Impl.hpp:44-60 (Kahan) */
+double orc_reduce(int64_t n, const double* in) {
+  if (n <= 0) return 0;
+  double sum = in[0], c = 0;
+  for (int64_t i = 1; i < n; i++) {
+    const double y = in[i] - c;
+    const double t = sum + y;
+    c = (t - sum) - y;
+    sum = t;
+  }
+  return sum;
+}
+
+/* Conjugate gradients (textbook, Hestenes-Stiefel), for checking the fused
+ * iteration: x0 given, r = b - A x0, p = r; each iteration q = A p,
+ * alpha = rr / p.q, x += alpha p, r -= alpha q, rr' = r.r, p = r + (rr'/rr) p.
+ * A is CSR.  Returns the final r.r. */
+double orc_cg_csr(int64_t n, const int32_t* ro, const int32_t* ci, const double* v,
+                  const double* b, double* x, int64_t iters) {
+  double* r = (double*)malloc(sizeof(double) * (size_t)n);
+  double* p = (double*)malloc(sizeof(double) * (size_t)n);
+  double* q = (double*)malloc(sizeof(double) * (size_t)n);
+  orc_csr_spmv(n, ro, ci, v, x, q, 1);
+  for (int64_t i = 0; i < n; i++) {
+    r[i] = b[i] - q[i];
+    p[i] = r[i];
+  }
+  double rr = orc_dot(n, r, r);
+  for (int64_t k = 0; k < iters; k++) {
+    orc_csr_spmv(n, ro, ci, v, p, q, 1);
+    const double alpha = rr / orc_dot(n, p, q);
+    for (int64_t i = 0; i < n; i++) {
+      x[i] += alpha * p[i];
+      r[i] -= alpha * q[i];
+    }
+    const double rr2 = orc_dot(n, r, r);
+    const double beta = rr2 / rr;
+    for (int64_t i = 0; i < n; i++) p[i] = r[i] + beta * p[i];
+    rr = rr2;
+  }
+  free(r);
+  free(p);
+  free(q);
+  return rr;
+}
+
+/* ========================================================================
  * Convert (Serial backend)
  * ===================================================================== */
 

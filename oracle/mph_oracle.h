@@ -67,6 +67,12 @@ int orc_coo_is_sorted(int64_t nnz, const int32_t* ri, const int32_t* ci);
 /* stable sort of the first n entries by (row, col); in place. */
 int orc_coo_sort_by_row_and_column(int64_t n, int32_t* ri, int32_t* ci,
                                    double* v);
+double orc_dot(int64_t n, const double* x, const double* y);
+void orc_waxpby(int64_t n, double alpha, const double* x, double beta, const double* y,
+                double* w);
+double orc_reduce(int64_t n, const double* in);
+double orc_cg_csr(int64_t n, const int32_t* ro, const int32_t* ci, const double* v,
+                  const double* b, double* x, int64_t iters);
 void orc_count_nnz_per_diagonal(int64_t nrows, int64_t ncols, int64_t nnz,
                                 const int32_t* ri, const int32_t* ci,
                                 int32_t* out /* nrows+ncols-1, accumulated */);

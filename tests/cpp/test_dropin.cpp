@@ -702,6 +702,36 @@ TEST(Analytics_AllFormats) {
   }
 }
 
+// core/tests/Test_Dot.hpp, Test_WAXPBY.hpp, Test_Reduction.hpp: device and
+// host vectors against the oracle's Serial loops
+TEST(DenseVector_Algorithms) {
+  const int n = 100000;
+  Vec<Host> x = random_x(n, 5), y = random_x(n, 6), w(n, 0.0);
+  Vec<Dev> xd(n), yd(n), wd(n);
+  Morpheus::copy(x, xd);
+  Morpheus::copy(y, yd);
+  const double ref = orc_dot(n, x.data(), y.data());
+  double absdot = 0;
+  for (int i = 0; i < n; i++) absdot += std::abs(x(i) * y(i));
+  EXPECT_TRUE(std::abs(Morpheus::dot<HIPExec>(n, xd, yd) - ref) <= 4.0 * n * 1.2e-16 * absdot);
+  EXPECT_TRUE(std::abs(Morpheus::dot<Serial>(n, x, y) - ref) <= 4.0 * n * 1.2e-16 * absdot);
+  std::vector<double> wref(n);
+  orc_waxpby(n, 2.5, x.data(), -0.5, y.data(), wref.data());
+  Morpheus::waxpby<HIPExec>(n, 2.5, xd, -0.5, yd, wd);
+  Morpheus::copy(wd, w);
+  bool same = true;
+  for (int i = 0; i < n; i++) same = same && w(i) == wref[i];
+  EXPECT_TRUE(same);
+  Vec<Host> wh(n, 0.0);
+  Morpheus::waxpby<Serial>(n, 2.5, x, -0.5, y, wh);
+  same = true;
+  for (int i = 0; i < n; i++) same = same && wh(i) == wref[i];
+  EXPECT_TRUE(same);
+  double absx = 0;
+  for (int i = 0; i < n; i++) absx += std::abs(x(i));
+  EXPECT_TRUE(std::abs(Morpheus::reduce<HIPExec>(xd, n) - orc_reduce(n, x.data())) <= 4e-16 * absx);
+}
+
 // ============================================================ larger cases vs the oracle
 // A random matrix with ragged rows (one long row), converted on the device
 // from CSR to every format and multiplied: conversions are compared with the
