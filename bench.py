@@ -204,6 +204,56 @@ def cold_and_graph(mats, vecs, order, reps):
     return cold, graph
 
 
+def run_cg(mats, iters):
+    """The consumer side: `iters` fused CG iterations on S7 (the SPD 3-D
+    Laplacian) with the DIA and ELL handles of the step, captured once per
+    handle in a hipGraph of 10 iterations and replayed.  Bytes per iteration:
+    the SpMV's bytes_alg plus 12 vector passes of N * 8 bytes (p.q: p, q;
+    update: x, r, p, q read, x, r written; direction: r, p read, p written;
+    the partial sums are a few KB)."""
+    import numpy as np
+    import torch
+    from morpheus_amd import device
+    out = {}
+    for name in ("DIA", "ELL"):
+        if name not in mats:
+            continue
+        D, host = mats[name]
+        n = host.nrows
+        b = torch.empty(n, dtype=torch.float64, device="cuda").uniform_(-1.0, 1.0)
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        cg = device.CG(D)
+        cg.start(b, x)
+        cg.iterate(x, 2)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            cg.iterate(x, 10)
+        g.replay()
+        torch.cuda.synchronize()
+        reps = max(1, iters // 10)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            g.replay()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / (10 * reps)
+        rr = cg.residual()
+        nbytes = D.algorithmic_bytes() + 12 * n * 8
+        out[name] = {"matrix": "S7", "iterations": 10 * reps, "ms_per_iteration": round(ms, 5),
+                     "gbs": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                     "hbm_frac": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "bytes_per_iteration": int(nbytes),
+                     "relative_residual": float(np.sqrt(rr) / np.sqrt(float(torch.dot(b, b))))}
+        log(f"CG on S7 ({name}): {ms * 1e3:.1f} us per iteration, "
+            f"{out[name]['gbs']:.0f} GB/s, relative residual {out[name]['relative_residual']:.1e} "
+            f"after {out[name]['iterations'] + 2} iterations")
+        del g
+        cg.close()
+    return out or None
+
+
 def run_cpu(mats, order, budget):
     import numpy as np
     import oracle  # the checker's OpenMP restatement: cpu_baseline leg only
@@ -519,6 +569,7 @@ def main():
     del sa, sb
     log(f"elementwise 1 GiB read + 1 GiB write: {stream_gbs:.0f} GB/s")
 
+    cg = run_cg(mats, 100)
     batch = None if args.no_batch else run_batch(args.batch_steps)
     sweep = None if (args.no_sweep or only) else run_sweep(z15, s7, 20)
 
@@ -573,6 +624,7 @@ def main():
         "cpu_baseline": cpu,
         "formats": formats,
         "batched_spmm": batch,
+        "cg_iteration": cg,
         "precision_sweep": sweep,
     }
     print(json.dumps(line), flush=True)
