@@ -134,11 +134,13 @@ int mph_matrix_format(mph_matrix_t A);
  *   lanes        : fixed lanes per row for CSR row blocks (1..64, 0 = adaptive)
  *   coo_items    : entries per lane of the COO segmented reduction (4, 8, 16)
  *   rows_per_thread : DIA / ELL rows per thread (1 or 2)
- *   xtile        : x-tiled long rows of CSR / HYB / HDC SpMV (1 = off,
+ *   xtile        : x-tiled long rows of CSR / COO / HYB / HDC SpMV (1 = off,
  *                  2 = on; 0 = on when x exceeds 2 MiB).  Rows longer than
- *                  rows_chunk with sorted columns are summed per 64 KiB tile
+ *                  512 entries with sorted columns are summed per 32 KiB tile
  *                  of x staged in LDS, so their x reads are LDS reads instead
- *                  of scattered cache-line fetches. */
+ *                  of scattered cache-line fetches.  On (x-tiling) or from
+ *                  65,536 rows, HYB / HDC run their ELL / DIA part as its own
+ *                  pass and the COO / CSR part over the rows that have one. */
 typedef struct mph_tuning {
   int32_t rows_chunk;
   int32_t lanes;
@@ -294,6 +296,21 @@ int mph_coo_count_nnz_per_diagonal(int64_t nrows, int64_t ncols, int64_t nnz,
 /* core/src/impl/Morpheus_Utils.hpp:91-105 (host-side decision, in float) */
 int mph_exceeds_tolerance(int64_t num_rows, int64_t num_entries,
                           int64_t num_columns);
+
+/* ------------------------------------------------------------ matrix operations */
+/* Morpheus::update_diagonal<HIP>(A, diagonal) (core/src/Morpheus_MatrixOperations.hpp
+ * :54-58), with the Serial semantics of core/src/impl/<Fmt>/Serial/
+ * Morpheus_MatrixOperations_Impl.hpp: CSR / ELL the first entry of row i in
+ * column i, COO every diagonal entry, DIA the main diagonal's slots (a stored
+ * zero stays zero), HYB = ELL + COO, HDC = DIA + CSR.  diagonal: min(nrows,
+ * ncols) device values of the handle's vector type.  Writes the handle's
+ * value arrays in place and releases its analysis. */
+int mph_matrix_update_diagonal(mph_matrix_t A, const void* diagonal, mph_stream_t stream);
+/* The diagonal the product sees, the sum of every stored (i, i) entry, into
+ * min(nrows, ncols) device values.  The reference's get_diagonal
+ * (core/src/Morpheus_MatrixOperations.hpp:69-73) throws
+ * NotImplementedException. */
+int mph_matrix_get_diagonal(mph_matrix_t A, void* diagonal, mph_stream_t stream);
 
 /* ------------------------------------------------------------ vectors */
 /* Dense-vector algorithms on device vectors of `value_type` (MPH_F64:

@@ -122,6 +122,8 @@ _SIGS = {
     "mph_coo_sort_by_row_and_column": (I, [i64, p, p, p, I, p]),
     "mph_coo_is_sorted": (I, [i64, p, p, pI, p]),
     "mph_coo_count_nnz_per_diagonal": (I, [i64, i64, i64, p, p, p, I, p]),
+    "mph_matrix_update_diagonal": (I, [p, p, p]),
+    "mph_matrix_get_diagonal": (I, [p, p, p]),
     "mph_dot": (I, [i64, p, p, I, C.POINTER(C.c_double), p]),
     "mph_waxpby": (I, [i64, C.c_double, p, C.c_double, p, p, I, p]),
     "mph_reduce": (I, [i64, p, I, C.POINTER(C.c_double), p]),

@@ -344,6 +344,20 @@ def from_coo(A: DeviceMatrix, fmt: int, stream=None) -> DeviceMatrix:
     raise capi.InvalidInputException(f"unknown format {fmt}")
 
 
+# ------------------------------------------------------------ matrix operations
+def update_diagonal(A: DeviceMatrix, diagonal, stream=None):
+    """Morpheus::update_diagonal<HIP>(A, diagonal): A's values in place."""
+    capi.call("mph_matrix_update_diagonal", A.handle, _p(diagonal), stream_handle(stream))
+
+
+def get_diagonal(A: DeviceMatrix, out=None, stream=None):
+    """The diagonal the product sees (sum of the stored (i, i) entries)."""
+    if out is None:
+        out = torch.zeros(min(A.nrows, A.ncols), dtype=vector_dtype(A.dtype), device="cuda")
+    capi.call("mph_matrix_get_diagonal", A.handle, _p(out), stream_handle(stream))
+    return out
+
+
 # ------------------------------------------------------------ vector algorithms
 def _vt_of(v) -> int:
     return capi.F64 if v.dtype == torch.float64 else capi.F32

@@ -899,6 +899,62 @@ typename Vector::size_type count_nnz(const Vector& vec, typename Vector::value_t
   return (typename Vector::size_type)Impl::int_stats(vec, vec.size(), 0.0, (int)threshold).count;
 }
 
+// ============================================================ matrix operations
+// core/src/Morpheus_MatrixOperations.hpp:54-58, with the Serial semantics of
+// core/src/impl/<Fmt>/Serial/Morpheus_MatrixOperations_Impl.hpp (CSR / ELL
+// the first entry of row i in column i, COO every diagonal entry, DIA the
+// main diagonal's slots with stored zeros kept, HYB = ELL + COO, HDC = DIA +
+// CSR).  A's values change in place; its cached analysis is rebuilt on the
+// next product.
+template <typename ExecSpace, typename Matrix, typename Vector>
+void update_diagonal(Matrix& A, const Vector& diagonal) {
+  static_assert(is_dense_vector_format_container_v<Vector>, "diagonal must be a DenseVector");
+  if (diagonal.size() < std::min(A.nrows(), A.ncols()))
+    throw InvalidInputException("Morpheus::update_diagonal: diagonal too short");
+  if constexpr (is_dynamic_matrix_format_container_v<Matrix>) {
+    std::visit([&](auto& a) { update_diagonal<ExecSpace>(a, diagonal); }, A.formats());
+  } else {
+    Impl::check_access<ExecSpace, Matrix>();
+    auto& dm = const_cast<Vector&>(diagonal);
+    if constexpr (Matrix::memory_space::is_device) {
+      Impl::with_device_vector(dm, true, false, [&](auto* pd) {
+        Impl::check(mph_matrix_update_diagonal(Impl::matrix_handle(A), pd, nullptr),
+                    "Morpheus::update_diagonal");
+      });
+    } else {
+      Impl::device_type_of<Matrix> dev;
+      dev.allocate(A);
+      Impl::copy_sparse(A, dev);
+      mph_matrix_t h = Impl::create_handle(dev);
+      int st = MPH_OK;
+      Impl::with_device_vector(dm, true, false, [&](auto* pd) {
+        st = mph_matrix_update_diagonal(h, pd, nullptr);
+      });
+      mph_matrix_destroy(h);
+      Impl::check(st, "Morpheus::update_diagonal");
+      Impl::copy_sparse(dev, A);
+      A.touch();  // the staged copy of A held the old values
+    }
+  }
+}
+
+// The diagonal the product sees: the sum of every stored (i, i) entry, into
+// min(nrows, ncols) entries.  The reference's get_diagonal
+// (core/src/Morpheus_MatrixOperations.hpp:69-73) throws NotImplementedException.
+template <typename ExecSpace, typename Matrix, typename Vector>
+void get_diagonal(const Matrix& A, Vector& diagonal) {
+  static_assert(is_dense_vector_format_container_v<Vector>, "diagonal must be a DenseVector");
+  const size_t n = std::min(A.nrows(), A.ncols());
+  if (diagonal.size() < n) throw InvalidInputException("Morpheus::get_diagonal: diagonal too short");
+  Impl::with_concrete(A, [&](const auto& a) {
+    Impl::check_access<ExecSpace, std::decay_t<decltype(a)>>();
+    Impl::with_device_vector(diagonal, false, true, [&](auto* pd) {
+      Impl::check(mph_matrix_get_diagonal(Impl::matrix_handle(a), pd, nullptr),
+                  "Morpheus::get_diagonal");
+    });
+  });
+}
+
 // ============================================================ extensions
 // Launch-shape autotuning of one HIPSpace matrix (no reference counterpart):
 // times the admissible launch shapes of A on x / y and keeps the fastest.

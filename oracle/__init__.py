@@ -58,6 +58,10 @@ _SIGS = {
     "orc_coo_is_sorted": (I, [i64, p, p]),
     "orc_coo_sort_by_row_and_column": (I, [i64, p, p, p]),
     "orc_count_nnz_per_diagonal": (None, [i64, i64, i64, p, p, p]),
+    "orc_csr_update_diagonal": (None, [i64, p, p, p, p]),
+    "orc_coo_update_diagonal": (None, [i64, p, p, p, p]),
+    "orc_dia_update_diagonal": (None, [i64, i64, i64, i64, p, p, p]),
+    "orc_ell_update_diagonal": (None, [i64, i64, i64, p, p, p]),
     "orc_dot": (C.c_double, [i64, p, p]),
     "orc_waxpby": (None, [i64, C.c_double, p, C.c_double, p, p]),
     "orc_reduce": (C.c_double, [i64, p]),
@@ -203,6 +207,48 @@ def count_nnz_per_diagonal(A: Coo) -> np.ndarray:
     lib().orc_count_nnz_per_diagonal(A.nrows, A.ncols, A.nnz, _ptr(A.row_indices),
                                      _ptr(A.column_indices), _ptr(out))
     return out
+
+
+# ------------------------------------------------------------ matrix operations
+def update_diagonal(A, diag):
+    """Morpheus::update_diagonal (Serial) on a copy of the host container;
+    returns the updated copy."""
+    import copy
+    B = copy.deepcopy(A)
+    d = np.ascontiguousarray(diag, dtype=np.float64)
+    L = lib()
+    if isinstance(B, Csr):
+        L.orc_csr_update_diagonal(B.nrows, _ptr(B.row_offsets), _ptr(B.column_indices),
+                                  _ptr(B.values), _ptr(d))
+    elif isinstance(B, Coo):
+        L.orc_coo_update_diagonal(B.nnz, _ptr(B.row_indices), _ptr(B.column_indices),
+                                  _ptr(B.values), _ptr(d))
+    elif isinstance(B, Dia):
+        L.orc_dia_update_diagonal(B.nrows, B.ncols, B.ndiag, B.pitch, _ptr(B.offsets),
+                                  _ptr(B.values), _ptr(d))
+    elif isinstance(B, Ell):
+        L.orc_ell_update_diagonal(B.nrows, B.width, B.pitch, _ptr(B.column_indices),
+                                  _ptr(B.values), _ptr(d))
+    elif isinstance(B, Hyb):
+        # core/src/impl/Hyb/Serial/Morpheus_MatrixOperations_Impl.hpp: ELL, then COO
+        B.ell = update_diagonal(B.ell, d)
+        B.coo = update_diagonal(B.coo, d)
+    elif isinstance(B, Hdc):
+        B.dia = update_diagonal(B.dia, d)
+        B.csr = update_diagonal(B.csr, d)
+    else:
+        raise TypeError(type(B))
+    return B
+
+
+def get_diagonal(A) -> np.ndarray:
+    """The diagonal the product sees: the sum of every stored (i, i) entry
+    (the reference's get_diagonal throws NotImplementedException)."""
+    C_ = to_coo(A)
+    d = np.zeros(A.nrows)
+    m = C_.row_indices == C_.column_indices
+    np.add.at(d, C_.row_indices[m], C_.values[m])
+    return d
 
 
 # ------------------------------------------------------------ vector algorithms

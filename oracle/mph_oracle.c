@@ -379,6 +379,54 @@ double orc_cg_csr(int64_t n, const int32_t* ro, const int32_t* ci, const double*
 }
 
 /* ========================================================================
+ * Matrix operations (Serial backend)
+ * ===================================================================== */
+
+/* core/src/impl/Csr/Serial/Morpheus_MatrixOperations_Impl.hpp:51-60: the
+ * first entry of row i in column i */
+void orc_csr_update_diagonal(int64_t nrows, const int32_t* ro, const int32_t* ci, double* v,
+                             const double* d) {
+  for (int64_t i = 0; i < nrows; i++)
+    for (int32_t jj = ro[i]; jj < ro[i + 1]; jj++)
+      if (ci[jj] == i) {
+        v[jj] = d[i];
+        break;
+      }
+}
+
+/* core/src/impl/Coo/Serial/Morpheus_MatrixOperations_Impl.hpp:49-55: every
+ * diagonal entry */
+void orc_coo_update_diagonal(int64_t nnz, const int32_t* ri, const int32_t* ci, double* v,
+                             const double* d) {
+  for (int64_t n = 0; n < nnz; n++)
+    if (ri[n] == ci[n]) v[n] = d[ci[n]];
+}
+
+/* core/src/impl/Dia/Serial/Morpheus_MatrixOperations_Impl.hpp:52-66: the main
+ * diagonal's slots, a stored zero staying zero */
+void orc_dia_update_diagonal(int64_t nrows, int64_t ncols, int64_t ndiag, int64_t pitch,
+                             const int32_t* off, double* v, const double* d) {
+  for (int64_t row = 0; row < nrows; row++)
+    for (int64_t n = 0; n < ndiag; n++) {
+      const int64_t col = row + off[n];
+      if (col >= 0 && col < ncols && col == row)
+        v[row + pitch * n] = (v[row + pitch * n] == 0) ? 0 : d[col];
+    }
+}
+
+/* core/src/impl/Ell/Serial/Morpheus_MatrixOperations_Impl.hpp:53-66: the
+ * first slot of row i holding column i */
+void orc_ell_update_diagonal(int64_t nrows, int64_t width, int64_t pitch, const int32_t* ci,
+                             double* v, const double* d) {
+  for (int64_t i = 0; i < nrows; i++)
+    for (int64_t n = 0; n < width; n++)
+      if (ci[i + pitch * n] == i) {
+        v[i + pitch * n] = d[i];
+        break;
+      }
+}
+
+/* ========================================================================
  * Convert (Serial backend)
  * ===================================================================== */
 

@@ -67,6 +67,14 @@ int orc_coo_is_sorted(int64_t nnz, const int32_t* ri, const int32_t* ci);
 /* stable sort of the first n entries by (row, col); in place. */
 int orc_coo_sort_by_row_and_column(int64_t n, int32_t* ri, int32_t* ci,
                                    double* v);
+void orc_csr_update_diagonal(int64_t nrows, const int32_t* ro, const int32_t* ci, double* v,
+                             const double* d);
+void orc_coo_update_diagonal(int64_t nnz, const int32_t* ri, const int32_t* ci, double* v,
+                             const double* d);
+void orc_dia_update_diagonal(int64_t nrows, int64_t ncols, int64_t ndiag, int64_t pitch,
+                             const int32_t* off, double* v, const double* d);
+void orc_ell_update_diagonal(int64_t nrows, int64_t width, int64_t pitch, const int32_t* ci,
+                             double* v, const double* d);
 double orc_dot(int64_t n, const double* x, const double* y);
 void orc_waxpby(int64_t n, double alpha, const double* x, double beta, const double* y,
                 double* w);

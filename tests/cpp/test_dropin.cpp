@@ -732,6 +732,61 @@ TEST(DenseVector_Algorithms) {
   EXPECT_TRUE(std::abs(Morpheus::reduce<HIPExec>(xd, n) - orc_reduce(n, x.data())) <= 4e-16 * absx);
 }
 
+// core/tests/Test_MatrixOperations.hpp: update_diagonal in every format, on
+// HIPSpace through HIP and on HostSpace through Serial, then the dense
+// diagonal is the new one; a product afterwards sees the new values
+TEST(MatrixOperations_UpdateDiagonal) {
+  const int n = 200;
+  Coo<Host> R = analytics_matrix(n, false);
+  Vec<Host> d = random_x(n, 17);
+  Vec<Dev> dd(n);
+  Morpheus::copy(d, dd);
+  auto check_diag = [&](const Coo<Host>& C) {
+    bool ok = true;
+    for (size_t k = 0; k < C.nnnz(); k++)
+      if (C.crow_indices(k) == C.ccolumn_indices(k)) ok = ok && C.cvalues(k) == d(C.crow_indices(k));
+    return ok;
+  };
+  for_each_small([&](const auto& proto) {
+    using M = std::decay_t<decltype(proto)>;
+    M Ah;
+    Morpheus::convert<Serial>(R, Ah);
+    auto Ad = Morpheus::create_mirror_container<Dev>(Ah);
+    Morpheus::copy(Ah, Ad);
+    Morpheus::update_diagonal<HIPExec>(Ad, dd);
+    M back;
+    back = Morpheus::create_mirror(Ad);
+    Morpheus::copy(Ad, back);
+    Coo<Host> C;
+    Morpheus::convert<Serial>(back, C);
+    EXPECT_TRUE(check_diag(C));
+    Morpheus::update_diagonal<Serial>(Ah, d);
+    Coo<Host> C2;
+    Morpheus::convert<Serial>(Ah, C2);
+    EXPECT_TRUE(check_diag(C2));
+    Vec<Dev> gd(n, 0.0);
+    Morpheus::get_diagonal<HIPExec>(Ad, gd);
+    Vec<Host> gh(n, 0.0);
+    Morpheus::copy(gd, gh);
+    bool ok = true;
+    for (int i = 0; i < n; i++) ok = ok && gh(i) == d(i);
+    EXPECT_TRUE(ok);
+    // a product on the host container after the update sees the new values
+    Vec<Host> x = random_x(n, 3), y(n, 0.0);
+    Morpheus::multiply<Serial>(Ah, x, y);
+    std::vector<double> yref(n);
+    std::vector<int> ri(C2.nnnz()), ci(C2.nnnz());
+    for (size_t k = 0; k < C2.nnnz(); k++) {
+      ri[k] = C2.crow_indices(k);
+      ci[k] = C2.ccolumn_indices(k);
+    }
+    orc_coo_spmv(n, C2.nnnz(), ri.data(), ci.data(), C2.values().data(), x.data(), yref.data(), 1);
+    ok = true;
+    for (int i = 0; i < n; i++) ok = ok && std::abs(y(i) - yref[i]) <= 1e-12 * (1 + std::abs(yref[i]));
+    EXPECT_TRUE(ok);
+  });
+}
+
 // ============================================================ larger cases vs the oracle
 // A random matrix with ragged rows (one long row), converted on the device
 // from CSR to every format and multiplied: conversions are compared with the
