@@ -288,7 +288,7 @@ def run_batch(steps):
     import torch
     from morpheus_amd import device
     from morpheus_amd import generators as gen
-    from morpheus_amd.formats import COO_FORMAT, CSR_FORMAT, FORMAT_NAMES
+    from morpheus_amd.formats import FORMAT_NAMES
 
     K = 8
     t0 = time.time()
@@ -319,13 +319,7 @@ def run_batch(steps):
     # format choice per matrix, by timed trials of the FP32 product with k = 8
     Dcsr32 = [device.to_device(A, "f32") for A in mats]
     choice = [device.autotune_format(D, k=K, reps=3)[0] for D in Dcsr32]
-    # the batch takes COO matrices as CSR (the persistent kernel's tiles are
-    # row blocks); COO wins only by a hair where the standalone timing picks it
-    n_coo = choice.count(COO_FORMAT)
-    choice = [CSR_FORMAT if f == COO_FORMAT else f for f in choice]
     hist = {FORMAT_NAMES[f]: choice.count(f) for f in sorted(set(choice))}
-    if n_coo:
-        hist["COO->CSR"] = n_coo
     log(f"batch: autotuned formats {hist} in {time.time() - t0:.1f}s")
 
     res = {"matrices": len(mats), "k": K, "nnz": int(nnz), "formats_chosen": hist}

@@ -381,6 +381,8 @@ int mph_i32_stats(const int32_t* v, int64_t n, double mean, int32_t threshold,
  * in one persistent launch over a work queue of (matrix, row block) tiles.
  * No reference counterpart (the reference multiplies one matrix by one
  * vector per call).  Matrices are given as handles; their formats may differ.
+ * A COO member must have its entries sorted by row (its row offsets are
+ * derived on the device and kept by the batch); an unsorted one is refused.
  * X[i] is ncols_i x k and Y[i] nrows_i x k, row-major. */
 typedef struct mph_batch_s* mph_batch_t;
 int mph_batch_create(mph_batch_t* out, int32_t nmat, const mph_matrix_t* mats,

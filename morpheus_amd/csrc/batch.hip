@@ -400,6 +400,7 @@ struct mph_batch_s {
   int32_t* d_counter = nullptr;
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t exec[2] = {nullptr, nullptr};
+  std::vector<void*> owned;  // row offsets derived for COO members
 };
 
 namespace mph {
@@ -470,11 +471,83 @@ int mph_batch_create(mph_batch_t* out, int32_t nmat, const mph_matrix_t* mats, i
   std::vector<BatchTile> tiles;
   std::vector<int64_t> cost;
   int vt = nmat > 0 ? mats[0]->vt : MPH_F32;
+  // COO members are batched as the CSR view of their row-sorted entries: row
+  // offsets derived on the device (owned by the batch) and the row blocks of
+  // that view
+  std::vector<void*> owned;
+  struct OwnedGuard {
+    std::vector<void*>* v;
+    bool keep = false;
+    ~OwnedGuard() {
+      if (!keep)
+        for (void* p : *v) dfree(p);
+    }
+  } guard{&owned};
   for (int32_t i = 0; i < nmat; i++) {
     mph_matrix_impl* A = mats[i];
     if (!A) return invalid("null matrix in batch");
     if (A->vt != vt) return invalid("all batched matrices must share one value type");
-    if (A->format == MPH_COO) return invalid("COO matrices are batched through CSR");
+    if (A->format == MPH_COO) {
+      int sorted = 1;
+      MPH_CHECK_STATUS(check_index_range(A->coo_ri, A->coo_nnz, A->nrows, false, s, "COO rows"));
+      MPH_CHECK_STATUS(check_index_range(A->coo_ci, A->coo_nnz, A->ncols, false, s, "COO columns"));
+      MPH_CHECK_STATUS(rows_sorted(A->coo_ri, A->coo_nnz, A->nrows, s, &sorted));
+      if (!sorted) return invalid("a COO matrix with unsorted rows cannot be batched");
+      int32_t* d_ro = nullptr;
+      MPH_HIP_CHECK(hipMalloc(&d_ro, sizeof(int32_t) * (A->nrows + 1)));
+      owned.push_back(d_ro);
+      MPH_CHECK_STATUS(coo_row_offsets(A->coo_ri, A->coo_nnz, A->nrows, d_ro, s));
+      mph_matrix_s view;
+      view.format = MPH_CSR;
+      view.vt = A->vt;
+      view.nrows = A->nrows;
+      view.ncols = A->ncols;
+      view.nnz = view.csr_nnz = A->coo_nnz;
+      view.csr_ro = d_ro;
+      view.csr_ci = A->coo_ci;
+      view.csr_v = A->coo_v;
+      view.tuning = A->tuning;
+      view.tuning.xtile = 1;  // the batch reads row blocks only
+      BatchMat m{};
+      m.nrows = A->nrows;
+      m.ncols = A->ncols;
+      m.x = X[i];
+      m.y = Y[i];
+      if (A->nrows > 0 && (!X[i] || !Y[i])) return invalid("null X / Y in batch");
+      if ((k * vector_size(vt)) % 16 == 0 && ((reinterpret_cast<uintptr_t>(X[i]) % 16) != 0 ||
+                                              (reinterpret_cast<uintptr_t>(Y[i]) % 16) != 0))
+        return invalid("batched X / Y rows must be 16-byte aligned");
+      m.ro = d_ro;
+      m.ci = A->coo_ci;
+      m.val = A->coo_v;
+      m.pre = BPRE_NONE;
+      int st = rowblock_analyse(&view, s);
+      if (st == MPH_OK && A->nrows > 0) {
+        std::vector<int32_t> ro(A->nrows + 1);
+        std::vector<RowBlock> bl(view.rb.nblocks + 1);
+        hipError_t e = hipMemcpyAsync(ro.data(), d_ro, sizeof(int32_t) * ro.size(),
+                                      hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess)
+          e = hipMemcpyAsync(bl.data(), view.rb.blocks, sizeof(RowBlock) * bl.size(),
+                             hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+          release_plans(&view);
+          MPH_HIP_CHECK(e);
+        }
+        for (int32_t q = 0; q < view.rb.nblocks; q++) {
+          if (bl[q].sub > 0) continue;  // long rows: one whole-row tile
+          BatchTile t{i, bl[q].sub < 0 ? TILE_SHORT : TILE_WHOLEROW, bl[q].row,
+                      bl[q].sub < 0 ? bl[q + 1].row : bl[q].row + 1};
+          tiles.push_back(t);
+          cost.push_back(tile_cost(t, &ro, A));
+        }
+      }
+      release_plans(&view);
+      if (st != MPH_OK) return st;
+      hm[i] = m;
+      continue;
+    }
     if (!A->analysed) MPH_CHECK_STATUS(mph_matrix_analyse(mats[i], stream));
     if (A->format == MPH_HYB && A->rb.tail_ro == nullptr && A->coo_nnz > 0)
       return invalid("HYB with an unsorted COO part cannot be batched");
@@ -549,6 +622,8 @@ int mph_batch_create(mph_batch_t* out, int32_t nmat, const mph_matrix_t* mats, i
   qoff[kQueues] = (int32_t)sorted.size();
 
   mph_batch_s* b = new mph_batch_s();
+  b->owned = owned;
+  guard.keep = true;
   b->vt = vt;
   b->k = k;
   b->ntiles = (int32_t)sorted.size();
@@ -622,6 +697,7 @@ int mph_batch_destroy(mph_batch_t b) {
   dfree(b->d_woff);
   dfree(b->d_qoff);
   dfree(b->d_counter);
+  for (void* p : b->owned) dfree(p);
   delete b;
   return MPH_OK;
 }
