@@ -50,7 +50,7 @@ def parse():
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the library's default launch shapes (no autotuning)")
     ap.add_argument("--no-sweep", action="store_true",
-                    help="skip the FP32 / BF16 precision sweep (S7, Z15, B9)")
+                    help="skip the format x precision sweep (FP64 / FP32 / BF16 on S7, Z15, B9)")
     ap.add_argument("--formats", default="",
                     help="comma-separated subset of the step's formats (profiling one "
                          "format at a time; the JSON line then covers that subset)")
@@ -60,16 +60,19 @@ def parse():
 def product_kernels(fmt, tuning):
     """Kernels one FP64 product launches for `fmt` on the bench matrices, in
     launch order, with the launch shape `tuning` picked: Z15's long rows take
-    the x-tiled pass; HYB runs its ELL part as its own pass and the COO tail
-    over the rows that have one; S7's HDC has an empty CSR part."""
+    the x-tiled pass; row-sorted COO runs the row-block plan unless a
+    segmented shape (coo_items) was picked; Z15's HYB (ELL slots about 10 %
+    filled) runs over its merged CSR copy; S7's HDC has an empty CSR part."""
     rpt = 2 if tuning.get("rows_per_thread") == 2 else 1
-    items = tuning.get("coo_items") or 8
+    items = tuning.get("coo_items") or 0
     xt = ["xtile_kernel<double, false>", "xtile_combine_kernel<double, 0, false>"]
+    rows = ["rowblock_spmv_kernel<double, 0, false>", *xt]
     return {
-        "CSR": ["rowblock_spmv_kernel<double, 0, false>", *xt],
-        "COO": ["coo_zero_kernel<double>", *xt, f"coo_segmented_kernel<double, {items}, false>",
-                f"coo_fixup_kernel<double, {items}, false>"],
-        "HYB": [f"ell_kernel<double, 1, {rpt}>", "rowblock_spmv_kernel<double, 0, false>", *xt],
+        "CSR": rows,
+        "COO": rows if items == 0 else [
+            "coo_zero_kernel<double>", *xt, f"coo_segmented_kernel<double, {items}, false>",
+            f"coo_fixup_kernel<double, {items}, false>"],
+        "HYB": rows,
         "HDC": [f"dia_kernel<double, 1, {rpt}>"],
         "DIA": [f"dia_kernel<double, 1, {rpt}>"],
         "ELL": [f"ell_kernel<double, 1, {rpt}>"],
@@ -399,12 +402,15 @@ def run_batch(steps):
 
 
 def run_sweep(z15, s7, steps):
-    """Precision sweep: S7, Z15 and B9 (4M rows, bandwidth 9, 2 % off-band)
-    stored in FP32 and BF16 (int32 indices, FP32 accumulation, TwoSum on rows
-    longer than 1k), every format the reference admits for the matrix; per
-    (matrix, format, value type) the product time and the max-row relative
-    error max_i |y_i - y_ref_i| / sum_j |a_ij x_j| against the compensated
-    FP64 oracle on the original FP64 matrix and x."""
+    """Format x precision sweep: S7, Z15 and B9 (4M rows, bandwidth 9, 2 %
+    off-band) stored in FP64, FP32 and BF16 (int32 indices; FP32 accumulation
+    with TwoSum on rows longer than 1k for the reduced types), every format
+    the reference admits for the matrix; per (matrix, format, value type) the
+    product time, its share of the 8 TB/s roofline on the algorithmic bytes,
+    and the max-row relative error max_i |y_i - y_ref_i| / sum_j |a_ij x_j|
+    against the compensated FP64 oracle on the original FP64 matrix and x
+    (FP64 also reports whether every row is within 4 nnz_row u sum_j
+    |a_ij x_j|)."""
     import numpy as np
     import torch
     import oracle  # the checker: FP64 reference products and row scales
@@ -422,8 +428,9 @@ def run_sweep(z15, s7, steps):
         yref = oracle.spmv_compensated(A, x)
         scale = oracle.absrow(A, x)
         mask = scale > 0
+        bound64 = 4.0 * np.diff(A.row_offsets) * 2.0 ** -53 * scale
         out[mname] = {}
-        for dtype in ("f32", "bf16"):
+        for dtype in ("f64", "f32", "bf16"):
             base = device.to_device(A, dtype)
             vdt = device.vector_dtype(dtype)
             xt = torch.from_numpy(x).cuda().to(vdt)
@@ -446,11 +453,17 @@ def run_sweep(z15, s7, steps):
                 ms = e0.elapsed_time(e1) / steps
                 yh = y.double().cpu().numpy()
                 err = float(np.max(np.abs(yh[mask] - yref[mask]) / scale[mask])) if mask.any() else 0.0
-                out[mname].setdefault(fname, {})[dtype] = {
+                gbs = D.algorithmic_bytes() / (ms * 1e-3) / 1e9
+                rec = {
                     "ms": round(ms, 5),
                     "gflops": round(2.0 * A.nnz / (ms * 1e-3) / 1e9, 2),
+                    "gbs": round(gbs, 1),
+                    "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
                     "max_rel_err": err,
                 }
+                if dtype == "f64":
+                    rec["within_fp64_bound"] = bool(np.all(np.abs(yh - yref) <= bound64))
+                out[mname].setdefault(fname, {})[dtype] = rec
                 if D is not base:
                     D.close()
             base.close()
@@ -461,6 +474,8 @@ def run_sweep(z15, s7, steps):
     out["max_rel_err"] = {dt: max(v[dt]["max_rel_err"] for m in ("S7", "Z15", "B9")
                                   for v in out[m].values() if isinstance(v.get(dt), dict))
                           for dt in ("f32", "bf16")}
+    out["fp64_all_within_bound"] = all(v["f64"]["within_fp64_bound"] for m in ("S7", "Z15", "B9")
+                                       for v in out[m].values() if isinstance(v.get("f64"), dict))
     return out
 
 

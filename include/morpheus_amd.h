@@ -132,7 +132,10 @@ int mph_matrix_format(mph_matrix_t A);
 /* Launch-shape knobs the autotuner searches (0 = library default).
  *   rows_chunk   : nnz budget of a CSR row block (short rows) / long-row chunk
  *   lanes        : fixed lanes per row for CSR row blocks (1..64, 0 = adaptive)
- *   coo_items    : entries per lane of the COO segmented reduction (4, 8, 16)
+ *   coo_items    : entries per lane of the COO segmented reduction (4, 8, 16);
+ *                  0 = a row-sorted COO matrix runs the CSR row-block plan
+ *                  over row offsets derived from its row indices (rows_chunk
+ *                  and lanes apply), an unsorted one the atomic pass
  *   rows_per_thread : DIA / ELL rows per thread (1 or 2)
  *   xtile        : x-tiled long rows of CSR / COO / HYB / HDC SpMV (1 = off,
  *                  2 = on; 0 = on when x exceeds 2 MiB).  Rows longer than
@@ -140,7 +143,10 @@ int mph_matrix_format(mph_matrix_t A);
  *                  of x staged in LDS, so their x reads are LDS reads instead
  *                  of scattered cache-line fetches.  On (x-tiling) or from
  *                  65,536 rows, HYB / HDC run their ELL / DIA part as its own
- *                  pass and the COO / CSR part over the rows that have one. */
+ *                  pass and the COO / CSR part over the rows that have one;
+ *                  a HYB whose ELL slots are less than a third filled runs
+ *                  instead as CSR over a merged copy made by the analysis
+ *                  (per row its ELL entries, then its COO entries). */
 typedef struct mph_tuning {
   int32_t rows_chunk;
   int32_t lanes;

@@ -88,7 +88,10 @@ int mph_matrix_autotune(mph_matrix_t A, int32_t k, const void* x, void* y, int32
   std::vector<mph_tuning> cands;
   switch (A->format) {
     case MPH_COO:
+      // the segmented reduction, and the row-block plan (row-sorted only)
       for (int it : {4, 8, 16}) cands.push_back(mph_tuning{0, 0, it, 0});
+      for (int chunk : {0, 512})
+        for (int lanes : {0, 4, 16}) cands.push_back(mph_tuning{chunk, lanes, 0, 0});
       break;
     case MPH_DIA:
     case MPH_ELL:

@@ -37,7 +37,12 @@ void release_plans(mph_matrix_impl* A) {
   dfree(p.blocks);
   if (p.partial) dfree(p.partial);
   if (p.tail_ro) dfree(p.tail_ro);
+  dfree(p.m_ro);
+  dfree(p.m_ci);
+  dfree(p.m_val);
   if (p.sblocks) dfree(p.sblocks);
+  dfree(p.blanes);
+  dfree(p.sblanes);
   if (p.smap) dfree(p.smap);
   if (p.sro) dfree(p.sro);
   xtile_release(p.xt);

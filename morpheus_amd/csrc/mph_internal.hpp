@@ -43,6 +43,9 @@ constexpr int64_t kXTileMinRowLen = 512;       // rows longer than this are cand
 // HYB / HDC with at least this many rows run the ELL / DIA part as its own
 // coalesced pass and the tail / CSR part over the rows that have one.
 constexpr int64_t kSplitMinRows = 1 << 16;
+// HYB from kSplitMinRows rows (or with x-tiling on) whose ELL slots are
+// filled below this fraction runs over a merged CSR copy
+constexpr double kHybMergeMaxFill = 1.0 / 3.0;
 
 // The x-tiled pass of one matrix's long rows (see xtile_kernel).
 struct XTilePlan {
@@ -68,8 +71,22 @@ struct RowBlockPlan {
   int32_t nblocks = 0;
   RowBlock* blocks = nullptr;    // nblocks + 1 (sentinel)
   void* partial = nullptr;       // per-block accumulators (long-row chunks)
-  int32_t* tail_ro = nullptr;    // HYB only: row offsets of the COO tail
+  int32_t* tail_ro = nullptr;    // HYB / COO: row offsets of the COO entries
+  // the CSR-shaped arrays the row pass reads: CSR / HDC's CSR part, the COO
+  // arrays over tail_ro, or the merged copy below
+  const int32_t* ro = nullptr;
+  const int32_t* ci = nullptr;
+  const void* val = nullptr;
+  // HYB whose ELL part is mostly padding: a CSR copy of the whole matrix
+  // (per row its ELL slots in slot order, then its COO entries), owned here
+  int32_t* m_ro = nullptr;
+  int32_t* m_ci = nullptr;
+  void* m_val = nullptr;
   int32_t nlong = 0;             // rows split into chunks
+  // lanes per row of each short block's reduction (blocks / sblocks order),
+  // picked by the analysis; 0 for chunk and tiled records
+  uint8_t* blanes = nullptr;
+  uint8_t* sblanes = nullptr;
   // x-tiled long rows (SpMV); xt.rows == 0: long rows use position chunks
   XTilePlan xt;
   // split SpMV plan (x-tiled rows, and / or HYB / HDC in two passes):
@@ -96,6 +113,7 @@ struct CooPlan {
   void* last_val = nullptr;    // nchunks accumulators
   bool sorted = true;          // rows non-decreasing
   bool prezero = false;        // y cleared by a fill pass instead of gap writes
+  bool rowplan = false;        // row-sorted, coo_items 0: the row-block plan
   // with x-tiled long rows, the segmented pass covers the other entries only,
   // through an explicit chunk table
   XTilePlan xt;
@@ -167,7 +185,9 @@ int check_index_range(const int32_t* idx, int64_t n, int64_t hi, bool allow_m1,
 int check_row_offsets(const int32_t* ro, int64_t nrows, int64_t nnz,
                       hipStream_t s, const char* what);
 int validate_rowblock_plan(const int32_t* d_ro, int64_t nrows, RowBlockPlan& p,
-                           hipStream_t s);
+                           hipStream_t s, std::vector<uint8_t>* lanes = nullptr);
+// lanes per row for a short block of nr rows of lengths len(0..nr-1)
+uint8_t block_lanes(int nr, const int32_t* ro_begin);
 
 // x-tiled long rows.  xtile_build plans the candidate rows `lrows` (host row
 // offsets `ro`, device row offsets / columns / values) when the x-tile mode

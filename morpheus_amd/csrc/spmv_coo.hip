@@ -389,6 +389,12 @@ int coo_analyse(mph_matrix_impl* A, hipStream_t s) {
   int sorted = 1;
   MPH_CHECK_STATUS(rows_sorted(A->coo_ri, A->nnz, A->nrows, s, &sorted));
   p.sorted = sorted;
+  // row-sorted and no segmented shape asked for: the row-block plan over
+  // row offsets derived from the row indices (no copy of the entries)
+  if (sorted && A->tuning.coo_items == 0) {
+    p.rowplan = true;
+    return rowblock_analyse(A, s);
+  }
   p.nchunks = ceil_div(A->nnz, (int64_t)kWave * p.items);
   p.nblocks = (int32_t)ceil_div(p.nchunks, 256 / kWave);
   p.prezero = false;
@@ -489,6 +495,7 @@ int coo_run(mph_matrix_impl* A, const void* x, void* y, int init, int64_t ld,
 // reading x and y with element stride k.
 int coo_multiply(mph_matrix_impl* A, int k, const void* x, void* y, int init,
                  hipStream_t s) {
+  if (A->coo.rowplan) return A->nrows == 0 ? MPH_OK : rowblock_multiply(A, k, x, y, init, s);
   const size_t vec = A->vt == MPH_F64 ? 8 : 4;
   for (int j = 0; j < k; j++) {
     const void* xj = static_cast<const char*>(x) + vec * j;

@@ -66,6 +66,8 @@ struct RBArgs {
   // split plan: block rows are list rows, list row i is matrix row rmap[i]
   // (nullptr: the identity); used with PRE_NONE only
   const int32_t* __restrict__ rmap;
+  // lanes per row of each short block (nullptr: from the block's mean)
+  const uint8_t* __restrict__ blanes;
 };
 
 template <typename V>
@@ -268,7 +270,18 @@ __global__ __launch_bounds__(kRowBlockThreads) void rowblock_spmv_kernel(
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ Acc s_red[2 * kRowBlockThreads / kWave];
   const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
-  const RowBlock blk = a.blocks[b];
+  // the record and the next one's row (the sentinel bounds the last block)
+  // as one scalar load
+  int32_t rec[3];
+  __builtin_memcpy(rec, &a.blocks[b], sizeof(rec));
+  const RowBlock blk{rec[0], rec[1]};
+  const int32_t next_row = rec[2];
+  // the block's lane count is read with its record, as a dword of the
+  // (4-byte padded) table, so that it is a scalar load that arrives with the
+  // record instead of a vector load waited on before the product phase
+  const int lanes_b =
+      a.blanes ? (int)((reinterpret_cast<const uint32_t*>(a.blanes)[b >> 2] >> (8 * (b & 3))) & 0xffu)
+               : 0;
   if (blk.sub < -1) return;  // a long row handled by the x-tiled pass
   if (blk.sub >= 0) {
     long_chunk_spmv<V, COMP>(a, b, blk, s_red);
@@ -279,7 +292,7 @@ __global__ __launch_bounds__(kRowBlockThreads) void rowblock_spmv_kernel(
   int32_t* s_ro = reinterpret_cast<int32_t*>(s_pre + (PRE ? kRowBlockMaxRows : 0));
 
   const int64_t row0 = blk.row;
-  const int64_t row1 = a.blocks[b + 1].row;
+  const int64_t row1 = next_row;
   const int nr = (int)(row1 - row0);
   const int64_t p0 = a.ro[row0];
   const int n = (int)(a.ro[row1] - p0);
@@ -297,7 +310,9 @@ __global__ __launch_bounds__(kRowBlockThreads) void rowblock_spmv_kernel(
   }
   __syncthreads();
 
-  const int L = choose_lanes(n, nr, a.lanes_fixed);
+  const int L = a.lanes_fixed > 0 ? a.lanes_fixed
+                : lanes_b > 0    ? lanes_b
+                                 : choose_lanes(n, nr, 0);
   const int groups = kRowBlockThreads / L;
   const int g = threadIdx.x / L, l = threadIdx.x % L;
   for (int r = g; r < nr; r += groups) {
@@ -883,6 +898,47 @@ __global__ void tail_offsets_kernel(const int32_t* __restrict__ ri, int64_t nnz,
   for (int64_t r = prev + 1; r <= cur; r++) ro[r] = (int32_t)i;
 }
 
+// HYB merged copy, counts: per row its ELL slots that hold a column (in any
+// position, as the ELL product tests every slot) plus its COO entries.
+__global__ void hyb_merge_count_kernel(const int32_t* __restrict__ eci, int64_t pitch,
+                                       int64_t loop, int64_t nrows,
+                                       const int32_t* __restrict__ tro,
+                                       int32_t* __restrict__ cnt) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  int32_t c = tro[r + 1] - tro[r];
+  for (int64_t n = 0; n < loop; n++) c += eci[r + pitch * n] >= 0 ? 1 : 0;
+  cnt[r] = c;
+}
+
+// HYB merged copy, fill: one thread per row writes its ELL entries in slot
+// order, then its COO entries in storage order (the order of the Serial HYB
+// product, an ELL pass then a COO pass), into [mro[r], mro[r + 1]).
+template <typename T>
+__global__ void hyb_merge_fill_kernel(const int32_t* __restrict__ eci, const T* __restrict__ ev,
+                                      int64_t pitch, int64_t loop, int64_t nrows,
+                                      const int32_t* __restrict__ tro,
+                                      const int32_t* __restrict__ cci, const T* __restrict__ cv,
+                                      const int32_t* __restrict__ mro, int32_t* __restrict__ mci,
+                                      T* __restrict__ mv) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  int64_t o = mro[r];
+  for (int64_t n = 0; n < loop; n++) {
+    const int32_t c = eci[r + pitch * n];
+    if (c >= 0) {
+      mci[o] = c;
+      mv[o] = ev[r + pitch * n];
+      o++;
+    }
+  }
+  for (int32_t j = tro[r]; j < tro[r + 1]; j++) {
+    mci[o] = cci[j];
+    mv[o] = cv[j];
+    o++;
+  }
+}
+
 __global__ void rows_unsorted_kernel(const int32_t* __restrict__ ri, int64_t nnz,
                                      int64_t nrows, int32_t* __restrict__ flag) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -903,6 +959,43 @@ __global__ void ell_tail_check_kernel(const int32_t* __restrict__ ci,
 }
 
 // --------------------------------------------------------------- host side
+// Lanes per row for the reduction of one short block (rows of lengths
+// ro[r + 1] - ro[r], r < nr): the power of two 1..64 with the least modelled
+// time of the block's slowest wave.  Groups of L lanes take rows g,
+// g + 256 / L, ...; each row iteration of a wave costs the longest trip
+// count ceil(len / L) among its groups' rows (an LDS read and an add per
+// trip), 1.5 per shuffle step (log2 L) and 3 of row overhead (offsets,
+// store).  Regular short rows get few lanes (S7, 7 per row: 1), blocks with
+// a few long rows among short ones more, so that one row does not hold a
+// whole wave.
+uint8_t block_lanes(int nr, const int32_t* ro) {
+  constexpr int kWaves = kRowBlockThreads / kWave;
+  int best_l = 1;
+  double best = 0;
+  for (int L = 1, lg = 0; L <= kWave; L <<= 1, lg++) {
+    const int groups = kRowBlockThreads / L, per_wave = kWave / L;
+    const int iters = (nr + groups - 1) / groups;
+    double worst = 0;
+    for (int w = 0; w < kWaves; w++) {
+      double c = 0;
+      for (int k = 0; k < iters; k++) {
+        int mx = -1;
+        for (int g = w * per_wave; g < (w + 1) * per_wave; g++) {
+          const int r = k * groups + g;
+          if (r < nr) mx = std::max(mx, (ro[r + 1] - ro[r] + L - 1) / L);
+        }
+        if (mx >= 0) c += mx + 1.5 * lg + 3;
+      }
+      worst = std::max(worst, c);
+    }
+    if (L == 1 || worst < best) {
+      best = worst;
+      best_l = L;
+    }
+  }
+  return (uint8_t)best_l;
+}
+
 int default_chunk(int vt) {
   return vt == MPH_F64 ? kDefaultChunkF64 : kDefaultChunkF32;
 }
@@ -1274,11 +1367,11 @@ int split_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
   const size_t vec_bytes = A->vt == MPH_F64 ? 8 : 4;
   const int mode = A->tuning.xtile;
   const bool xt_on = mode == 2 || (mode == 0 && (int64_t)A->ncols * (int64_t)vec_bytes >= kXTileMinXBytes);
-  const bool prefixed = A->format == MPH_HYB || A->format == MPH_HDC;
+  const bool prefixed = (A->format == MPH_HYB && !p.m_ro) || A->format == MPH_HDC;
   const bool split = prefixed && (xt_on || nrows >= kSplitMinRows);
   if ((!xt_on && !split) || nrows == 0 || A->ncols == 0) return MPH_OK;
-  const int32_t* d_ci = A->format == MPH_HYB ? A->coo_ci : A->csr_ci;
-  const void* d_val = A->format == MPH_HYB ? A->coo_v : A->csr_v;
+  const int32_t* d_ci = p.ci;
+  const void* d_val = p.val;
 
   std::vector<int32_t> ro(nrows + 1);
   MPH_HIP_CHECK(hipMemcpyAsync(ro.data(), d_ro, sizeof(int32_t) * (nrows + 1),
@@ -1362,19 +1455,25 @@ int split_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
       return MPH_ERR_RUNTIME;
     }
   }
+  std::vector<uint8_t> sl((sb.size() + 3) / 4 * 4, 0);  // read as dwords
+  for (size_t b = 0; b + 1 < sb.size(); b++)
+    if (sb[b].sub == -1) sl[b] = block_lanes(sb[b + 1].row - sb[b].row, L.data() + sb[b].row);
   RowBlock* d_sb = nullptr;
+  uint8_t* d_sl = nullptr;
   void* d_part = nullptr;
   int32_t* d_map = nullptr;
   int32_t* d_lro = nullptr;
   const int accs = (A->vt == MPH_F64 ? 8 : 4);
   const bool grow = (int64_t)sb.size() > (int64_t)p.nblocks + 1;
   bool fail = hipMalloc(&d_sb, sizeof(RowBlock) * sb.size()) != hipSuccess ||
+              hipMalloc(&d_sl, sl.size()) != hipSuccess ||
               (grow && hipMalloc(&d_part, (size_t)accs * 16 * sb.size()) != hipSuccess);
   if (!fail && split)
     fail = hipMalloc(&d_map, sizeof(int32_t) * (size_t)std::max<int64_t>(m, 1)) != hipSuccess ||
            hipMalloc(&d_lro, sizeof(int32_t) * (size_t)(m + 1)) != hipSuccess;
   if (fail) {
     dfree(d_sb);
+    dfree(d_sl);
     dfree(d_part);
     dfree(d_map);
     dfree(d_lro);
@@ -1383,7 +1482,8 @@ int split_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
     return MPH_ERR_OUT_OF_MEMORY;
   }
   fail = hipMemcpyAsync(d_sb, sb.data(), sizeof(RowBlock) * sb.size(), hipMemcpyHostToDevice, s) !=
-         hipSuccess;
+             hipSuccess ||
+         hipMemcpyAsync(d_sl, sl.data(), sl.size(), hipMemcpyHostToDevice, s) != hipSuccess;
   if (!fail && split && m > 0)
     fail = hipMemcpyAsync(d_map, map.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, s) !=
            hipSuccess;
@@ -1392,6 +1492,7 @@ int split_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
            hipSuccess;
   if (fail || hipStreamSynchronize(s) != hipSuccess) {
     dfree(d_sb);
+    dfree(d_sl);
     dfree(d_part);
     dfree(d_map);
     dfree(d_lro);
@@ -1405,6 +1506,7 @@ int split_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
     p.partial = d_part;
   }
   p.sblocks = d_sb;
+  p.sblanes = d_sl;
   p.nsblocks = (int32_t)sb.size() - 1;
   p.nchunked = nchunked;
   p.split = split;
@@ -1415,22 +1517,132 @@ int split_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
 
 }  // namespace
 
-// Builds the row blocks of A's CSR-shaped part (CSR, HDC's CSR part or the
-// HYB tail viewed through offsets built here).
+namespace {
+
+// A HYB with a row-sorted COO part, from kSplitMinRows rows or with x-tiling
+// on: when its ELL slots are filled below kHybMergeMaxFill, the product runs
+// over a merged CSR copy (p.m_*) instead of an ELL pass that reads mostly
+// padding.  Counts first; the copy is made only when the fill is low.
+int hyb_merge(mph_matrix_impl* A, hipStream_t s) {
+  RowBlockPlan& p = A->rb;
+  const int64_t nrows = A->nrows, loop = A->ell_loop;
+  const size_t vec_bytes = A->vt == MPH_F64 ? 8 : 4;
+  const int mode = A->tuning.xtile;
+  const bool xt_on = mode == 2 || (mode == 0 && (int64_t)A->ncols * (int64_t)vec_bytes >= kXTileMinXBytes);
+  if (A->coo_nnz == 0 || nrows == 0 || loop == 0) return MPH_OK;
+  if (!xt_on && nrows < kSplitMinRows) return MPH_OK;
+  if (nrows * loop + A->coo_nnz >= (int64_t)INT32_MAX) return MPH_OK;  // int32 offsets
+  int32_t* cnt = nullptr;
+  int32_t* mro = nullptr;
+  int64_t* tmp = nullptr;
+  auto fail = [&](int st) {
+    dfree(cnt);
+    dfree(mro);
+    dfree(tmp);
+    return st;
+  };
+  if (hipMalloc(&cnt, sizeof(int32_t) * nrows) != hipSuccess ||
+      hipMalloc(&mro, sizeof(int32_t) * (nrows + 1)) != hipSuccess ||
+      hipMalloc(&tmp, sizeof(int64_t) * scan_tmp_elems(nrows)) != hipSuccess) {
+    set_error("hipMalloc failed in the HYB merge analysis");
+    return fail(MPH_ERR_OUT_OF_MEMORY);
+  }
+  const unsigned g = (unsigned)ceil_div(nrows, 256);
+  hipLaunchKernelGGL(hyb_merge_count_kernel, dim3(g), dim3(256), 0, s, A->ell_ci, A->ell_pitch,
+                     loop, nrows, p.tail_ro, cnt);
+  if (hipGetLastError() != hipSuccess) {
+    set_error("HYB merge count launch failed");
+    return fail(MPH_ERR_HIP);
+  }
+  int st = exclusive_scan_i32(cnt, mro, nrows, tmp, s);
+  if (st != MPH_OK) return fail(st);
+  int32_t total = 0;
+  if (hipMemcpyAsync(&total, mro + nrows, sizeof(int32_t), hipMemcpyDeviceToHost, s) !=
+          hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    set_error("HYB merge analysis: copy failed");
+    return fail(MPH_ERR_HIP);
+  }
+  const int64_t ell_valid = (int64_t)total - A->coo_nnz;
+  if (ell_valid < 0 || ell_valid > nrows * loop) {
+    set_error("HYB merge analysis check failed: counts");
+    return fail(MPH_ERR_RUNTIME);
+  }
+  if ((double)ell_valid >= kHybMergeMaxFill * (double)nrows * (double)loop) return fail(MPH_OK);
+  const size_t vs = value_size(A->vt);
+  if (hipMalloc(&p.m_ci, sizeof(int32_t) * std::max<int64_t>(total, 1)) != hipSuccess ||
+      hipMalloc(&p.m_val, vs * std::max<int64_t>(total, 1)) != hipSuccess) {
+    dfree(p.m_ci);
+    dfree(p.m_val);
+    p.m_ci = nullptr;
+    p.m_val = nullptr;
+    set_error("hipMalloc failed for the HYB merged copy");
+    return fail(MPH_ERR_OUT_OF_MEMORY);
+  }
+  switch (vs) {
+    case 8:
+      hipLaunchKernelGGL(hyb_merge_fill_kernel<uint64_t>, dim3(g), dim3(256), 0, s, A->ell_ci,
+                         static_cast<const uint64_t*>(A->ell_v), A->ell_pitch, loop, nrows,
+                         p.tail_ro, A->coo_ci, static_cast<const uint64_t*>(A->coo_v), mro, p.m_ci,
+                         static_cast<uint64_t*>(p.m_val));
+      break;
+    case 4:
+      hipLaunchKernelGGL(hyb_merge_fill_kernel<uint32_t>, dim3(g), dim3(256), 0, s, A->ell_ci,
+                         static_cast<const uint32_t*>(A->ell_v), A->ell_pitch, loop, nrows,
+                         p.tail_ro, A->coo_ci, static_cast<const uint32_t*>(A->coo_v), mro, p.m_ci,
+                         static_cast<uint32_t*>(p.m_val));
+      break;
+    default:
+      hipLaunchKernelGGL(hyb_merge_fill_kernel<uint16_t>, dim3(g), dim3(256), 0, s, A->ell_ci,
+                         static_cast<const uint16_t*>(A->ell_v), A->ell_pitch, loop, nrows,
+                         p.tail_ro, A->coo_ci, static_cast<const uint16_t*>(A->coo_v), mro, p.m_ci,
+                         static_cast<uint16_t*>(p.m_val));
+  }
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+    set_error("HYB merge fill failed");
+    return fail(MPH_ERR_HIP);
+  }
+  dfree(cnt);
+  dfree(tmp);
+  p.m_ro = mro;
+  return MPH_OK;
+}
+
+}  // namespace
+
+// Builds the row blocks of A's CSR-shaped part: CSR, HDC's CSR part, the
+// HYB tail or a row-sorted COO matrix viewed through offsets built here, or
+// a HYB's merged copy.
 int rowblock_analyse(mph_matrix_impl* A, hipStream_t s) {
   RowBlockPlan& p = A->rb;
   const int64_t nrows = A->nrows;
   const int32_t* ro = A->csr_ro;
-  if (A->format == MPH_HYB) {
-    int sorted = 1;
-    MPH_CHECK_STATUS(rows_sorted(A->coo_ri, A->coo_nnz, nrows, s, &sorted));
-    if (!sorted) return MPH_ERR_NOT_IMPLEMENTED;  // caller falls back
+  p.ro = A->csr_ro;
+  p.ci = A->csr_ci;
+  p.val = A->csr_v;
+  if (A->format == MPH_HYB || A->format == MPH_COO) {
+    // (a COO matrix comes here only once its rows are known to be sorted)
+    if (A->format == MPH_HYB) {
+      int sorted = 1;
+      MPH_CHECK_STATUS(rows_sorted(A->coo_ri, A->coo_nnz, nrows, s, &sorted));
+      if (!sorted) return MPH_ERR_NOT_IMPLEMENTED;  // caller falls back
+    }
     MPH_HIP_CHECK(hipMalloc(&p.tail_ro, sizeof(int32_t) * (nrows + 1)));
     hipLaunchKernelGGL(tail_offsets_kernel,
                        dim3((unsigned)ceil_div(A->coo_nnz + 1, 256)), dim3(256),
                        0, s, A->coo_ri, A->coo_nnz, nrows, p.tail_ro);
     MPH_HIP_CHECK(hipGetLastError());
-    ro = p.tail_ro;
+    ro = p.ro = p.tail_ro;
+    p.ci = A->coo_ci;
+    p.val = A->coo_v;
+    if (A->format == MPH_HYB) {
+      MPH_CHECK_STATUS(hyb_merge(A, s));
+      if (p.m_ro) {
+        ro = p.ro = p.m_ro;
+        p.ci = p.m_ci;
+        p.val = p.m_val;
+      }
+    }
   }
   p.chunk = A->tuning.rows_chunk > 0 ? A->tuning.rows_chunk : default_chunk(A->vt);
   if (nrows == 0) {
@@ -1463,7 +1675,18 @@ int rowblock_analyse(mph_matrix_impl* A, hipStream_t s) {
                          p.chunk, first, p.blocks);
       MPH_HIP_CHECK(hipGetLastError());
       MPH_HIP_CHECK(hipStreamSynchronize(s));
-      st = validate_rowblock_plan(ro, nrows, p, s);
+      std::vector<uint8_t> lanes;
+      st = validate_rowblock_plan(ro, nrows, p, s, &lanes);
+      if (st == MPH_OK && nb > 0) {
+        lanes.resize((lanes.size() + 3) / 4 * 4, 0);  // read as dwords
+        if (hipMalloc(&p.blanes, lanes.size()) != hipSuccess ||
+            hipMemcpyAsync(p.blanes, lanes.data(), lanes.size(), hipMemcpyHostToDevice, s) !=
+                hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess) {
+          set_error("row-block analysis: lane table upload failed");
+          st = MPH_ERR_HIP;
+        }
+      }
     }
   }
   dfree(cnt);
@@ -1481,15 +1704,9 @@ RBArgs<V> make_args(mph_matrix_impl* A, const void* x, void* y, int ld,
   RBArgs<V> a{};
   a.nrows = A->nrows;
   a.ncols = A->ncols;
-  if (A->format == MPH_HYB) {
-    a.ro = A->rb.tail_ro;
-    a.ci = A->coo_ci;
-    a.val = static_cast<const V*>(A->coo_v);
-  } else {
-    a.ro = A->csr_ro;
-    a.ci = A->csr_ci;
-    a.val = static_cast<const V*>(A->csr_v);
-  }
+  a.ro = A->rb.ro;
+  a.ci = A->rb.ci;
+  a.val = static_cast<const V*>(A->rb.val);
   a.blocks = A->rb.blocks;
   a.chunk = A->rb.chunk;
   a.lanes_fixed = A->tuning.lanes;
@@ -1507,6 +1724,7 @@ RBArgs<V> make_args(mph_matrix_impl* A, const void* x, void* y, int ld,
   a.init = init;
   a.partial = static_cast<typename vtraits<V>::acc*>(A->rb.partial);
   a.nblocks = A->rb.nblocks;
+  a.blanes = A->rb.blanes;
   return a;
 }
 
@@ -1523,6 +1741,7 @@ int launch_split_spmv(mph_matrix_impl* A, const void* x, void* y, int init, hipS
   RBArgs<V> sa = a;
   sa.blocks = p.sblocks;
   sa.nblocks = p.nsblocks;
+  sa.blanes = p.sblanes;
   if (p.split) {
     sa.ro = p.sro;
     sa.rmap = p.smap;
@@ -1624,7 +1843,10 @@ int launch_spmm(mph_matrix_impl* A, int k, const void* x, void* y, int init,
 template <typename V>
 int dispatch_pre(mph_matrix_impl* A, int k, const void* x, void* y, int init,
                  hipStream_t s) {
-  int pre = A->format == MPH_HDC ? PRE_DIA : A->format == MPH_HYB ? PRE_ELL : PRE_NONE;
+  // (a HYB over its merged copy, like COO and CSR, has no prefix part)
+  int pre = A->format == MPH_HDC                    ? PRE_DIA
+            : A->format == MPH_HYB && !A->rb.m_ro ? PRE_ELL
+                                                    : PRE_NONE;
   // an HDC without diagonals / a HYB without ELL slots is its CSR-shaped part
   if ((pre == PRE_DIA && A->ndiag == 0) || (pre == PRE_ELL && A->ell_loop == 0)) pre = PRE_NONE;
   if (k == 1) {

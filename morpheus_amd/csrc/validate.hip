@@ -88,7 +88,7 @@ int check_row_offsets(const int32_t* ro, int64_t nrows, int64_t nnz,
 }
 
 int validate_rowblock_plan(const int32_t* d_ro, int64_t nrows, RowBlockPlan& p,
-                           hipStream_t s) {
+                           hipStream_t s, std::vector<uint8_t>* lanes) {
   const int64_t nb = p.nblocks;
   std::vector<int32_t> ro(nrows + 1);
   std::vector<RowBlock> bl(nb + 1);
@@ -125,6 +125,11 @@ int validate_rowblock_plan(const int32_t* d_ro, int64_t nrows, RowBlockPlan& p,
       if (k.sub == nsub - 1 && bl[b + 1].row != k.row + 1 && bl[b + 1].row <= k.row)
         return fail("chunk successor");
     }
+  }
+  if (lanes) {
+    lanes->assign(nb, 0);
+    for (int64_t b = 0; b < nb; b++)
+      if (bl[b].sub < 0) (*lanes)[b] = block_lanes(bl[b + 1].row - bl[b].row, ro.data() + bl[b].row);
   }
   return MPH_OK;
 }

@@ -263,6 +263,62 @@ def test_split_two_pass_deterministic(gpu, tall):
     assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
 
 
+def holey_hyb(n, fill, seed):
+    """A HYB built by hand: K = 4 ELL slots per row, each filled with
+    probability `fill` in any position (holes of -1 between entries, some
+    explicit zeros), and a row-sorted COO tail on every tenth row plus one
+    row of 3000 entries."""
+    from morpheus_amd.formats import Ell, Hyb
+    rng = np.random.default_rng(seed)
+    K, width, pitch = 4, 32, n
+    ci = np.full(pitch * width, -1, np.int32)
+    v = np.zeros(pitch * width)
+    for k in range(K):
+        on = rng.random(n) < fill
+        ci[k * pitch: k * pitch + n] = np.where(on, rng.integers(0, n, n), -1)
+        v[k * pitch: k * pitch + n] = np.where(on, rng.standard_normal(n) * (rng.random(n) > 0.05), 0.0)
+    ell = Ell(n, n, int((ci >= 0).sum()), K, width, ci, v, pitch)
+    rows = [np.repeat(r, rng.integers(1, 4)) for r in range(0, n, 10)]
+    rows.insert(len(rows) // 2, np.repeat(n // 2 + 1, 3000))
+    ri = np.sort(np.concatenate(rows), kind="stable")
+    cc = rng.integers(0, n, ri.size)
+    coo = Coo(n, n, ri, cc, rng.standard_normal(ri.size))
+    return Hyb(ell, coo)
+
+
+@pytest.mark.parametrize("fill", [0.2, 0.8])
+@pytest.mark.parametrize("xtile", [0, 2])
+@pytest.mark.parametrize("init", [True, False])
+def test_hyb_merged_and_split_plans(gpu, fill, xtile, init):
+    """70k rows: a HYB whose ELL slots are a fifth filled runs over the merged
+    CSR copy, one four-fifths filled runs the ELL pass then the tail; both
+    against the Serial HYB product (ELL, then COO)."""
+    H = holey_hyb(70000, fill, 3)
+    x = gen.random_vector(H.ncols, 17)
+    y0 = gen.random_vector(H.nrows, 18)
+    y = gpu_spmv(H, x, y0=y0, init=init, tuning=dict(xtile=xtile))
+    ref = oracle.spmv(H, x, y0, init)
+    from morpheus_amd.formats import Ell, Hyb
+    e, c = H.ell, H.coo
+    absH = Hyb(Ell(e.nrows, e.ncols, e.nnz, e.entries_per_row, e.width, e.column_indices,
+                   np.abs(e.values), e.pitch),
+               Coo(c.nrows, c.ncols, c.row_indices, c.column_indices, np.abs(c.values)))
+    absrow = oracle.spmv(absH, np.abs(x))
+    nnz_row = (e.column_indices.reshape(e.width, e.pitch)[:, :e.nrows] >= 0).sum(axis=0) + \
+        np.bincount(c.row_indices, minlength=H.nrows)
+    bound = fp64_bound(nnz_row, absrow, None if init else y0)
+    assert np.all(np.abs(y - ref) <= bound)
+
+
+def test_hyb_merged_plan_spmm(gpu):
+    H = holey_hyb(70000, 0.2, 4)
+    X = np.random.default_rng(5).uniform(-1, 1, (H.ncols, 3))
+    Y = gpu_spmv(H, X, k=3)
+    for j in range(3):
+        ref = oracle.spmv(H, X[:, j])
+        assert np.allclose(Y[:, j], ref, rtol=1e-12, atol=1e-12)
+
+
 # ------------------------------------------------------------ hipGraph capture
 @pytest.mark.parametrize("name", ALL)
 @pytest.mark.parametrize("xtile", [1, 2])
