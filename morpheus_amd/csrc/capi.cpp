@@ -41,8 +41,8 @@ void release_plans(mph_matrix_impl* A) {
   dfree(p.m_ci);
   dfree(p.m_val);
   if (p.sblocks) dfree(p.sblocks);
-  dfree(p.blanes);
-  dfree(p.sblanes);
+  dfree(p.krec);
+  dfree(p.skrec);
   if (p.smap) dfree(p.smap);
   if (p.sro) dfree(p.sro);
   xtile_release(p.xt);

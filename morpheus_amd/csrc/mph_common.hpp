@@ -196,6 +196,11 @@ inline int invalid(const std::string& msg) {
   return MPH_ERR_INVALID_INPUT;
 }
 
+inline int runtime_error(const std::string& msg) {
+  set_error(msg);
+  return MPH_ERR_RUNTIME;
+}
+
 // device free whose status is deliberately ignored (release paths)
 inline void dfree(void* p) {
   if (p) (void)hipFree(p);

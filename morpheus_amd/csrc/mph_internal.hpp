@@ -83,10 +83,11 @@ struct RowBlockPlan {
   int32_t* m_ci = nullptr;
   void* m_val = nullptr;
   int32_t nlong = 0;             // rows split into chunks
-  // lanes per row of each short block's reduction (blocks / sblocks order),
-  // picked by the analysis; 0 for chunk and tiled records
-  uint8_t* blanes = nullptr;
-  uint8_t* sblanes = nullptr;
+  // the short-block kernel's records, blocks / sblocks order, sentinel
+  // included: {row, sub, first item, lanes per row of the block's reduction
+  // (picked by the analysis; 0 for chunk and tiled records)}
+  int32_t* krec = nullptr;
+  int32_t* skrec = nullptr;
   // x-tiled long rows (SpMV); xt.rows == 0: long rows use position chunks
   XTilePlan xt;
   // split SpMV plan (x-tiled rows, and / or HYB / HDC in two passes):
@@ -185,7 +186,7 @@ int check_index_range(const int32_t* idx, int64_t n, int64_t hi, bool allow_m1,
 int check_row_offsets(const int32_t* ro, int64_t nrows, int64_t nnz,
                       hipStream_t s, const char* what);
 int validate_rowblock_plan(const int32_t* d_ro, int64_t nrows, RowBlockPlan& p,
-                           hipStream_t s, std::vector<uint8_t>* lanes = nullptr);
+                           hipStream_t s, std::vector<int32_t>* krec = nullptr);
 // lanes per row for a short block of nr rows of lengths len(0..nr-1)
 uint8_t block_lanes(int nr, const int32_t* ro_begin);
 

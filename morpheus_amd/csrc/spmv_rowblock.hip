@@ -66,8 +66,10 @@ struct RBArgs {
   // split plan: block rows are list rows, list row i is matrix row rmap[i]
   // (nullptr: the identity); used with PRE_NONE only
   const int32_t* __restrict__ rmap;
-  // lanes per row of each short block (nullptr: from the block's mean)
-  const uint8_t* __restrict__ blanes;
+  // {row, sub, first item, lanes} per record of `blocks`, sentinel
+  // included: built by the analysis for every list the short-block kernel
+  // runs over
+  const int32_t* __restrict__ krec;
 };
 
 template <typename V>
@@ -270,18 +272,19 @@ __global__ __launch_bounds__(kRowBlockThreads) void rowblock_spmv_kernel(
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ Acc s_red[2 * kRowBlockThreads / kWave];
   const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
-  // the record and the next one's row (the sentinel bounds the last block)
-  // as one scalar load
-  int32_t rec[3];
-  __builtin_memcpy(rec, &a.blocks[b], sizeof(rec));
+  // the block's record and the next one (the sentinel bounds the last
+  // block) in one scalar load: rows, first items and the lane count, so
+  // that the column / value loads need not wait for the row offsets
+  int32_t rec[8];
+  __builtin_memcpy(rec, a.krec + 4 * (int64_t)b, sizeof(rec));
   const RowBlock blk{rec[0], rec[1]};
-  const int32_t next_row = rec[2];
-  // the block's lane count is read with its record, as a dword of the
-  // (4-byte padded) table, so that it is a scalar load that arrives with the
-  // record instead of a vector load waited on before the product phase
-  const int lanes_b =
-      a.blanes ? (int)((reinterpret_cast<const uint32_t*>(a.blanes)[b >> 2] >> (8 * (b & 3))) & 0xffu)
-               : 0;
+  const int32_t next_row = rec[4];
+  const int lanes_b = rec[3];
+  const int32_t item[2] = {rec[2], rec[6]};
+  // an empty asm statement that takes the fields as operands, so that the
+  // compiler loads them here, together, instead of sinking each load to its
+  // first use behind the branches below (a round trip each)
+  asm volatile("" : : "s"(next_row), "s"(lanes_b), "s"(item[0]), "s"(item[1]));
   if (blk.sub < -1) return;  // a long row handled by the x-tiled pass
   if (blk.sub >= 0) {
     long_chunk_spmv<V, COMP>(a, b, blk, s_red);
@@ -294,13 +297,28 @@ __global__ __launch_bounds__(kRowBlockThreads) void rowblock_spmv_kernel(
   const int64_t row0 = blk.row;
   const int64_t row1 = next_row;
   const int nr = (int)(row1 - row0);
-  const int64_t p0 = a.ro[row0];
-  const int n = (int)(a.ro[row1] - p0);
+  const int64_t p0 = item[0];
+  const int n = item[1] - item[0];
 
-  for (int i = threadIdx.x; i <= nr; i += kRowBlockThreads)
-    s_ro[i] = (int32_t)(a.ro[row0 + i] - p0);
+  // the block's row offsets (at most kRowBlockMaxRows + 1) are loaded into
+  // registers first and stored to LDS after the products, so that offsets,
+  // columns and values are in flight together; every lane loads (indices
+  // past the block clamped to its last offset), since a load under a branch
+  // is waited on where the branch joins
+  constexpr int kRoLoads = (kRowBlockMaxRows + kRowBlockThreads) / kRowBlockThreads;
+  int32_t rv[kRoLoads];
+#pragma unroll
+  for (int k = 0; k < kRoLoads; k++) {
+    const int i = threadIdx.x + k * kRowBlockThreads;
+    rv[k] = a.ro[row0 + min(i, nr)];
+  }
   batched_products<V>(a.ci + p0, a.val + p0, a.x, 0, n,
                       [&](int64_t i, Acc p) { s_prod[i] = p; });
+#pragma unroll
+  for (int k = 0; k < kRoLoads; k++) {
+    const int i = threadIdx.x + k * kRowBlockThreads;
+    if (i <= nr) s_ro[i] = (int32_t)(rv[k] - p0);
+  }
   if constexpr (PRE != PRE_NONE) {
     for (int i = threadIdx.x; i < nr; i += kRowBlockThreads) {
       Acc p[1] = {0};
@@ -310,9 +328,7 @@ __global__ __launch_bounds__(kRowBlockThreads) void rowblock_spmv_kernel(
   }
   __syncthreads();
 
-  const int L = a.lanes_fixed > 0 ? a.lanes_fixed
-                : lanes_b > 0    ? lanes_b
-                                 : choose_lanes(n, nr, 0);
+  const int L = a.lanes_fixed > 0 ? a.lanes_fixed : lanes_b;
   const int groups = kRowBlockThreads / L;
   const int g = threadIdx.x / L, l = threadIdx.x % L;
   for (int r = g; r < nr; r += groups) {
@@ -1455,25 +1471,31 @@ int split_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
       return MPH_ERR_RUNTIME;
     }
   }
-  std::vector<uint8_t> sl((sb.size() + 3) / 4 * 4, 0);  // read as dwords
-  for (size_t b = 0; b + 1 < sb.size(); b++)
-    if (sb[b].sub == -1) sl[b] = block_lanes(sb[b + 1].row - sb[b].row, L.data() + sb[b].row);
+  std::vector<int32_t> kr(4 * sb.size());
+  for (size_t b = 0; b < sb.size(); b++) {
+    kr[4 * b] = sb[b].row;
+    kr[4 * b + 1] = sb[b].sub;
+    kr[4 * b + 2] = L[sb[b].row];
+    kr[4 * b + 3] = b + 1 < sb.size() && sb[b].sub == -1
+                        ? block_lanes(sb[b + 1].row - sb[b].row, L.data() + sb[b].row)
+                        : 0;
+  }
   RowBlock* d_sb = nullptr;
-  uint8_t* d_sl = nullptr;
+  int32_t* d_kr = nullptr;
   void* d_part = nullptr;
   int32_t* d_map = nullptr;
   int32_t* d_lro = nullptr;
   const int accs = (A->vt == MPH_F64 ? 8 : 4);
   const bool grow = (int64_t)sb.size() > (int64_t)p.nblocks + 1;
   bool fail = hipMalloc(&d_sb, sizeof(RowBlock) * sb.size()) != hipSuccess ||
-              hipMalloc(&d_sl, sl.size()) != hipSuccess ||
+              hipMalloc(&d_kr, sizeof(int32_t) * kr.size()) != hipSuccess ||
               (grow && hipMalloc(&d_part, (size_t)accs * 16 * sb.size()) != hipSuccess);
   if (!fail && split)
     fail = hipMalloc(&d_map, sizeof(int32_t) * (size_t)std::max<int64_t>(m, 1)) != hipSuccess ||
            hipMalloc(&d_lro, sizeof(int32_t) * (size_t)(m + 1)) != hipSuccess;
   if (fail) {
     dfree(d_sb);
-    dfree(d_sl);
+    dfree(d_kr);
     dfree(d_part);
     dfree(d_map);
     dfree(d_lro);
@@ -1483,7 +1505,8 @@ int split_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
   }
   fail = hipMemcpyAsync(d_sb, sb.data(), sizeof(RowBlock) * sb.size(), hipMemcpyHostToDevice, s) !=
              hipSuccess ||
-         hipMemcpyAsync(d_sl, sl.data(), sl.size(), hipMemcpyHostToDevice, s) != hipSuccess;
+         hipMemcpyAsync(d_kr, kr.data(), sizeof(int32_t) * kr.size(), hipMemcpyHostToDevice, s) !=
+             hipSuccess;
   if (!fail && split && m > 0)
     fail = hipMemcpyAsync(d_map, map.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, s) !=
            hipSuccess;
@@ -1492,7 +1515,7 @@ int split_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
            hipSuccess;
   if (fail || hipStreamSynchronize(s) != hipSuccess) {
     dfree(d_sb);
-    dfree(d_sl);
+    dfree(d_kr);
     dfree(d_part);
     dfree(d_map);
     dfree(d_lro);
@@ -1506,7 +1529,7 @@ int split_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
     p.partial = d_part;
   }
   p.sblocks = d_sb;
-  p.sblanes = d_sl;
+  p.skrec = d_kr;
   p.nsblocks = (int32_t)sb.size() - 1;
   p.nchunked = nchunked;
   p.split = split;
@@ -1675,15 +1698,14 @@ int rowblock_analyse(mph_matrix_impl* A, hipStream_t s) {
                          p.chunk, first, p.blocks);
       MPH_HIP_CHECK(hipGetLastError());
       MPH_HIP_CHECK(hipStreamSynchronize(s));
-      std::vector<uint8_t> lanes;
-      st = validate_rowblock_plan(ro, nrows, p, s, &lanes);
+      std::vector<int32_t> krec;
+      st = validate_rowblock_plan(ro, nrows, p, s, &krec);
       if (st == MPH_OK && nb > 0) {
-        lanes.resize((lanes.size() + 3) / 4 * 4, 0);  // read as dwords
-        if (hipMalloc(&p.blanes, lanes.size()) != hipSuccess ||
-            hipMemcpyAsync(p.blanes, lanes.data(), lanes.size(), hipMemcpyHostToDevice, s) !=
-                hipSuccess ||
+        if (hipMalloc(&p.krec, sizeof(int32_t) * krec.size()) != hipSuccess ||
+            hipMemcpyAsync(p.krec, krec.data(), sizeof(int32_t) * krec.size(),
+                           hipMemcpyHostToDevice, s) != hipSuccess ||
             hipStreamSynchronize(s) != hipSuccess) {
-          set_error("row-block analysis: lane table upload failed");
+          set_error("row-block analysis: kernel record upload failed");
           st = MPH_ERR_HIP;
         }
       }
@@ -1724,7 +1746,7 @@ RBArgs<V> make_args(mph_matrix_impl* A, const void* x, void* y, int ld,
   a.init = init;
   a.partial = static_cast<typename vtraits<V>::acc*>(A->rb.partial);
   a.nblocks = A->rb.nblocks;
-  a.blanes = A->rb.blanes;
+  a.krec = A->rb.krec;
   return a;
 }
 
@@ -1741,14 +1763,16 @@ int launch_split_spmv(mph_matrix_impl* A, const void* x, void* y, int init, hipS
   RBArgs<V> sa = a;
   sa.blocks = p.sblocks;
   sa.nblocks = p.nsblocks;
-  sa.blanes = p.sblanes;
+  sa.krec = p.skrec;
   if (p.split) {
     sa.ro = p.sro;
     sa.rmap = p.smap;
   }
-  if (p.nsblocks > 0)
+  if (p.nsblocks > 0) {
+    if (!sa.krec) return runtime_error("split SpMV plan without kernel records");
     hipLaunchKernelGGL((rowblock_spmv_kernel<V, PRE, comp>), dim3((unsigned)p.nsblocks),
                        dim3(kRowBlockThreads), lds, s, sa);
+  }
   if (p.nchunked > 0)
     hipLaunchKernelGGL((rowblock_combine_kernel<V, PRE, comp>),
                        dim3((unsigned)ceil_div(p.nsblocks, 256)), dim3(256), 0, s, sa);
@@ -1784,6 +1808,7 @@ int launch_spmv(mph_matrix_impl* A, const void* x, void* y, int init,
                                             : dia_multiply(A, 1, x, y, init, s));
     return launch_split_spmv<V, PRE_NONE>(A, x, y, 0, s);
   }
+  if (!a.krec) return runtime_error("row-block plan without kernel records");
   hipLaunchKernelGGL((rowblock_spmv_kernel<V, PRE, comp>),
                      dim3((unsigned)A->rb.nblocks), dim3(kRowBlockThreads), lds,
                      s, a);

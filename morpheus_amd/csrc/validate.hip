@@ -88,7 +88,7 @@ int check_row_offsets(const int32_t* ro, int64_t nrows, int64_t nnz,
 }
 
 int validate_rowblock_plan(const int32_t* d_ro, int64_t nrows, RowBlockPlan& p,
-                           hipStream_t s, std::vector<uint8_t>* lanes) {
+                           hipStream_t s, std::vector<int32_t>* krec) {
   const int64_t nb = p.nblocks;
   std::vector<int32_t> ro(nrows + 1);
   std::vector<RowBlock> bl(nb + 1);
@@ -126,10 +126,16 @@ int validate_rowblock_plan(const int32_t* d_ro, int64_t nrows, RowBlockPlan& p,
         return fail("chunk successor");
     }
   }
-  if (lanes) {
-    lanes->assign(nb, 0);
-    for (int64_t b = 0; b < nb; b++)
-      if (bl[b].sub < 0) (*lanes)[b] = block_lanes(bl[b + 1].row - bl[b].row, ro.data() + bl[b].row);
+  // the short-block kernel's records: {row, sub, first item, lanes}
+  if (krec) {
+    krec->assign(4 * (nb + 1), 0);
+    for (int64_t b = 0; b <= nb; b++) {
+      (*krec)[4 * b] = bl[b].row;
+      (*krec)[4 * b + 1] = bl[b].sub;
+      (*krec)[4 * b + 2] = ro[bl[b].row];
+      if (b < nb && bl[b].sub < 0)
+        (*krec)[4 * b + 3] = block_lanes(bl[b + 1].row - bl[b].row, ro.data() + bl[b].row);
+    }
   }
   return MPH_OK;
 }
