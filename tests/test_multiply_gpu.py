@@ -310,6 +310,24 @@ def test_hyb_merged_and_split_plans(gpu, fill, xtile, init):
     assert np.all(np.abs(y - ref) <= bound)
 
 
+@pytest.mark.parametrize("dtype,tol", [("f32", 1e-5), ("bf16", 1e-2)])
+@pytest.mark.parametrize("fill", [0.2, 0.8])
+def test_hyb_plans_reduced_precision(gpu, dtype, tol, fill):
+    """The merged and split HYB plans in FP32 / BF16 storage against the FP64
+    product of the original matrix, max_i |y_i - y_ref_i| / sum_j |a_ij x_j|."""
+    from morpheus_amd.formats import Ell, Hyb
+    H = holey_hyb(70000, fill, 6)
+    x = gen.random_vector(H.ncols, 19)
+    y = gpu_spmv(H, x, dtype)
+    ref = oracle.spmv(H, x)
+    e, c = H.ell, H.coo
+    absH = Hyb(Ell(e.nrows, e.ncols, e.nnz, e.entries_per_row, e.width, e.column_indices,
+                   np.abs(e.values), e.pitch),
+               Coo(c.nrows, c.ncols, c.row_indices, c.column_indices, np.abs(c.values)))
+    err = max_rel_err(y, ref, oracle.spmv(absH, np.abs(x)))
+    assert err <= tol, err
+
+
 def test_hyb_merged_plan_spmm(gpu):
     H = holey_hyb(70000, 0.2, 4)
     X = np.random.default_rng(5).uniform(-1, 1, (H.ncols, 3))
