@@ -7,7 +7,7 @@ its benchmark matrix, matrices and vectors resident in HBM:
     capped at 20k, uniform random columns, N(0,1) values;
   * DIA, HDC, ELL on S7: 3D 7-point stencil on a 128^3 grid.
 ELL on Z15 is inadmissible (the reference converter throws
-FormatConversionException; this build reproduces that), so ELL runs on S7.
+FormatConversionException; the library reproduces that), so ELL runs on S7.
 The formats are produced on the GPU by the library's own Convert from CSR.
 
 value = total FLOPs of the step (2 * nnz per SpMV) / step time.  Every SpMV
@@ -259,7 +259,7 @@ def run_cg(mats, iters):
 
 def run_cpu(mats, order, budget):
     import numpy as np
-    import oracle  # the checker's OpenMP restatement: cpu_baseline leg only
+    import oracle  # the oracle's OpenMP restatement: cpu_baseline leg only
 
     res = {}
     for name in order:
@@ -413,7 +413,7 @@ def run_sweep(z15, s7, steps):
     |a_ij x_j|)."""
     import numpy as np
     import torch
-    import oracle  # the checker: FP64 reference products and row scales
+    import oracle  # the oracle: FP64 reference products and row scales
     from morpheus_amd import capi, device
     from morpheus_amd import generators as gen
     from morpheus_amd.formats import FORMAT_NAMES

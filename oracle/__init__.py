@@ -3,8 +3,8 @@
 TEST INFRASTRUCTURE ONLY.  A ctypes wrapper over ``libmph_oracle.so`` (the
 plain-C restatement of the reference Serial backend in ``mph_oracle.c`` and
 of its OpenMP multiply in ``mph_omp_baseline.c``).  Only tests/, the smoke
-check and the benchmark's cpu_baseline leg may import it, as the checker;
-the product package never does.
+check and the benchmark's cpu_baseline leg may import it, and only to check
+or to time the CPU baseline; the product package never does.
 
 The reference itself (C++ on Kokkos 3.5.00, googletest) cannot be built in
 this image -- Kokkos is not vendored and cannot be fetched -- so there is no

@@ -1,8 +1,8 @@
 /*
  * mph_oracle.c -- CPU oracle (Serial restatement) for the Morpheus SpMV and
  * Convert hot path.  TEST INFRASTRUCTURE ONLY: loaded by tests/, the smoke
- * check and the benchmark's cpu_baseline leg as the checker; never by the
- * product path.
+ * check and the benchmark's cpu_baseline leg, to check results or time the
+ * CPU baseline; never by the product path.
  *
  * Each function cites the reference file:line it restates (paths relative to
  * the morpheus repository root).  Build with -ffp-contract=off so that the

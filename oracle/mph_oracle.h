@@ -4,8 +4,9 @@
  * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
  * Serial backend (the reference's own "textbook" loops) plus a restatement of
  * its OpenMP multiply, used as the timed CPU baseline.  Only tests/, the
- * smoke check and the benchmark's cpu_baseline leg may load it, and only as
- * the checker.  The product path (morpheus_amd) never calls into it.
+ * smoke check and the benchmark's cpu_baseline leg may load it, and only to
+ * check results or time the CPU baseline.  The product path (morpheus_amd)
+ * never calls into it.
  *
  * Storage conventions (identical to what the HIP kernels consume):
  *   - indices are int32, sizes int64, values double;
@@ -19,10 +20,9 @@
  * Return codes match the product C-ABI (include/morpheus_amd.h):
  *   0 OK, 2 INVALID_INPUT, 3 FORMAT_CONVERSION (fill tolerance exceeded).
  *
- * Parity status per function is listed in the oracle section of DESIGN.md's
- * table; every function below is pinned by the tests in
- * tests/test_oracle_golden.py (golden 10x10 fixtures, closed forms, brute
- * force or scipy cross-checks).
+ * Every function below is pinned by the tests in tests/test_oracle_golden.py
+ * and the other *_cpu.py tests (the reference's golden 10x10 fixtures and
+ * MatrixMarket files, closed forms, brute force or scipy cross-checks).
  */
 #ifndef MPH_ORACLE_H
 #define MPH_ORACLE_H

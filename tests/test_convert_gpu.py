@@ -2,7 +2,7 @@
 converters) and the reference's hand-written fixture in all six formats.
 
 The reference HIP backend throws NotImplementedException for every pair
-(core/tests/HIP/Test_Convert.hpp:164-165,178-179); this build converts on the
+(core/tests/HIP/Test_Convert.hpp:164-165,178-179); the library converts on the
 GPU instead, so parity is against the Serial output
 (core/tests/Serial/Test_Convert.hpp, core/tests/Serial/Test_Convert_Dynamic.hpp)."""
 import numpy as np
@@ -95,7 +95,7 @@ def test_csr_to_format_and_back(gpu, mats, name, fmt, dtype):
 
 
 def test_p2_roundtrip_csr_coo_dia_csr(gpu, mats):
-    """CSR -> COO -> DIA -> CSR bit-exact, on the GPU."""
+    """P2 (2D 5-point Poisson 256^2): CSR -> COO -> DIA -> CSR bit-exact, on the GPU."""
     import torch
     device = dev()
     A = mats["p2"]

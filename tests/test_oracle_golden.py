@@ -233,7 +233,7 @@ def test_ell_hyb_tolerance_exceptions():
 
 
 def test_p2_roundtrip_csr_coo_dia_csr_bitexact():
-    """CSR -> COO -> DIA -> CSR is the identity on P2."""
+    """CSR -> COO -> DIA -> CSR is the identity on P2 (2D 5-point Poisson 256^2)."""
     A = gen.P2()
     coo = oracle.to_coo(A)
     dia = oracle.from_coo(coo, DIA_FORMAT)
