@@ -136,7 +136,10 @@ int mph_matrix_format(mph_matrix_t A);
  *                  0 = a row-sorted COO matrix runs the CSR row-block plan
  *                  over row offsets derived from its row indices (rows_chunk
  *                  and lanes apply), an unsorted one the atomic pass
- *   rows_per_thread : DIA / ELL rows per thread (1 or 2)
+ *   rows_per_thread : DIA / ELL rows per thread (1 or 2, strided by 256 rows);
+ *                  0 = one row in FP64, and in FP32 / BF16 SpMV two adjacent
+ *                  rows whose values (and ELL column indices) are loaded
+ *                  as pairs, when the pitch is even
  *   xtile        : x-tiled long rows of CSR / COO / HYB / HDC SpMV (1 = off,
  *                  2 = on; 0 = on when x exceeds 2 MiB).  Rows longer than
  *                  512 entries with sorted columns are summed per 32 KiB tile

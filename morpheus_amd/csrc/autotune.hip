@@ -95,7 +95,7 @@ int mph_matrix_autotune(mph_matrix_t A, int32_t k, const void* x, void* y, int32
       break;
     case MPH_DIA:
     case MPH_ELL:
-      for (int rpt : {1, 2}) cands.push_back(mph_tuning{0, 0, 0, rpt});
+      for (int rpt : {0, 1, 2}) cands.push_back(mph_tuning{0, 0, 0, rpt});
       break;
     default: {
       const int maxc = A->vt == MPH_F64 ? 2048 : 4096;

@@ -12,7 +12,8 @@
 // a separate launch and then read-modify-writes it per 256-diagonal chunk,
 // and skips the zeroing when ndiag == 0).  ELL visits entries_per_row slots,
 // not the 32-padded width, when the analysis has confirmed that the slots
-// beyond entries_per_row hold only the -1 padding.
+// beyond entries_per_row hold only the -1 padding.  FP32 / BF16 SpMV takes
+// two adjacent rows per thread (row-pair kernels below).
 #include "mph_internal.hpp"
 
 namespace mph {
@@ -144,6 +145,101 @@ __global__ __launch_bounds__(256) void ell_kernel(EllArgs<V> a) {
   }
 }
 
+// ----------------------------------------------------------- row pairs
+// FP32 / BF16 SpMV: a thread takes the adjacent rows 2t and 2t + 1, so that
+// one load moves the two rows' values of a diagonal / slot (8 B in FP32, 4 B
+// in BF16) and, in ELL, their two column indices (8 B).  With one row per
+// thread a 16-bit value stream moved 128 B per wave load, and BF16 ran
+// slower than FP32.  Used when the pitch is even (a pair never straddles
+// two diagonals / slots; an odd row count then leaves the pair's second
+// element inside the pitch) and the arrays are aligned to the pair.
+template <typename V>
+struct Pair;
+template <>
+struct Pair<float> {
+  using T = uint64_t;
+  __device__ __forceinline__ static void split(T w, float& lo, float& hi) {
+    lo = __uint_as_float((uint32_t)w);
+    hi = __uint_as_float((uint32_t)(w >> 32));
+  }
+};
+template <>
+struct Pair<bf16> {
+  using T = uint32_t;
+  __device__ __forceinline__ static void split(T w, float& lo, float& hi) {
+    lo = __uint_as_float(w << 16);
+    hi = __uint_as_float(w & 0xffff0000u);
+  }
+};
+
+template <typename V>
+__global__ __launch_bounds__(256) void dia_pair_kernel(DiaArgs<V> a) {
+  using P = Pair<V>;
+  const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t row = ((int64_t)b * 256 + threadIdx.x) * 2;
+  const typename P::T* pv = reinterpret_cast<const typename P::T*>(a.val);
+  float t0 = 0, t1 = 0;
+  for (int d0 = 0; d0 < a.ndiag; d0 += 8) {
+    float v0[8], v1[8];
+    int64_t c0[8], c1[8];
+#pragma unroll
+    for (int w = 0; w < 8; w++) {
+      const int d = d0 + w;
+      const bool in = d < a.ndiag && row < a.nrows;
+      const int64_t o = d < a.ndiag ? (int64_t)a.off[d] : 0;
+      const typename P::T pw = in ? ld_stream(pv + (a.pitch * d + row) / 2) : (typename P::T)0;
+      P::split(pw, v0[w], v1[w]);
+      const int64_t col0 = row + o, col1 = row + 1 + o;
+      c0[w] = in && col0 >= 0 && col0 < a.ncols ? col0 : -1;
+      c1[w] = in && row + 1 < a.nrows && col1 >= 0 && col1 < a.ncols ? col1 : -1;
+    }
+#pragma unroll
+    for (int w = 0; w < 8; w++) {
+      if (c0[w] >= 0) t0 += v0[w] * a.x[c0[w]];
+      if (c1[w] >= 0) t1 += v1[w] * a.x[c1[w]];
+    }
+  }
+  if (row < a.nrows) a.y[row] = a.init ? t0 : a.y[row] + t0;
+  if (row + 1 < a.nrows) a.y[row + 1] = a.init ? t1 : a.y[row + 1] + t1;
+}
+
+template <typename V>
+__global__ __launch_bounds__(256) void ell_pair_kernel(EllArgs<V> a) {
+  using P = Pair<V>;
+  const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t row = ((int64_t)b * 256 + threadIdx.x) * 2;
+  const typename P::T* pv = reinterpret_cast<const typename P::T*>(a.val);
+  const uint64_t* pc = reinterpret_cast<const uint64_t*>(a.ci);
+  float t0 = 0, t1 = 0;
+  const bool in = row < a.nrows;
+  for (int n0 = 0; n0 < a.loop; n0 += 8) {
+    int32_t c0[8], c1[8];
+#pragma unroll
+    for (int w = 0; w < 8; w++) {
+      const int n = n0 + w;
+      const uint64_t cw = in && n < a.loop ? ld_stream(pc + (a.pitch * n + row) / 2) : ~0ull;
+      c0[w] = (int32_t)(uint32_t)cw;
+      // the pair's second row may lie past nrows, inside the pitch
+      c1[w] = row + 1 < a.nrows ? (int32_t)(uint32_t)(cw >> 32) : -1;
+    }
+    float v0[8], v1[8];
+#pragma unroll
+    for (int w = 0; w < 8; w++) {
+      const int n = n0 + w;
+      const typename P::T pw =
+          (c0[w] >= 0 || c1[w] >= 0) ? ld_stream(pv + (a.pitch * n + row) / 2) : (typename P::T)0;
+      P::split(pw, v0[w], v1[w]);
+    }
+#pragma unroll
+    for (int w = 0; w < 8; w++) {
+      if (c0[w] >= 0) t0 += v0[w] * a.x[c0[w]];
+      if (c1[w] >= 0) t1 += v1[w] * a.x[c1[w]];
+    }
+  }
+  if (row < a.nrows) a.y[row] = a.init ? t0 : a.y[row] + t0;
+  if (row + 1 < a.nrows) a.y[row + 1] = a.init ? t1 : a.y[row + 1] + t1;
+}
+
 namespace {
 
 template <typename V, int KR, int RPT>
@@ -218,9 +314,59 @@ int launch_fmt(mph_matrix_impl* A, int k, const void* x, void* y, int init,
   return MPH_OK;
 }
 
+// the row-pair kernels: FP32 / BF16 SpMV with the default launch shape, an
+// even pitch and arrays aligned to the pair loads
+template <typename V, bool DIA>
+int launch_pair(mph_matrix_impl* A, const void* x, void* y, int init, hipStream_t s) {
+  const int64_t nb = ceil_div(A->nrows, 512);
+  if (nb == 0) return MPH_OK;
+  if constexpr (DIA) {
+    DiaArgs<V> a{};
+    a.nrows = A->nrows;
+    a.ncols = A->ncols;
+    a.ndiag = (int32_t)A->ndiag;
+    a.pitch = A->dia_pitch;
+    a.off = A->dia_off;
+    a.val = static_cast<const V*>(A->dia_v);
+    a.x = static_cast<const float*>(x);
+    a.y = static_cast<float*>(y);
+    a.ld = 1;
+    a.init = init;
+    hipLaunchKernelGGL(dia_pair_kernel<V>, dim3((unsigned)nb), dim3(256), 0, s, a);
+  } else {
+    EllArgs<V> a{};
+    a.nrows = A->nrows;
+    a.loop = (int32_t)A->ell_loop;
+    a.pitch = A->ell_pitch;
+    a.ci = A->ell_ci;
+    a.val = static_cast<const V*>(A->ell_v);
+    a.x = static_cast<const float*>(x);
+    a.y = static_cast<float*>(y);
+    a.ld = 1;
+    a.init = init;
+    hipLaunchKernelGGL(ell_pair_kernel<V>, dim3((unsigned)nb), dim3(256), 0, s, a);
+  }
+  MPH_HIP_CHECK(hipGetLastError());
+  return MPH_OK;
+}
+
+template <bool DIA>
+bool pair_ok(const mph_matrix_impl* A, int k) {
+  if (k != 1 || A->tuning.rows_per_thread != 0) return false;
+  if (A->vt != MPH_F32 && A->vt != MPH_BF16) return false;
+  const int64_t pitch = DIA ? A->dia_pitch : A->ell_pitch;
+  const uintptr_t v = reinterpret_cast<uintptr_t>(DIA ? A->dia_v : A->ell_v);
+  const size_t pair_bytes = 2 * value_size(A->vt);
+  if (pitch % 2 != 0 || v % pair_bytes != 0) return false;
+  return DIA || reinterpret_cast<uintptr_t>(A->ell_ci) % 8 == 0;
+}
+
 template <bool DIA>
 int dispatch(mph_matrix_impl* A, int k, const void* x, void* y, int init,
              hipStream_t s) {
+  if (pair_ok<DIA>(A, k))
+    return A->vt == MPH_F32 ? launch_pair<float, DIA>(A, x, y, init, s)
+                            : launch_pair<bf16, DIA>(A, x, y, init, s);
   const bool two = A->tuning.rows_per_thread == 2;
   switch (A->vt) {
     case MPH_F64:

@@ -433,6 +433,27 @@ def test_reduced_precision_error(gpu, powerlaw, dtype, tol, name):
     assert err <= tol, err
 
 
+@pytest.mark.parametrize("dtype,tol", [("f32", 1e-5), ("bf16", 1e-2)])
+@pytest.mark.parametrize("name", ["DIA", "ELL"])
+@pytest.mark.parametrize("n", [20000, 20001, 31])
+@pytest.mark.parametrize("init", [True, False])
+def test_row_pair_kernels(gpu, dtype, tol, name, n, init):
+    """FP32 / BF16 DIA and ELL SpMV with two adjacent rows per thread (the
+    default launch shape; an odd row count, where ELL's pitch is odd and the
+    one-row kernel runs, and DIA's even pitch holds the pair's second row):
+    against the FP64 product, and equal to the one-row kernel."""
+    A = gen.banded(n, 4, 0.0, 7)
+    Ah = as_format(A, name)
+    x = gen.random_vector(A.ncols, 21)
+    y0 = gen.random_vector(A.nrows, 22)
+    y = gpu_spmv(Ah, x, dtype, y0=None if init else y0, init=init)
+    y1 = gpu_spmv(Ah, x, dtype, y0=None if init else y0, init=init, tuning=dict(rows_per_thread=1))
+    assert np.array_equal(y, y1)
+    ref = oracle.spmv_compensated(A, x) + (0 if init else y0)
+    scale = oracle.absrow(A, x) + (0 if init else np.abs(y0))
+    assert max_rel_err(y, ref, scale) <= tol
+
+
 # ------------------------------------------------------------ SpMM
 @pytest.mark.parametrize("name", ["CSR", "COO", "DIA", "ELL", "HYB", "HDC"])
 @pytest.mark.parametrize("k", [2, 8, 11])
