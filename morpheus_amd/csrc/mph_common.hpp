@@ -68,6 +68,26 @@ __device__ __forceinline__ bf16 ld_stream<bf16>(const bf16* p) {
   return r;
 }
 
+// Two adjacent FP32 / BF16 values loaded as one word (8 B / 4 B) and split.
+template <typename V>
+struct Pair;
+template <>
+struct Pair<float> {
+  using T = uint64_t;
+  __device__ __forceinline__ static void split(T w, float& lo, float& hi) {
+    lo = __uint_as_float((uint32_t)w);
+    hi = __uint_as_float((uint32_t)(w >> 32));
+  }
+};
+template <>
+struct Pair<bf16> {
+  using T = uint32_t;
+  __device__ __forceinline__ static void split(T w, float& lo, float& hi) {
+    lo = __uint_as_float(w << 16);
+    hi = __uint_as_float(w & 0xffff0000u);
+  }
+};
+
 // ---------------------------------------------------------------- wave64
 constexpr int kWave = 64;
 

@@ -154,25 +154,6 @@ __global__ __launch_bounds__(256) void ell_kernel(EllArgs<V> a) {
 // two diagonals / slots; an odd row count then leaves the pair's second
 // element inside the pitch) and the arrays are aligned to the pair.
 template <typename V>
-struct Pair;
-template <>
-struct Pair<float> {
-  using T = uint64_t;
-  __device__ __forceinline__ static void split(T w, float& lo, float& hi) {
-    lo = __uint_as_float((uint32_t)w);
-    hi = __uint_as_float((uint32_t)(w >> 32));
-  }
-};
-template <>
-struct Pair<bf16> {
-  using T = uint32_t;
-  __device__ __forceinline__ static void split(T w, float& lo, float& hi) {
-    lo = __uint_as_float(w << 16);
-    hi = __uint_as_float(w & 0xffff0000u);
-  }
-};
-
-template <typename V>
 __global__ __launch_bounds__(256) void dia_pair_kernel(DiaArgs<V> a) {
   using P = Pair<V>;
   const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
