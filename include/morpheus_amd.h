@@ -147,9 +147,11 @@ int mph_matrix_format(mph_matrix_t A);
  *                  of scattered cache-line fetches.  On (x-tiling) or from
  *                  65,536 rows, HYB / HDC run their ELL / DIA part as its own
  *                  pass and the COO / CSR part over the rows that have one;
- *                  a HYB whose ELL slots are less than a third filled runs
- *                  instead as CSR over a merged copy made by the analysis
- *                  (per row its ELL entries, then its COO entries). */
+ *                  a HYB whose ELL slots are less than a third filled, or
+ *                  in which more than an eighth of the rows have COO
+ *                  entries, runs instead as CSR over a merged copy made by
+ *                  the analysis (per row its ELL entries, then its COO
+ *                  entries); an HDC without diagonals runs as its CSR part. */
 typedef struct mph_tuning {
   int32_t rows_chunk;
   int32_t lanes;

@@ -44,8 +44,10 @@ constexpr int64_t kXTileMinRowLen = 512;       // rows longer than this are cand
 // coalesced pass and the tail / CSR part over the rows that have one.
 constexpr int64_t kSplitMinRows = 1 << 16;
 // HYB from kSplitMinRows rows (or with x-tiling on) whose ELL slots are
-// filled below this fraction runs over a merged CSR copy
+// filled below kHybMergeMaxFill, or with more than 1 / kHybMergeTailRows of
+// its rows holding COO entries, runs over a merged CSR copy
 constexpr double kHybMergeMaxFill = 1.0 / 3.0;
+constexpr int64_t kHybMergeTailRows = 8;
 
 // The x-tiled pass of one matrix's long rows (see xtile_kernel).
 struct XTilePlan {

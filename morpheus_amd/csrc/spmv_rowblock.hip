@@ -1383,7 +1383,10 @@ int split_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
   const size_t vec_bytes = A->vt == MPH_F64 ? 8 : 4;
   const int mode = A->tuning.xtile;
   const bool xt_on = mode == 2 || (mode == 0 && (int64_t)A->ncols * (int64_t)vec_bytes >= kXTileMinXBytes);
-  const bool prefixed = (A->format == MPH_HYB && !p.m_ro) || A->format == MPH_HDC;
+  // (an HDC without diagonals / a HYB without ELL slots is its CSR-shaped
+  // part: no prefix pass)
+  const bool prefixed = (A->format == MPH_HYB && !p.m_ro && A->ell_loop > 0) ||
+                        (A->format == MPH_HDC && A->ndiag > 0);
   const bool split = prefixed && (xt_on || nrows >= kSplitMinRows);
   if ((!xt_on && !split) || nrows == 0 || A->ncols == 0) return MPH_OK;
   const int32_t* d_ci = p.ci;
@@ -1543,9 +1546,11 @@ int split_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
 namespace {
 
 // A HYB with a row-sorted COO part, from kSplitMinRows rows or with x-tiling
-// on: when its ELL slots are filled below kHybMergeMaxFill, the product runs
-// over a merged CSR copy (p.m_*) instead of an ELL pass that reads mostly
-// padding.  Counts first; the copy is made only when the fill is low.
+// on: when its ELL slots are filled below kHybMergeMaxFill (an ELL pass would
+// read mostly padding), or when more than 1 / kHybMergeTailRows of its rows
+// have COO entries (the tail pass would be a second pass over most of the
+// row structure), the product runs over a merged CSR copy (p.m_*).  Counts
+// first; the copy is made only when one of the two holds.
 int hyb_merge(mph_matrix_impl* A, hipStream_t s) {
   RowBlockPlan& p = A->rb;
   const int64_t nrows = A->nrows, loop = A->ell_loop;
@@ -1591,7 +1596,19 @@ int hyb_merge(mph_matrix_impl* A, hipStream_t s) {
     set_error("HYB merge analysis check failed: counts");
     return fail(MPH_ERR_RUNTIME);
   }
-  if ((double)ell_valid >= kHybMergeMaxFill * (double)nrows * (double)loop) return fail(MPH_OK);
+  const bool sparse_ell = (double)ell_valid < kHybMergeMaxFill * (double)nrows * (double)loop;
+  if (!sparse_ell) {
+    std::vector<int32_t> tro(nrows + 1);
+    if (hipMemcpyAsync(tro.data(), p.tail_ro, sizeof(int32_t) * (nrows + 1),
+                       hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      set_error("HYB merge analysis: copy failed");
+      return fail(MPH_ERR_HIP);
+    }
+    int64_t tail_rows = 0;
+    for (int64_t r = 0; r < nrows; r++) tail_rows += tro[r + 1] > tro[r] ? 1 : 0;
+    if (tail_rows * kHybMergeTailRows <= nrows) return fail(MPH_OK);
+  }
   const size_t vs = value_size(A->vt);
   if (hipMalloc(&p.m_ci, sizeof(int32_t) * std::max<int64_t>(total, 1)) != hipSuccess ||
       hipMalloc(&p.m_val, vs * std::max<int64_t>(total, 1)) != hipSuccess) {

@@ -263,11 +263,11 @@ def test_split_two_pass_deterministic(gpu, tall):
     assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
 
 
-def holey_hyb(n, fill, seed):
+def holey_hyb(n, fill, seed, tail_every=10):
     """A HYB built by hand: K = 4 ELL slots per row, each filled with
     probability `fill` in any position (holes of -1 between entries, some
-    explicit zeros), and a row-sorted COO tail on every tenth row plus one
-    row of 3000 entries."""
+    explicit zeros), and a row-sorted COO tail on every `tail_every`-th row
+    plus one row of 3000 entries."""
     from morpheus_amd.formats import Ell, Hyb
     rng = np.random.default_rng(seed)
     K, width, pitch = 4, 32, n
@@ -278,7 +278,7 @@ def holey_hyb(n, fill, seed):
         ci[k * pitch: k * pitch + n] = np.where(on, rng.integers(0, n, n), -1)
         v[k * pitch: k * pitch + n] = np.where(on, rng.standard_normal(n) * (rng.random(n) > 0.05), 0.0)
     ell = Ell(n, n, int((ci >= 0).sum()), K, width, ci, v, pitch)
-    rows = [np.repeat(r, rng.integers(1, 4)) for r in range(0, n, 10)]
+    rows = [np.repeat(r, rng.integers(1, 4)) for r in range(0, n, tail_every)]
     rows.insert(len(rows) // 2, np.repeat(n // 2 + 1, 3000))
     ri = np.sort(np.concatenate(rows), kind="stable")
     cc = rng.integers(0, n, ri.size)
@@ -286,14 +286,15 @@ def holey_hyb(n, fill, seed):
     return Hyb(ell, coo)
 
 
-@pytest.mark.parametrize("fill", [0.2, 0.8])
+@pytest.mark.parametrize("fill,tail_every", [(0.2, 10), (0.8, 10), (0.8, 4)])
 @pytest.mark.parametrize("xtile", [0, 2])
 @pytest.mark.parametrize("init", [True, False])
-def test_hyb_merged_and_split_plans(gpu, fill, xtile, init):
+def test_hyb_merged_and_split_plans(gpu, fill, tail_every, xtile, init):
     """70k rows: a HYB whose ELL slots are a fifth filled runs over the merged
-    CSR copy, one four-fifths filled runs the ELL pass then the tail; both
-    against the Serial HYB product (ELL, then COO)."""
-    H = holey_hyb(70000, fill, 3)
+    CSR copy, one four-fifths filled with a tail on every tenth row runs the
+    ELL pass then the tail, and one with a tail on every fourth row the merged
+    copy again; all against the Serial HYB product (ELL, then COO)."""
+    H = holey_hyb(70000, fill, 3, tail_every)
     x = gen.random_vector(H.ncols, 17)
     y0 = gen.random_vector(H.nrows, 18)
     y = gpu_spmv(H, x, y0=y0, init=init, tuning=dict(xtile=xtile))
