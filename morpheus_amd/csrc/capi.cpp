@@ -116,6 +116,7 @@ int check_inputs(mph_matrix_t A, hipStream_t s) {
 
 int analyse(mph_matrix_t A, hipStream_t s) {
   if (A->analysed) return MPH_OK;
+  release_plans(A);  // what a failed earlier attempt left behind
   MPH_CHECK_STATUS(check_inputs(A, s));
   int st = MPH_OK;
   switch (A->format) {
