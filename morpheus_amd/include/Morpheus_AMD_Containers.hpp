@@ -21,8 +21,9 @@
 // Each sparse container carries a shared analysis cache: the C-ABI handle
 // (row blocks, carry scratch, tuned launch shape) of its device arrays, or,
 // for host containers, a device copy of the matrix plus that handle.  Calls
-// of the non-const index accessors, and any host-side element write through a
-// non-const accessor, mark the cache stale; resize detaches it.
+// of the non-const index and value accessors mark the cache stale (the
+// analysis of a device container holds copies of values: the x-tiled long
+// rows, a HYB's merged copy); resize detaches it.
 #pragma once
 
 #include <hip/hip_runtime_api.h>
@@ -601,7 +602,7 @@ class CooMatrix : public Impl::MatrixBase {
     return _column_indices(n);
   }
   value_array_reference values(size_type n) {
-    if (!memory_space::is_device) touch();
+    touch();
     return _values(n);
   }
   const non_const_index_type& crow_indices(size_type n) const { return _row_indices(n); }
@@ -616,7 +617,7 @@ class CooMatrix : public Impl::MatrixBase {
     return _column_indices;
   }
   value_array_type& values() {
-    if (!memory_space::is_device) touch();
+    touch();
     return _values;
   }
   const index_array_type& crow_indices() const { return _row_indices; }
@@ -696,7 +697,7 @@ class CsrMatrix : public Impl::MatrixBase {
     return _column_indices(n);
   }
   value_array_reference values(size_type n) {
-    if (!memory_space::is_device) touch();
+    touch();
     return _values(n);
   }
   const non_const_index_type& crow_offsets(size_type n) const { return _row_offsets(n); }
@@ -711,7 +712,7 @@ class CsrMatrix : public Impl::MatrixBase {
     return _column_indices;
   }
   value_array_type& values() {
-    if (!memory_space::is_device) touch();
+    touch();
     return _values;
   }
   const index_array_type& crow_offsets() const { return _row_offsets; }
@@ -784,7 +785,7 @@ class DiaMatrix : public Impl::MatrixBase {
   }
   // values(i, d) = A(i, i + offset[d])
   value_array_reference values(size_type i, size_type j) {
-    if (!memory_space::is_device) touch();
+    touch();
     return _values(i, j);
   }
   const non_const_index_type& cdiagonal_offsets(size_type n) const { return _diagonal_offsets(n); }
@@ -794,7 +795,7 @@ class DiaMatrix : public Impl::MatrixBase {
     return _diagonal_offsets;
   }
   value_array_type& values() {
-    if (!memory_space::is_device) touch();
+    touch();
     return _values;
   }
   const index_array_type& cdiagonal_offsets() const { return _diagonal_offsets; }
@@ -874,7 +875,7 @@ class EllMatrix : public Impl::MatrixBase {
     return _column_indices(i, j);
   }
   value_array_reference values(size_type i, size_type j) {
-    if (!memory_space::is_device) touch();
+    touch();
     return _values(i, j);
   }
   const non_const_index_type& ccolumn_indices(size_type i, size_type j) const {
@@ -886,7 +887,7 @@ class EllMatrix : public Impl::MatrixBase {
     return _column_indices;
   }
   value_array_type& values() {
-    if (!memory_space::is_device) touch();
+    touch();
     return _values;
   }
   const index_matrix_type& ccolumn_indices() const { return _column_indices; }

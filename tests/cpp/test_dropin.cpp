@@ -485,6 +485,42 @@ TEST(Multiply_PlanInvalidation) {
   EXPECT_TRUE(within(y, ref));
 }
 
+// A device container whose values are rewritten in place through values():
+// the cached analysis (here the x-tiled copy of a 2000-entry row, x of
+// 2.4 MB) is rebuilt, so the second product sees the new values.
+TEST(Multiply_PlanInvalidation_DeviceValues) {
+  const size_t nr = 4, nc = 300000, nlong = 2000;
+  const size_t nnz = nlong + 3 * 5;
+  Csr<Host> A(nr, nc, nnz);
+  size_t k = 0;
+  A.row_offsets(0) = 0;
+  for (size_t r = 0; r < nr; r++) {
+    const size_t len = r == 0 ? nlong : 5;
+    for (size_t j = 0; j < len; j++, k++) {
+      A.column_indices(k) = (int)(j * (nc / len) + r);
+      A.values(k) = 1.0 + 0.001 * (double)k;
+    }
+    A.row_offsets(r + 1) = (int)k;
+  }
+  auto Ad = Morpheus::create_mirror_container<Dev>(A);
+  Morpheus::copy(A, Ad);
+  const auto x = random_x(nc, 9);
+  Vec<Dev> xd(nc), yd(nr, 0.0);
+  Morpheus::copy(x, xd);
+  Morpheus::multiply<HIPExec>(Ad, xd, yd);
+  for (size_t q = 0; q < nnz; q++) A.values(q) *= -2.0;
+  // rewrite the device values in place
+  const bool copied = hipMemcpy(Ad.values().data(), A.cvalues().data(), sizeof(double) * nnz,
+                                hipMemcpyHostToDevice) == hipSuccess;
+  EXPECT_TRUE(copied);
+  Morpheus::multiply<HIPExec>(Ad, xd, yd);
+  Vec<Host> y(nr, 0.0);
+  Morpheus::copy(yd, y);
+  Coo<Host> c;
+  Morpheus::convert<Serial>(A, c);
+  EXPECT_TRUE(within(y, dense_ref(c, x, nullptr)));
+}
+
 // ============================================================ convert
 template <int S, int D, class ExecSpace>
 static void convert_pair() {
