@@ -207,6 +207,37 @@ def cold_and_graph(mats, vecs, order, reps):
     return cold, graph
 
 
+def read_stream_flushed(nbytes, reps=20):
+    """The device's read rate at a given size, as the flushed products see
+    it: a one-pass sum over `nbytes` of FP64 data (torch's reduction kernel:
+    read once, nothing written but the result), median of `reps` runs each
+    after the same 1 GiB MALL-flushing sweep, outside the events.  GB/s."""
+    import numpy as np
+    import torch
+    stream = torch.cuda.current_stream()
+    scratch = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    data = torch.ones(max(1, nbytes // 8), dtype=torch.float64, device="cuda")
+    sink = torch.empty((), dtype=torch.float64, device="cuda")
+    out = torch.empty((), dtype=torch.float64, device="cuda")
+    torch.sum(data, dim=0, out=out)
+    evs = []
+    for _ in range(reps):
+        torch.sum(scratch, dim=0, out=sink)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        torch.sum(data, dim=0, out=out)
+        e1.record(stream)
+        evs.append((e0, e1))
+    torch.cuda.synchronize()
+    ms = float(np.median([a.elapsed_time(b) for a, b in evs]))
+    del scratch, data
+    return data_gbs(nbytes, ms)
+
+
+def data_gbs(nbytes, ms):
+    return nbytes / (ms * 1e-3) / 1e9
+
+
 def run_cg(mats, iters):
     """The consumer side: `iters` fused CG iterations on S7 (the SPD 3-D
     Laplacian) with the DIA and ELL handles of the step, captured once per
@@ -577,6 +608,16 @@ def main():
     stream_gbs = 5 * 2 * sa.numel() * 8 / (e0.elapsed_time(e1) * 1e-3) / 1e9
     del sa, sb
     log(f"elementwise 1 GiB read + 1 GiB write: {stream_gbs:.0f} GB/s")
+    # per format: the read rate of a plain one-pass sum over the product's
+    # algorithmic bytes, MALL flushed as for ms_mall_flushed: how far each
+    # product is from what a pure read stream of its size reaches here
+    for n, rec in formats.items():
+        rs = read_stream_flushed(rec["bytes_alg"])
+        rec["read_stream_same_size_gbs"] = round(rs, 1)
+        rec["frac_of_read_stream_mall_flushed"] = round(
+            data_gbs(rec["bytes_alg"], rec["ms_mall_flushed"]) / rs, 4)
+        log(f"{n}: a {rec['bytes_alg'] / 1e6:.0f} MB read stream runs at {rs:.0f} GB/s; "
+            f"the product (MALL flushed) at {rec['frac_of_read_stream_mall_flushed']:.2f} of it")
 
     cg = run_cg(mats, 100)
     batch = None if args.no_batch else run_batch(args.batch_steps)
