@@ -49,6 +49,9 @@ def parse():
                     help="also run a 1 GiB device copy (PMC traffic calibration)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the library's default launch shapes (no autotuning)")
+    ap.add_argument("--large", action="store_true",
+                    help="also time FP64 products on S27 (3D 27-point, 160^3) and Z12 "
+                         "(2M rows, power-law(1.2), mean 30): parity configs, off by default")
     ap.add_argument("--no-sweep", action="store_true",
                     help="skip the format x precision sweep (FP64 / FP32 / BF16 on S7, Z15, B9)")
     ap.add_argument("--formats", default="",
@@ -205,6 +208,71 @@ def cold_and_graph(mats, vecs, order, reps):
         del g
     del scratch, sink
     return cold, graph
+
+
+def run_large(reps=10):
+    """FP64 products on the two large configurations (S27: 3D 27-point
+    stencil on 160^3, 4.1M rows, 109M nnz; Z12: 2M rows, power-law(1.2) row
+    lengths, mean 30, cap 200k), each format the reference admits: time
+    (median of `reps` back-to-back products), fraction of 8 TB/s and of a
+    same-size read stream, and the FP64 per-row bound against the oracle."""
+    import numpy as np
+    import torch
+    import oracle  # the oracle: FP64 reference products and row scales
+    from morpheus_amd import capi, device
+    from morpheus_amd import generators as gen
+    from morpheus_amd.formats import FORMAT_NAMES
+    stream = torch.cuda.current_stream()
+    out = {}
+    for mname, make in (("S27", gen.S27), ("Z12", gen.Z12)):
+        t0 = time.time()
+        A = make()
+        log(f"large: generated {mname} (nnz {A.nnz}) in {time.time() - t0:.1f}s")
+        x = gen.random_vector(A.ncols, 31)
+        ref = oracle.spmv(A, x)
+        bound = 4.0 * (np.diff(A.row_offsets) + 1) * 2.0 ** -53 * oracle.absrow(A, x)
+        xt = torch.from_numpy(x).cuda()
+        y = torch.empty(A.nrows, dtype=torch.float64, device="cuda")
+        base = device.to_device(A, "f64")
+        out[mname] = {"nnz": int(A.nnz)}
+        for f, fname in enumerate(FORMAT_NAMES):
+            try:
+                D = base if f == base.format else device.convert(base, f)
+            except capi.FormatConversionException:
+                out[mname][fname] = "FormatConversionException"
+                continue
+            D.analyse()
+            D.multiply(xt, y, True, stream)
+            torch.cuda.synchronize()
+            ok = bool(np.all(np.abs(y.cpu().numpy() - ref) <= bound))
+            ts = []
+            for _ in range(reps):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                D.multiply(xt, y, True, stream)
+                e1.record(stream)
+                torch.cuda.synchronize()
+                ts.append(e0.elapsed_time(e1))
+            ms = float(np.median(ts))
+            nbytes = D.algorithmic_bytes()
+            rs = read_stream_flushed(nbytes, 5)
+            out[mname][fname] = {
+                "ms": round(ms, 4),
+                "gflops": round(2.0 * A.nnz / (ms * 1e-3) / 1e9, 2),
+                "bytes_alg": int(nbytes),
+                "hbm_frac": round(data_gbs(nbytes, ms) / HBM_PEAK_GBS, 4),
+                "read_stream_same_size_gbs": round(rs, 1),
+                "frac_of_read_stream": round(data_gbs(nbytes, ms) / rs, 4),
+                "within_fp64_bound": ok,
+            }
+            log(f"large {mname} {fname}: {ms * 1e3:.0f} us, "
+                f"{out[mname][fname]['hbm_frac']:.2f} of 8 TB/s, "
+                f"{out[mname][fname]['frac_of_read_stream']:.2f} of a read stream, bound {ok}")
+            if D is not base:
+                D.close()
+        base.close()
+        del A
+    return out
 
 
 def read_stream_flushed(nbytes, reps=20):
@@ -622,6 +690,7 @@ def main():
     cg = run_cg(mats, 100)
     batch = None if args.no_batch else run_batch(args.batch_steps)
     sweep = None if (args.no_sweep or only) else run_sweep(z15, s7, 20)
+    large = run_large() if args.large else None
 
     cpu = None
     if not args.no_cpu:
@@ -677,6 +746,8 @@ def main():
         "cg_iteration": cg,
         "precision_sweep": sweep,
     }
+    if large is not None:
+        line["large_configs"] = large
     print(json.dumps(line), flush=True)
 
 
