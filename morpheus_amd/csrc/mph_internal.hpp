@@ -195,7 +195,8 @@ uint8_t block_lanes(int nr, const int32_t* ro_begin);
 // x-tiled long rows.  xtile_build plans the candidate rows `lrows` (host row
 // offsets `ro`, device row offsets / columns / values) when the x-tile mode
 // and x size call for it; rows with unsorted columns cancel the plan and rows
-// with a run longer than half a sub-block in some tile are left out.
+// with a run longer than a sub-block (kXTileCap items) in some tile are left
+// out.
 // tiled[r] != 0 marks the rows taken.  xtile_multiply runs the pass of a
 // plan without DIA / ELL parts (COO) and writes y of the tiled rows (x and y
 // read with element stride ld).

@@ -1148,12 +1148,13 @@ int xtile_build(mph_matrix_impl* A, const std::vector<int32_t>& ro, const int32_
         return MPH_ERR_RUNTIME;
       }
   }
-  // tiled rows: every run fits half a sub-block
+  // tiled rows: every run fits a sub-block (a longer run, over an eighth of
+  // a tile's columns, leaves its row to the position chunks)
   std::vector<int32_t> trow;     // indices into lrows
   for (int32_t i = 0; i < nl; i++) {
     int32_t mx = 0;
     for (int32_t q = 0; q < ntiles; q++) mx = std::max(mx, B(q + 1, i) - B(q, i));
-    if (mx <= kXTileCap / 2) {
+    if (mx <= kXTileCap) {
       trow.push_back(i);
       tiled[lrows[i]] = 1;
     }

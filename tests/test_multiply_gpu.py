@@ -196,6 +196,34 @@ def test_xtile_long_rows_at_the_ends(gpu, name, init):
     check_fp64(A, as_format(A, name), x, y0, init, tuning=dict(xtile=2))
 
 
+@pytest.mark.parametrize("xtile", [0, 2])
+@pytest.mark.parametrize("init", [True, False])
+def test_xtile_run_limit(gpu, xtile, init):
+    """Runs of a long row inside one 4096-column tile: 512 items (a whole
+    sub-block: the row is x-tiled) and 513 (the row keeps the position
+    chunks), beside rows of 450 items per tile over many tiles."""
+    rng = np.random.default_rng(12)
+    m, n = 300000, 400
+    rows = []
+    for r in range(n):
+        if r == 5:
+            cols = np.concatenate([np.arange(512), 8192 + rng.choice(200000, 100, replace=False)])
+        elif r == 9:
+            cols = np.concatenate([4096 + np.arange(513), 12288 + rng.choice(200000, 100, replace=False)])
+        elif r % 50 == 7:
+            cols = np.concatenate([t * 4096 + rng.choice(4096, 450, replace=False) for t in range(0, 70, 3)])
+        else:
+            cols = rng.choice(m, int(rng.integers(0, 12)), replace=False)
+        rows.append(np.unique(cols))
+    ro = np.concatenate([[0], np.cumsum([c.size for c in rows])]).astype(np.int32)
+    A = Csr(n, m, ro, np.concatenate(rows).astype(np.int32),
+            rng.standard_normal(int(ro[-1])))
+    x = gen.random_vector(m, 13)
+    y0 = gen.random_vector(n, 14)
+    for name in ("CSR", "COO", "HYB"):
+        check_fp64(A, as_format(A, name), x, y0, init, tuning=dict(xtile=xtile))
+
+
 def test_xtile_unsorted_columns(gpu):
     """Long rows whose columns are not sorted keep the position chunks."""
     A = gen.powerlaw_rows(4000, 300000, 1.5, 30.0, 6000, seed=8)
