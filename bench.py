@@ -49,9 +49,9 @@ def parse():
                     help="also run a 1 GiB device copy (PMC traffic calibration)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the library's default launch shapes (no autotuning)")
-    ap.add_argument("--large", action="store_true",
-                    help="also time FP64 products on S27 (3D 27-point, 160^3) and Z12 "
-                         "(2M rows, power-law(1.2), mean 30): parity configs, off by default")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the FP64 products on S27 (3D 27-point, 160^3) and Z12 "
+                         "(2M rows, power-law(1.2), mean 30), the large parity configs")
     ap.add_argument("--no-sweep", action="store_true",
                     help="skip the format x precision sweep (FP64 / FP32 / BF16 on S7, Z15, B9)")
     ap.add_argument("--formats", default="",
@@ -690,7 +690,7 @@ def main():
     cg = run_cg(mats, 100)
     batch = None if args.no_batch else run_batch(args.batch_steps)
     sweep = None if (args.no_sweep or only) else run_sweep(z15, s7, 20)
-    large = run_large() if args.large else None
+    large = None if (args.no_large or only) else run_large()
 
     cpu = None
     if not args.no_cpu:
