@@ -38,6 +38,7 @@ constexpr int kXTileBytes = 32 << 10;          // x tile in LDS
 constexpr int kXTileCap = 512;                 // items of a (one-wave) sub-block
 constexpr int kXTileRows = 64;                 // rows of a sub-block
 constexpr int kXTileMaxGroupRows = 1024;       // rows of a group (run offsets in LDS)
+constexpr int64_t kXTileMaxBounds = 1 << 26;  // entries of the per-row tile-bound table for a halved tile
 constexpr int64_t kXTileMinXBytes = 2 << 20;  // auto: x larger than this
 constexpr int64_t kXTileMinRowLen = 512;       // rows longer than this are candidates
 // HYB / HDC with at least this many rows run the ELL / DIA part as its own
@@ -196,7 +197,8 @@ uint8_t block_lanes(int nr, const int32_t* ro_begin);
 // offsets `ro`, device row offsets / columns / values) when the x-tile mode
 // and x size call for it; rows with unsorted columns cancel the plan and rows
 // with a run longer than a sub-block (kXTileCap items) in some tile are left
-// out.
+// out (the tile is halved, down to an eighth, while more than an eighth of
+// the candidate items would be).
 // tiled[r] != 0 marks the rows taken.  xtile_multiply runs the pass of a
 // plan without DIA / ELL parts (COO) and writes y of the tiled rows (x and y
 // read with element stride ld).

@@ -1096,12 +1096,15 @@ int xtile_build(mph_matrix_impl* A, const std::vector<int32_t>& ro, const int32_
     return ro[u + 1] - ro[u] > ro[v + 1] - ro[v];
   });
   const int32_t nl = (int32_t)lrows.size();
-  const int32_t tile = (int32_t)(kXTileBytes / vec_bytes);
-  const int32_t ntiles = (int32_t)ceil_div((int64_t)A->ncols, (int64_t)tile);
+  const int32_t tile_max = (int32_t)(kXTileBytes / vec_bytes);
+  int32_t tile = tile_max, ntiles = 0;
 
-  // per-row tile bounds on the device
-  std::vector<int32_t> bnd((size_t)nl * (ntiles + 1)), uns(nl);
-  {
+  // per-row tile bounds on the device for a tile of tl columns, checked here
+  std::vector<int32_t> bnd, uns(nl);
+  auto bounds = [&](int32_t tl) -> int {
+    tile = tl;
+    ntiles = (int32_t)ceil_div((int64_t)A->ncols, (int64_t)tile);
+    bnd.assign((size_t)nl * (ntiles + 1), 0);
     int32_t *d_rows = nullptr, *d_bnd = nullptr, *d_uns = nullptr;
     int st = MPH_OK;
     if (hipMalloc(&d_rows, sizeof(int32_t) * nl) != hipSuccess ||
@@ -1132,29 +1135,53 @@ int xtile_build(mph_matrix_impl* A, const std::vector<int32_t>& ro, const int32_
     dfree(d_bnd);
     dfree(d_uns);
     if (st != MPH_OK) return st;
-  }
-  for (int32_t i = 0; i < nl; i++)
-    if (uns[i]) return MPH_OK;  // unsorted columns: keep the position chunks
-  auto B = [&](int32_t q, int32_t i) { return bnd[(size_t)q * nl + i]; };
-  for (int32_t i = 0; i < nl; i++) {
-    const int32_t r = lrows[i];
-    if (B(0, i) != ro[r] || B(ntiles, i) != ro[r + 1]) {
-      set_error("x-tile analysis check failed: row bounds");
-      return MPH_ERR_RUNTIME;
-    }
-    for (int32_t q = 0; q < ntiles; q++)
-      if (B(q, i) > B(q + 1, i)) {
-        set_error("x-tile analysis check failed: bounds not monotone");
+    for (int32_t i = 0; i < nl; i++) {
+      const int32_t r = lrows[i];
+      if (bnd[i] != ro[r] || bnd[(size_t)ntiles * nl + i] != ro[r + 1]) {
+        set_error("x-tile analysis check failed: row bounds");
         return MPH_ERR_RUNTIME;
       }
-  }
-  // tiled rows: every run fits a sub-block (a longer run, over an eighth of
-  // a tile's columns, leaves its row to the position chunks)
-  std::vector<int32_t> trow;     // indices into lrows
-  for (int32_t i = 0; i < nl; i++) {
+      for (int32_t q = 0; q < ntiles; q++)
+        if (bnd[(size_t)q * nl + i] > bnd[(size_t)(q + 1) * nl + i]) {
+          set_error("x-tile analysis check failed: bounds not monotone");
+          return MPH_ERR_RUNTIME;
+        }
+    }
+    return MPH_OK;
+  };
+  auto B = [&](int32_t q, int32_t i) { return bnd[(size_t)q * nl + i]; };
+  // a row takes the x-tiled pass when every one of its runs fits a
+  // sub-block (a longer run, over an eighth of a tile's columns, leaves its
+  // row to the position chunks)
+  auto fits = [&](int32_t i) {
     int32_t mx = 0;
     for (int32_t q = 0; q < ntiles; q++) mx = std::max(mx, B(q + 1, i) - B(q, i));
-    if (mx <= kXTileCap) {
+    return mx <= kXTileCap;
+  };
+
+  MPH_CHECK_STATUS(bounds(tile_max));
+  for (int32_t i = 0; i < nl; i++)
+    if (uns[i]) return MPH_OK;  // unsorted columns: keep the position chunks
+  // Rows dense enough that a run exceeds a sub-block: a tile of half the
+  // columns halves their runs.  The tile is halved (down to an eighth of
+  // the LDS tile) while more than an eighth of the candidate items would be
+  // left to the position chunks; FP32 / BF16 tiles hold twice the columns
+  // of FP64 ones, so a matrix whose longest rows fit FP64 tiles may not fit
+  // FP32 ones.
+  for (;;) {
+    int64_t all = 0, left = 0;
+    for (int32_t i = 0; i < nl; i++) {
+      const int64_t len = (int64_t)ro[lrows[i] + 1] - ro[lrows[i]];
+      all += len;
+      if (!fits(i)) left += len;
+    }
+    if (8 * left <= all || tile <= tile_max / 8) break;
+    if ((int64_t)nl * (2 * (int64_t)ntiles + 1) > kXTileMaxBounds) break;  // host table size
+    MPH_CHECK_STATUS(bounds(tile / 2));
+  }
+  std::vector<int32_t> trow;     // indices into lrows
+  for (int32_t i = 0; i < nl; i++) {
+    if (fits(i)) {
       trow.push_back(i);
       tiled[lrows[i]] = 1;
     }
@@ -1317,10 +1344,12 @@ int xtile_build(mph_matrix_impl* A, const std::vector<int32_t>& ro, const int32_
   }
   dfree(d_tb);
   dfree(d_rs);
-  // the x tile plus the sub-block products exceed the default 64 KiB of LDS
+  // the x tile plus the sub-block products exceed the default 64 KiB of LDS;
+  // the attribute is per kernel, so it is set for the largest tile (a plan
+  // with a halved tile launches with less)
   {
     hipError_t e = hipSuccess;
-    const size_t xlds = xtile_lds_bytes(vec_bytes, acc, tile);
+    const size_t xlds = xtile_lds_bytes(vec_bytes, acc, tile_max);
     if (A->vt == MPH_F64)
       e = hipFuncSetAttribute(reinterpret_cast<const void*>(&xtile_kernel<double, false>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)xlds);

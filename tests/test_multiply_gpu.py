@@ -224,6 +224,36 @@ def test_xtile_run_limit(gpu, xtile, init):
         check_fp64(A, as_format(A, name), x, y0, init, tuning=dict(xtile=xtile))
 
 
+@pytest.mark.parametrize("xtile", [0, 2])
+def test_xtile_dense_rows_halve_the_tile(gpu, xtile):
+    """Rows with 1,500 items in each of 15 windows of 4,096 columns: every
+    run exceeds a sub-block in the full tile (4,096 FP64 / 8,192 FP32
+    columns), so the plan halves the tile until the runs fit (1,024
+    columns); FP64 within the per-row bound, FP32 / BF16 against the
+    compensated oracle."""
+    rng = np.random.default_rng(21)
+    m, n = 300000, 600
+    rows = []
+    for r in range(n):
+        if r % 15 == 3:
+            cols = np.concatenate([t * 4096 + rng.choice(4096, 1500, replace=False)
+                                   for t in range(0, 60, 4)])
+        else:
+            cols = rng.choice(m, int(rng.integers(0, 30)), replace=False)
+        rows.append(np.unique(cols))
+    ro = np.concatenate([[0], np.cumsum([c.size for c in rows])]).astype(np.int32)
+    A = Csr(n, m, ro, np.concatenate(rows).astype(np.int32),
+            rng.standard_normal(int(ro[-1])))
+    x = gen.random_vector(m, 22)
+    y0 = gen.random_vector(n, 23)
+    for name in ("CSR", "COO", "HYB"):
+        check_fp64(A, as_format(A, name), x, y0, False, tuning=dict(xtile=xtile))
+    ref = oracle.spmv_compensated(A, x)
+    for dtype, tol in (("f32", 1e-5), ("bf16", 1e-2)):
+        y = gpu_spmv(A, x, dtype, tuning=dict(xtile=xtile))
+        assert max_rel_err(y, ref, oracle.absrow(A, x)) <= tol
+
+
 def test_xtile_unsorted_columns(gpu):
     """Long rows whose columns are not sorted keep the position chunks."""
     A = gen.powerlaw_rows(4000, 300000, 1.5, 30.0, 6000, seed=8)
