@@ -271,8 +271,86 @@ def run_large(reps=10):
             if D is not base:
                 D.close()
         base.close()
+        out[mname]["autotuned"] = run_large_autotuned(A, mname)
         del A
     return out
+
+
+def run_large_autotuned(A, mname, ks=(1, 4, 16), reps=5):
+    """The DynamicMatrix switching run on one large matrix: per value type
+    (FP64, FP32, BF16; int32 indices) the autotuner times every admissible
+    format at k = 4 right-hand sides and picks one; the picked format's
+    product with k in `ks` (row-major X / Y, U[-1,1]) is then timed (median
+    of `reps` back to back) and its first column checked against the FP64
+    oracle: the per-row FP64 bound, or the max-row relative error
+    max_i |y_i - y_ref_i| / sum_j |a_ij x_j| for FP32 / BF16."""
+    import numpy as np
+    import torch
+    import oracle  # the oracle: FP64 reference products and row scales
+    from morpheus_amd import device
+    from morpheus_amd.formats import FORMAT_NAMES
+    stream = torch.cuda.current_stream()
+    rng = np.random.default_rng(41)
+    X = {k: rng.uniform(-1.0, 1.0, (A.ncols, k)) for k in ks}
+    col0 = {k: np.ascontiguousarray(X[k][:, 0]) for k in ks}
+    ref = {k: oracle.spmv_compensated(A, col0[k]) for k in ks}
+    scale = {k: oracle.absrow(A, col0[k]) for k in ks}
+    nrow = np.diff(A.row_offsets)
+    res = {}
+    for dtype in ("f64", "f32", "bf16"):
+        t0 = time.time()
+        D = device.to_device(A, dtype)
+        best, trials = device.autotune_format(D, k=4, reps=3)
+        M = D if best == D.format else device.convert(D, best)
+        M.analyse()
+        rec = {"format": FORMAT_NAMES[best],
+               "trial_ms_k4": {FORMAT_NAMES[f]: round(t, 4) for f, t in trials.items()}}
+        vdt = device.vector_dtype(dtype)
+        for k in ks:
+            Xt = torch.from_numpy(X[k] if k > 1 else col0[k]).cuda().to(vdt)
+            Y = torch.empty((A.nrows, k) if k > 1 else (A.nrows,), dtype=vdt, device="cuda")
+            run = (lambda: M.multiply(Xt, Y, True, stream)) if k == 1 else \
+                (lambda: M.multiply_multi(Xt, Y, True, stream))
+            run()
+            torch.cuda.synchronize()
+            y0 = (Y if k == 1 else Y[:, 0]).double().cpu().numpy()
+            ts = []
+            for _ in range(reps):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                run()
+                e1.record(stream)
+                torch.cuda.synchronize()
+                ts.append(e0.elapsed_time(e1))
+            ms = float(np.median(ts))
+            nbytes = M.algorithmic_bytes(k)
+            mask = scale[k] > 0
+            err = float(np.max(np.abs(y0[mask] - ref[k][mask]) / scale[k][mask])) if mask.any() else 0.0
+            r = {"ms": round(ms, 4),
+                 "gflops": round(2.0 * A.nnz * k / (ms * 1e-3) / 1e9, 2),
+                 "matrhs_per_s": round(k / (ms * 1e-3), 1),
+                 "bytes_alg": int(nbytes),
+                 "hbm_frac": round(data_gbs(nbytes, ms) / HBM_PEAK_GBS, 4),
+                 "max_rel_err_col0": err}
+            if dtype == "f64":
+                r["within_fp64_bound_col0"] = bool(np.all(
+                    np.abs(y0 - ref[k]) <= 4.0 * (nrow + 1) * 2.0 ** -53 * scale[k]))
+            rec[f"k{k}"] = r
+            del Xt, Y
+        if M is not D:
+            M.close()
+        D.close()
+        res[dtype] = rec
+        log(f"large {mname} autotuned {dtype}: {rec['format']}; " + ", ".join(
+            f"k={k} {rec[f'k{k}']['ms'] * 1e3:.0f} us ({rec[f'k{k}']['hbm_frac']:.2f} of 8 TB/s, "
+            f"err {rec[f'k{k}']['max_rel_err_col0']:.1e})" for k in ks)
+            + f" [{time.time() - t0:.1f}s]")
+    for dtype in ("f32", "bf16"):
+        res[dtype]["matrhs_per_s_vs_f64"] = {
+            f"k{k}": round(res[dtype][f"k{k}"]["matrhs_per_s"] / res["f64"][f"k{k}"]["matrhs_per_s"], 3)
+            for k in ks}
+    res["tolerance"] = {"f32": 1e-5, "bf16": 1e-2}
+    return res
 
 
 def read_stream_flushed(nbytes, reps=20):

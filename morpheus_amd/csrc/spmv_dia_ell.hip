@@ -145,6 +145,124 @@ __global__ __launch_bounds__(256) void ell_kernel(EllArgs<V> a) {
   }
 }
 
+// ----------------------------------------------------------- SpMM lane groups
+// k right-hand sides (row-major X[ncols][ld], Y[nrows][ld]): KL lanes per
+// row, each owning 16 bytes of the row's columns (2 FP64 / 4 FP32), so a
+// row of x or y moves as KL adjacent 16-byte pieces, and a wave moves 64 / KL
+// rows.  DIA's x rows of adjacent rows are adjacent, so a wave's x load
+// reads one contiguous 1 KiB span; the y stores are contiguous the same way.
+// (One thread per row with k scalar loads instead puts 64 rows 8k bytes
+// apart into every load instruction: 64 cache lines per instruction.)  The
+// KL lanes of a row load the same value / index (one broadcast line).  Sums
+// run over diagonals / slots in the same order as the one-thread kernels.
+template <typename Vec>
+struct Piece;
+template <>
+struct Piece<double> {
+  using T = double __attribute__((ext_vector_type(2)));
+  static constexpr int Q = 2;
+};
+template <>
+struct Piece<float> {
+  using T = float __attribute__((ext_vector_type(4)));
+  static constexpr int Q = 4;
+};
+
+template <typename V, int KL>
+__global__ __launch_bounds__(256) void dia_spmm_kernel(DiaArgs<V> a) {
+  using Acc = typename vtraits<V>::acc;
+  using Vec = typename vtraits<V>::vec;
+  using P = Piece<Vec>;
+  constexpr int Q = P::Q;
+  const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t row = (int64_t)b * (256 / KL) + threadIdx.x / KL;
+  const int l = threadIdx.x % KL;
+  if (row >= a.nrows) return;
+  Acc t[Q];
+#pragma unroll
+  for (int q = 0; q < Q; q++) t[q] = 0;
+  for (int d0 = 0; d0 < a.ndiag; d0 += 8) {
+    Acc v[8];
+    int64_t c[8];
+#pragma unroll
+    for (int w = 0; w < 8; w++) {
+      const int d = d0 + w;
+      const int64_t o = d < a.ndiag ? (int64_t)a.off[d] : 0;
+      const int64_t col = row + o;
+      const bool ok = d < a.ndiag && col >= 0 && col < a.ncols;
+      c[w] = ok ? col : -1;
+      v[w] = ok ? vtraits<V>::widen(ld_stream(a.val + a.pitch * d + row)) : (Acc)0;
+    }
+    typename P::T xv[8];
+#pragma unroll
+    for (int w = 0; w < 8; w++)
+      xv[w] = c[w] >= 0 ? *reinterpret_cast<const typename P::T*>(a.x + c[w] * a.ld + l * Q)
+                        : (typename P::T)0;
+#pragma unroll
+    for (int w = 0; w < 8; w++)
+      if (c[w] >= 0)
+#pragma unroll
+        for (int q = 0; q < Q; q++) t[q] += v[w] * (Acc)xv[w][q];
+  }
+  typename P::T* yp = reinterpret_cast<typename P::T*>(a.y + row * a.ld + l * Q);
+  typename P::T out;
+  if (a.init) {
+#pragma unroll
+    for (int q = 0; q < Q; q++) out[q] = (Vec)t[q];
+  } else {
+    const typename P::T old = *yp;
+#pragma unroll
+    for (int q = 0; q < Q; q++) out[q] = (Vec)((Acc)old[q] + t[q]);
+  }
+  *yp = out;
+}
+
+template <typename V, int KL>
+__global__ __launch_bounds__(256) void ell_spmm_kernel(EllArgs<V> a) {
+  using Acc = typename vtraits<V>::acc;
+  using Vec = typename vtraits<V>::vec;
+  using P = Piece<Vec>;
+  constexpr int Q = P::Q;
+  const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t row = (int64_t)b * (256 / KL) + threadIdx.x / KL;
+  const int l = threadIdx.x % KL;
+  if (row >= a.nrows) return;
+  Acc t[Q];
+#pragma unroll
+  for (int q = 0; q < Q; q++) t[q] = 0;
+  for (int n0 = 0; n0 < a.loop; n0 += 8) {
+    int64_t c[8];
+#pragma unroll
+    for (int w = 0; w < 8; w++)
+      c[w] = n0 + w < a.loop ? (int64_t)ld_stream(a.ci + row + a.pitch * (n0 + w)) : -1;
+    Acc v[8];
+#pragma unroll
+    for (int w = 0; w < 8; w++)
+      v[w] = c[w] >= 0 ? vtraits<V>::widen(ld_stream(a.val + row + a.pitch * (n0 + w))) : (Acc)0;
+    typename P::T xv[8];
+#pragma unroll
+    for (int w = 0; w < 8; w++)
+      xv[w] = c[w] >= 0 ? *reinterpret_cast<const typename P::T*>(a.x + c[w] * a.ld + l * Q)
+                        : (typename P::T)0;
+#pragma unroll
+    for (int w = 0; w < 8; w++)
+      if (c[w] >= 0)
+#pragma unroll
+        for (int q = 0; q < Q; q++) t[q] += v[w] * (Acc)xv[w][q];
+  }
+  typename P::T* yp = reinterpret_cast<typename P::T*>(a.y + row * a.ld + l * Q);
+  typename P::T out;
+  if (a.init) {
+#pragma unroll
+    for (int q = 0; q < Q; q++) out[q] = (Vec)t[q];
+  } else {
+    const typename P::T old = *yp;
+#pragma unroll
+    for (int q = 0; q < Q; q++) out[q] = (Vec)((Acc)old[q] + t[q]);
+  }
+  *yp = out;
+}
+
 // ----------------------------------------------------------- row pairs
 // FP32 / BF16 SpMV: a thread takes the adjacent rows 2t and 2t + 1, so that
 // one load moves the two rows' values of a diagonal / slot (8 B in FP32, 4 B
@@ -266,15 +384,69 @@ int launch_ell_k(mph_matrix_impl* A, const void* x, void* y, int ld, int init,
   return MPH_OK;
 }
 
+template <typename V, bool DIA, int KL>
+int launch_spmm_lanes(mph_matrix_impl* A, const void* x, void* y, int ld, int init,
+                      hipStream_t s) {
+  const int64_t nb = ceil_div(A->nrows, 256 / KL);
+  if (nb == 0) return MPH_OK;
+  if constexpr (DIA) {
+    DiaArgs<V> a{};
+    a.nrows = A->nrows;
+    a.ncols = A->ncols;
+    a.ndiag = (int32_t)A->ndiag;
+    a.pitch = A->dia_pitch;
+    a.off = A->dia_off;
+    a.val = static_cast<const V*>(A->dia_v);
+    a.x = static_cast<const typename vtraits<V>::vec*>(x);
+    a.y = static_cast<typename vtraits<V>::vec*>(y);
+    a.ld = ld;
+    a.init = init;
+    hipLaunchKernelGGL((dia_spmm_kernel<V, KL>), dim3((unsigned)nb), dim3(256), 0, s, a);
+  } else {
+    EllArgs<V> a{};
+    a.nrows = A->nrows;
+    a.loop = (int32_t)A->ell_loop;
+    a.pitch = A->ell_pitch;
+    a.ci = A->ell_ci;
+    a.val = static_cast<const V*>(A->ell_v);
+    a.x = static_cast<const typename vtraits<V>::vec*>(x);
+    a.y = static_cast<typename vtraits<V>::vec*>(y);
+    a.ld = ld;
+    a.init = init;
+    hipLaunchKernelGGL((ell_spmm_kernel<V, KL>), dim3((unsigned)nb), dim3(256), 0, s, a);
+  }
+  MPH_HIP_CHECK(hipGetLastError());
+  return MPH_OK;
+}
+
 template <typename V, bool DIA, int RPT>
 int launch_fmt(mph_matrix_impl* A, int k, const void* x, void* y, int init,
                hipStream_t s) {
   using Vec = typename vtraits<V>::vec;
+  constexpr int Q = Piece<Vec>::Q;
+  // lane groups when every row of X / Y starts on 16 bytes
+  const bool pieces = k > 1 && (k * sizeof(Vec)) % 16 == 0 &&
+                      reinterpret_cast<uintptr_t>(x) % 16 == 0 &&
+                      reinterpret_cast<uintptr_t>(y) % 16 == 0;
   int done = 0;
   while (done < k) {
     const int rem = k - done;
     const Vec* xs = static_cast<const Vec*>(x) + done;
     Vec* ys = static_cast<Vec*>(y) + done;
+    if (pieces && rem >= Q) {
+      const int kl = rem >= 16 * Q ? 16 : rem >= 8 * Q ? 8 : rem >= 4 * Q ? 4 : rem >= 2 * Q ? 2 : 1;
+      int st;
+      switch (kl) {
+        case 16: st = launch_spmm_lanes<V, DIA, 16>(A, xs, ys, k, init, s); break;
+        case 8: st = launch_spmm_lanes<V, DIA, 8>(A, xs, ys, k, init, s); break;
+        case 4: st = launch_spmm_lanes<V, DIA, 4>(A, xs, ys, k, init, s); break;
+        case 2: st = launch_spmm_lanes<V, DIA, 2>(A, xs, ys, k, init, s); break;
+        default: st = launch_spmm_lanes<V, DIA, 1>(A, xs, ys, k, init, s); break;
+      }
+      if (st != MPH_OK) return st;
+      done += kl * Q;
+      continue;
+    }
     int kr = rem >= 16 ? 16 : rem >= 8 ? 8 : rem >= 4 ? 4 : rem >= 2 ? 2 : 1;
     if (k == 1) kr = 1;
     int st;

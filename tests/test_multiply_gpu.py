@@ -528,6 +528,55 @@ def test_spmm_matches_columnwise(gpu, name, k):
     assert np.all(np.abs(Y - ref) <= bound)
 
 
+@pytest.mark.parametrize("name", ["DIA", "ELL"])
+@pytest.mark.parametrize("dtype,k", [("f64", 2), ("f64", 6), ("f64", 16), ("f64", 40),
+                                     ("f32", 4), ("f32", 12), ("f32", 20), ("bf16", 8),
+                                     ("f32", 5)])
+@pytest.mark.parametrize("init", [True, False])
+def test_spmm_lane_groups(gpu, name, dtype, k, init):
+    """DIA / ELL SpMM whose X / Y rows start on 16 bytes runs lane groups
+    (16 bytes of a row per lane; column passes of up to 32 FP64 / 64 FP32
+    columns, the rest in narrower passes); FP32 k = 5 keeps one thread per
+    row.  FP64: each column equals the single-vector product bit for bit
+    (same summation order); FP32 / BF16 within the tolerance of the
+    compensated FP64 oracle; init = false accumulates into Y."""
+    torch = _torch()
+    from morpheus_amd import device
+    A = gen.stencil3d_7(24) if name == "DIA" else gen.banded(20000, 9, 0.02, seed=4)
+    Ah = as_format(A, name)
+    rng = np.random.default_rng(k)
+    X = rng.uniform(-1, 1, (A.ncols, k))
+    Y0 = rng.uniform(-1, 1, (A.nrows, k))
+    D = device.to_device(Ah, dtype)
+    vdt = device.vector_dtype(dtype)
+    Xt = torch.from_numpy(X).cuda().to(vdt)
+    Yt = torch.from_numpy(Y0).cuda().to(vdt)
+    D.multiply_multi(Xt, Yt, init)
+    cols = []
+    for j in range(k):
+        yj = torch.from_numpy(Y0[:, j].copy()).cuda().to(vdt)
+        D.multiply(Xt[:, j].contiguous(), yj, init)
+        cols.append(yj)
+    torch.cuda.synchronize()
+    Y = Yt.cpu().numpy().astype(np.float64)
+    D.close()
+    if dtype == "f64":
+        Yc = torch.stack(cols, 1).cpu().numpy()
+        assert np.array_equal(Y.view(np.uint64), Yc.view(np.uint64))
+        nrow = np.diff(A.row_offsets)
+        for j in range(k):
+            ref = oracle.spmv(A, X[:, j], Y0[:, j].copy(), init)
+            bound = fp64_bound(nrow, oracle.absrow(A, X[:, j]), None if init else Y0[:, j])
+            assert np.all(np.abs(Y[:, j] - ref) <= bound)
+    else:
+        tol = 1e-5 if dtype == "f32" else 1e-2
+        y0c = Yt.new_tensor(Y0).cpu().numpy().astype(np.float64)  # Y0 as stored
+        for j in range(k):
+            ref = oracle.spmv_compensated(A, X[:, j]) + (0 if init else y0c[:, j])
+            scale = oracle.absrow(A, X[:, j]) + (0 if init else np.abs(y0c[:, j]))
+            assert max_rel_err(Y[:, j], ref, scale) <= tol
+
+
 def test_one_shot_entry_points(gpu):
     torch = _torch()
     import ctypes as C
