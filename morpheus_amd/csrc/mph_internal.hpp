@@ -38,7 +38,13 @@ constexpr int kXTileBytes = 32 << 10;          // x tile in LDS
 constexpr int kXTileCap = 512;                 // items of a (one-wave) sub-block
 constexpr int kXTileRows = 64;                 // rows of a sub-block
 constexpr int kXTileMaxGroupRows = 1024;       // rows of a group (run offsets in LDS)
-constexpr int64_t kXTileMaxBounds = 1 << 26;  // entries of the per-row tile-bound table for a halved tile
+// SpMM with x-tiled rows: the x-tiled pass runs once per column for k up to
+// kXTileSpmmMaxK, and for k up to kXTileSpmmLongRunMaxK when the tiled rows
+// average at least kXTileSpmmLongRun items per (row, tile) run
+constexpr int kXTileSpmmMaxK = 2;
+constexpr int kXTileSpmmLongRunMaxK = 4;
+constexpr int64_t kXTileSpmmLongRun = 32;
+constexpr int64_t kXTileMaxBounds = 1 << 26;  // entries of the per-row tile-bound table (rows x (tiles + 1))
 constexpr int64_t kXTileMinXBytes = 2 << 20;  // auto: x larger than this
 constexpr int64_t kXTileMinRowLen = 512;       // rows longer than this are candidates
 // HYB / HDC with at least this many rows run the ELL / DIA part as its own
@@ -57,6 +63,7 @@ struct XTilePlan {
   int32_t groups = 0;
   int32_t gr = 0;                // long rows per group
   int32_t rows = 0;              // tiled long rows (dealt to groups by length rank)
+  int64_t items = 0;             // items of the tiled rows
   int32_t* row = nullptr;        // [rows] row ids
   int32_t* lro = nullptr;        // [(groups*ntiles) * (gr+1)] item offsets of each row's run
   XTSub* sub = nullptr;          // sub-blocks, a sentinel after each (group, tile)

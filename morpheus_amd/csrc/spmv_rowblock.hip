@@ -762,6 +762,7 @@ __global__ __launch_bounds__(kRowBlockThreads) void rowblock_spmm_kernel(
   __shared__ Acc s_red[KR * kRowBlockThreads / kWave];
   const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
   const RowBlock blk = a.blocks[b];
+  if (blk.sub < -1) return;  // a long row handled by the x-tiled pass
   if (blk.sub >= 0) {
     const int64_t r = blk.row;
     const int64_t rs = a.ro[r], re = a.ro[r + 1];
@@ -1095,6 +1096,14 @@ int xtile_build(mph_matrix_impl* A, const std::vector<int32_t>& ro, const int32_
   std::stable_sort(lrows.begin(), lrows.end(), [&](int32_t u, int32_t v) {
     return ro[u + 1] - ro[u] > ro[v + 1] - ro[v];
   });
+  // the per-row tile-bound table and the (tile, row) partials grow with
+  // rows x tiles: past kXTileMaxBounds entries only the longest rows are
+  // planned, the others keep the position chunks
+  {
+    const int64_t nt0 = ceil_div((int64_t)A->ncols, (int64_t)(kXTileBytes / vec_bytes));
+    const int64_t cap = std::max<int64_t>(1, kXTileMaxBounds / (nt0 + 1));
+    if ((int64_t)lrows.size() > cap) lrows.resize((size_t)cap);
+  }
   const int32_t nl = (int32_t)lrows.size();
   const int32_t tile_max = (int32_t)(kXTileBytes / vec_bytes);
   int32_t tile = tile_max, ntiles = 0;
@@ -1378,6 +1387,7 @@ int xtile_build(mph_matrix_impl* A, const std::vector<int32_t>& ro, const int32_
   xp.groups = ngroups;
   xp.gr = gr;
   xp.rows = nt;
+  xp.items = total;
   return MPH_OK;
 }
 
@@ -1880,10 +1890,44 @@ int launch_spmm_k(mph_matrix_impl* A, const void* x, void* y, int ld, int init,
   return MPH_OK;
 }
 
+// The split plan's short blocks and untiled long-row chunks (tiled rows
+// skipped) with KR right-hand sides.
+template <typename V, int KR>
+int launch_split_spmm_k(mph_matrix_impl* A, const void* x, void* y, int ld, int init,
+                        hipStream_t s) {
+  RBArgs<V> a = make_args<V>(A, x, y, ld, init);
+  const RowBlockPlan& p = A->rb;
+  a.blocks = p.sblocks;
+  a.nblocks = p.nsblocks;
+  a.krec = p.skrec;
+  if (p.nsblocks > 0)
+    hipLaunchKernelGGL((rowblock_spmm_kernel<V, PRE_NONE, KR>), dim3((unsigned)p.nsblocks),
+                       dim3(kRowBlockThreads), 0, s, a);
+  if (p.nchunked > 0)
+    hipLaunchKernelGGL((rowblock_spmm_combine_kernel<V, PRE_NONE, KR>),
+                       dim3((unsigned)ceil_div(p.nsblocks, 256)), dim3(256), 0, s, a);
+  MPH_HIP_CHECK(hipGetLastError());
+  return MPH_OK;
+}
+
 template <typename V, int PRE>
 int launch_spmm(mph_matrix_impl* A, int k, const void* x, void* y, int init,
                 hipStream_t s) {
   using Vec = typename vtraits<V>::vec;
+  // A CSR-shaped matrix with x-tiled rows and few right-hand sides: the
+  // x-tiled pass runs once per column (x and y with stride k), reading the
+  // tile-major copy k times (10 B per item in FP64) instead of gathering a
+  // random k-wide row of X per item, which costs a 64-128 B line from
+  // beyond L2 once X (k times x) outgrows the caches; the other rows take
+  // the k-wide kernel over the split plan.  Measured: on a 2M-row
+  // power-law(1.2) matrix (runs of about 43 items) k = 2 / 4 run 2.1 / 1.4x
+  // faster and k = 8 1.2x slower; on a 1M-row power-law(1.5) matrix (runs of
+  // about 13) k = 2 1.1x faster and k = 4 1.1x slower.
+  const XTilePlan& xp = A->rb.xt;
+  const bool long_runs =
+      xp.rows > 0 && xp.items >= kXTileSpmmLongRun * (int64_t)xp.rows * xp.ntiles;
+  const bool xt_cols = PRE == PRE_NONE && A->rb.sblocks && !A->rb.split && xp.rows > 0 &&
+                       (k <= kXTileSpmmMaxK || (long_runs && k <= kXTileSpmmLongRunMaxK));
   // split k into column groups of 16, 8, 4, 2, 1 sharing the leading dim k
   int done = 0;
   while (done < k) {
@@ -1892,23 +1936,32 @@ int launch_spmm(mph_matrix_impl* A, int k, const void* x, void* y, int init,
     Vec* ys = static_cast<Vec*>(y) + done;
     int st;
     if (rem >= 16) {
-      st = launch_spmm_k<V, PRE, 16>(A, xs, ys, k, init, s);
+      st = xt_cols ? launch_split_spmm_k<V, 16>(A, xs, ys, k, init, s)
+                   : launch_spmm_k<V, PRE, 16>(A, xs, ys, k, init, s);
       done += 16;
     } else if (rem >= 8) {
-      st = launch_spmm_k<V, PRE, 8>(A, xs, ys, k, init, s);
+      st = xt_cols ? launch_split_spmm_k<V, 8>(A, xs, ys, k, init, s)
+                   : launch_spmm_k<V, PRE, 8>(A, xs, ys, k, init, s);
       done += 8;
     } else if (rem >= 4) {
-      st = launch_spmm_k<V, PRE, 4>(A, xs, ys, k, init, s);
+      st = xt_cols ? launch_split_spmm_k<V, 4>(A, xs, ys, k, init, s)
+                   : launch_spmm_k<V, PRE, 4>(A, xs, ys, k, init, s);
       done += 4;
     } else if (rem >= 2) {
-      st = launch_spmm_k<V, PRE, 2>(A, xs, ys, k, init, s);
+      st = xt_cols ? launch_split_spmm_k<V, 2>(A, xs, ys, k, init, s)
+                   : launch_spmm_k<V, PRE, 2>(A, xs, ys, k, init, s);
       done += 2;
     } else {
-      st = launch_spmm_k<V, PRE, 1>(A, xs, ys, k, init, s);
+      st = xt_cols ? launch_split_spmm_k<V, 1>(A, xs, ys, k, init, s)
+                   : launch_spmm_k<V, PRE, 1>(A, xs, ys, k, init, s);
       done += 1;
     }
     if (st != MPH_OK) return st;
   }
+  if (xt_cols)
+    for (int j = 0; j < k; j++)
+      MPH_CHECK_STATUS(xtile_multiply(A, A->rb.xt, static_cast<const Vec*>(x) + j,
+                                      static_cast<Vec*>(y) + j, init, k, s));
   return MPH_OK;
 }
 

@@ -254,6 +254,21 @@ def test_xtile_dense_rows_halve_the_tile(gpu, xtile):
         assert max_rel_err(y, ref, oracle.absrow(A, x)) <= tol
 
 
+def test_xtile_table_cap(gpu):
+    """32M columns (8,192 FP64 tiles) and 9,000 rows longer than 512: the
+    per-row tile-bound table would exceed its cap (2^26 entries), so only
+    the 8,191 longest rows are x-tiled and the rest keep the position
+    chunks."""
+    rng = np.random.default_rng(17)
+    m, n = 32 * 1024 * 1024, 9000
+    lens = rng.integers(513, 700, n)
+    ro = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    ci = np.concatenate([np.sort(rng.choice(m, L, replace=False)) for L in lens]).astype(np.int32)
+    A = Csr(n, m, ro, ci, rng.standard_normal(int(ro[-1])))
+    x = gen.random_vector(m, 18)
+    check_fp64(A, A, x)
+
+
 def test_xtile_unsorted_columns(gpu):
     """Long rows whose columns are not sorted keep the position chunks."""
     A = gen.powerlaw_rows(4000, 300000, 1.5, 30.0, 6000, seed=8)
@@ -270,25 +285,46 @@ def test_xtile_unsorted_columns(gpu):
     check_fp64(U, U, x, tuning=dict(xtile=2))
 
 
-def test_xtile_plan_keeps_spmm(gpu, wide):
-    """k > 1 products keep the position-chunk plan of the same handle."""
+@pytest.mark.parametrize("name", ["CSR", "COO", "HYB"])
+@pytest.mark.parametrize("k", [2, 3, 8, 12])
+@pytest.mark.parametrize("init", [True, False])
+def test_xtile_plan_spmm(gpu, wide, name, k, init):
+    """SpMM on a handle with x-tiled rows: up to 8 right-hand sides the
+    x-tiled pass runs once per column (its columns equal the single-vector
+    product of the tiled rows bit for bit) and the other rows take the k-wide
+    kernel over the split plan; 12 keeps the position chunks.  A k = 1
+    product on the same handle before and after.  FP64 within the per-row
+    bound, FP32 within 1e-5."""
     torch = _torch()
     from morpheus_amd import device
     A = wide
-    k = 3
     X = np.stack([gen.random_vector(A.ncols, 10 + j) for j in range(k)], axis=1)
-    D = device.to_device(A, "f64")
-    Xt = torch.from_numpy(np.ascontiguousarray(X)).cuda()
-    Y = torch.full((A.nrows, k), float("nan"), dtype=torch.float64, device="cuda")
-    y1 = torch.full((A.nrows,), float("nan"), dtype=torch.float64, device="cuda")
-    D.multiply(Xt[:, 0].contiguous(), y1)
-    D.multiply_multi(Xt, Y)
-    torch.cuda.synchronize()
-    for j in range(k):
-        ref = oracle.spmv(A, X[:, j])
-        bound = fp64_bound(np.diff(A.row_offsets), oracle.absrow(A, X[:, j]))
-        assert np.all(np.abs(Y[:, j].cpu().numpy() - ref) <= bound)
-    D.close()
+    Y0 = np.stack([gen.random_vector(A.nrows, 40 + j) for j in range(k)], axis=1)
+    nrow = np.diff(A.row_offsets)
+    for dtype in ("f64", "f32"):
+        D = device.to_device(as_format(A, name), dtype)
+        vdt = device.vector_dtype(dtype)
+        Xt = torch.from_numpy(np.ascontiguousarray(X)).cuda().to(vdt)
+        Y = torch.from_numpy(np.ascontiguousarray(Y0)).cuda().to(vdt)
+        y1 = torch.full((A.nrows,), float("nan"), dtype=vdt, device="cuda")
+        D.multiply(Xt[:, 0].contiguous(), y1)
+        D.multiply_multi(Xt, Y, init)
+        y2 = torch.full((A.nrows,), float("nan"), dtype=vdt, device="cuda")
+        D.multiply(Xt[:, 0].contiguous(), y2)
+        torch.cuda.synchronize()
+        assert torch.equal(y1, y2)
+        Yh = Y.double().cpu().numpy()
+        y0s = Y.new_tensor(Y0).double().cpu().numpy()  # Y0 as stored
+        for j in range(k):
+            if dtype == "f64":
+                ref = oracle.spmv(A, X[:, j], Y0[:, j].copy(), init)
+                bound = fp64_bound(nrow, oracle.absrow(A, X[:, j]), None if init else Y0[:, j])
+                assert np.all(np.abs(Yh[:, j] - ref) <= bound), f"column {j}"
+            else:
+                ref = oracle.spmv_compensated(A, X[:, j]) + (0 if init else y0s[:, j])
+                scale = oracle.absrow(A, X[:, j]) + (0 if init else np.abs(y0s[:, j]))
+                assert max_rel_err(Yh[:, j], ref, scale) <= 1e-5, f"column {j}"
+        D.close()
 
 
 # ------------------------------------------------------------ split two-pass HYB / HDC
