@@ -144,7 +144,11 @@ int mph_matrix_format(mph_matrix_t A);
  *                  2 = on; 0 = on when x exceeds 2 MiB).  Rows longer than
  *                  512 entries with sorted columns are summed per 32 KiB tile
  *                  of x staged in LDS, so their x reads are LDS reads instead
- *                  of scattered cache-line fetches.  On (x-tiling) or from
+ *                  of scattered cache-line fetches (the tile is halved, down
+ *                  to an eighth, when most of those rows are too dense for
+ *                  it; SpMM runs the pass once per column for k <= 2, or
+ *                  k <= 4 when the rows average 32+ items per tile).  On
+ *                  (x-tiling) or from
  *                  65,536 rows, HYB / HDC run their ELL / DIA part as its own
  *                  pass and the COO / CSR part over the rows that have one;
  *                  a HYB whose ELL slots are less than a third filled, or
