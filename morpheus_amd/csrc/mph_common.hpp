@@ -3,6 +3,7 @@
 // streaming loads and error plumbing.
 #pragma once
 
+#include <type_traits>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -68,6 +69,24 @@ __device__ __forceinline__ bf16 ld_stream<bf16>(const bf16* p) {
   bf16 r;
   r.bits = p->bits;
   return r;
+}
+
+// BF16 values loaded in a batch are kept raw until their products: widened
+// right after its load, each value made the compiler wait for that load
+// before issuing the next one (two loads in flight instead of the batch).
+// Called after the batch's remaining loads (the x gathers) are issued, the
+// empty asm statement, which takes each raw word as an in-out operand, keeps
+// the widening below it.  No-op for FP64 / FP32.
+template <typename V, int N>
+__device__ __forceinline__ void hold_raw(V (&v)[N]) {
+  if constexpr (std::is_same<V, bf16>::value) {
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      uint32_t w = v[k].bits;
+      asm volatile("" : "+v"(w));
+      v[k].bits = (uint16_t)w;
+    }
+  }
 }
 
 // Two adjacent FP32 / BF16 values loaded as one word (8 B / 4 B) and split.

@@ -45,7 +45,7 @@ __global__ __launch_bounds__(256) void dia_kernel(DiaArgs<V> a) {
   // batches of 8 diagonals: offsets, values and x entries of the batch are
   // all requested before any of them is consumed
   for (int d0 = 0; d0 < a.ndiag; d0 += 8) {
-    Acc v[RPT][8];
+    V v[RPT][8];
     int64_t c[RPT][8];
 #pragma unroll
     for (int w = 0; w < 8; w++) {
@@ -57,16 +57,19 @@ __global__ __launch_bounds__(256) void dia_kernel(DiaArgs<V> a) {
         const int64_t col = row + o;
         const bool ok = d < a.ndiag && row < a.nrows && col >= 0 && col < a.ncols;
         c[u][w] = ok ? col : -1;
-        v[u][w] = ok ? vtraits<V>::widen(ld_stream(a.val + a.pitch * d + row)) : (Acc)0;
+        v[u][w] = ok ? ld_stream(a.val + a.pitch * d + row) : V{};
       }
     }
+#pragma unroll
+    for (int u = 0; u < RPT; u++) hold_raw(v[u]);
 #pragma unroll
     for (int w = 0; w < 8; w++)
 #pragma unroll
       for (int u = 0; u < RPT; u++)
         if (c[u][w] >= 0)
 #pragma unroll
-          for (int q = 0; q < KR; q++) t[u][q] += v[u][w] * (Acc)a.x[c[u][w] * a.ld + q];
+          for (int q = 0; q < KR; q++)
+            t[u][q] += vtraits<V>::widen(v[u][w]) * (Acc)a.x[c[u][w] * a.ld + q];
   }
 #pragma unroll
   for (int u = 0; u < RPT; u++) {
@@ -116,21 +119,24 @@ __global__ __launch_bounds__(256) void ell_kernel(EllArgs<V> a) {
         const int n = n0 + w;
         c[u][w] = (n < a.loop && row < a.nrows) ? (int64_t)ld_stream(a.ci + row + a.pitch * n) : -1;
       }
-    Acc v[RPT][8];
+    V v[RPT][8];
 #pragma unroll
     for (int w = 0; w < 8; w++)
 #pragma unroll
       for (int u = 0; u < RPT; u++) {
         const int64_t row = row0 + u * 256;
-        v[u][w] = c[u][w] >= 0 ? vtraits<V>::widen(ld_stream(a.val + row + a.pitch * (n0 + w))) : (Acc)0;
+        v[u][w] = c[u][w] >= 0 ? ld_stream(a.val + row + a.pitch * (n0 + w)) : V{};
       }
+#pragma unroll
+    for (int u = 0; u < RPT; u++) hold_raw(v[u]);
 #pragma unroll
     for (int w = 0; w < 8; w++)
 #pragma unroll
       for (int u = 0; u < RPT; u++)
         if (c[u][w] >= 0)
 #pragma unroll
-          for (int q = 0; q < KR; q++) t[u][q] += v[u][w] * (Acc)a.x[c[u][w] * a.ld + q];
+          for (int q = 0; q < KR; q++)
+            t[u][q] += vtraits<V>::widen(v[u][w]) * (Acc)a.x[c[u][w] * a.ld + q];
   }
 #pragma unroll
   for (int u = 0; u < RPT; u++) {
@@ -182,7 +188,7 @@ __global__ __launch_bounds__(256) void dia_spmm_kernel(DiaArgs<V> a) {
 #pragma unroll
   for (int q = 0; q < Q; q++) t[q] = 0;
   for (int d0 = 0; d0 < a.ndiag; d0 += 8) {
-    Acc v[8];
+    V v[8];
     int64_t c[8];
 #pragma unroll
     for (int w = 0; w < 8; w++) {
@@ -191,18 +197,21 @@ __global__ __launch_bounds__(256) void dia_spmm_kernel(DiaArgs<V> a) {
       const int64_t col = row + o;
       const bool ok = d < a.ndiag && col >= 0 && col < a.ncols;
       c[w] = ok ? col : -1;
-      v[w] = ok ? vtraits<V>::widen(ld_stream(a.val + a.pitch * d + row)) : (Acc)0;
+      v[w] = ok ? ld_stream(a.val + a.pitch * d + row) : V{};
     }
     typename P::T xv[8];
 #pragma unroll
     for (int w = 0; w < 8; w++)
       xv[w] = c[w] >= 0 ? *reinterpret_cast<const typename P::T*>(a.x + c[w] * a.ld + l * Q)
                         : (typename P::T)0;
+    hold_raw(v);
 #pragma unroll
     for (int w = 0; w < 8; w++)
-      if (c[w] >= 0)
+      if (c[w] >= 0) {
+        const Acc vw = vtraits<V>::widen(v[w]);
 #pragma unroll
-        for (int q = 0; q < Q; q++) t[q] += v[w] * (Acc)xv[w][q];
+        for (int q = 0; q < Q; q++) t[q] += vw * (Acc)xv[w][q];
+      }
   }
   typename P::T* yp = reinterpret_cast<typename P::T*>(a.y + row * a.ld + l * Q);
   typename P::T out;
@@ -235,20 +244,23 @@ __global__ __launch_bounds__(256) void ell_spmm_kernel(EllArgs<V> a) {
 #pragma unroll
     for (int w = 0; w < 8; w++)
       c[w] = n0 + w < a.loop ? (int64_t)ld_stream(a.ci + row + a.pitch * (n0 + w)) : -1;
-    Acc v[8];
+    V v[8];
 #pragma unroll
     for (int w = 0; w < 8; w++)
-      v[w] = c[w] >= 0 ? vtraits<V>::widen(ld_stream(a.val + row + a.pitch * (n0 + w))) : (Acc)0;
+      v[w] = c[w] >= 0 ? ld_stream(a.val + row + a.pitch * (n0 + w)) : V{};
     typename P::T xv[8];
 #pragma unroll
     for (int w = 0; w < 8; w++)
       xv[w] = c[w] >= 0 ? *reinterpret_cast<const typename P::T*>(a.x + c[w] * a.ld + l * Q)
                         : (typename P::T)0;
+    hold_raw(v);
 #pragma unroll
     for (int w = 0; w < 8; w++)
-      if (c[w] >= 0)
+      if (c[w] >= 0) {
+        const Acc vw = vtraits<V>::widen(v[w]);
 #pragma unroll
-        for (int q = 0; q < Q; q++) t[q] += v[w] * (Acc)xv[w][q];
+        for (int q = 0; q < Q; q++) t[q] += vw * (Acc)xv[w][q];
+      }
   }
   typename P::T* yp = reinterpret_cast<typename P::T*>(a.y + row * a.ld + l * Q);
   typename P::T out;

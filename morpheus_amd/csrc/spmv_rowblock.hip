@@ -193,12 +193,12 @@ __device__ __forceinline__ void batched_products(const int32_t* __restrict__ ci,
   for (int64_t base = begin + threadIdx.x; base < end;
        base += (int64_t)kRowBlockThreads * kBatch) {
     int32_t c[kBatch];
-    Acc v[kBatch];
+    V v[kBatch];
 #pragma unroll
     for (int k = 0; k < kBatch; k++) {
       const int64_t i = base + (int64_t)k * kRowBlockThreads;
       c[k] = i < end ? ld_stream(ci + i) : 0;
-      v[k] = i < end ? vtraits<V>::widen(ld_stream(val + i)) : (Acc)0;
+      v[k] = i < end ? ld_stream(val + i) : V{};
     }
     Acc xv[kBatch];
 #pragma unroll
@@ -206,10 +206,11 @@ __device__ __forceinline__ void batched_products(const int32_t* __restrict__ ci,
       const int64_t i = base + (int64_t)k * kRowBlockThreads;
       xv[k] = i < end ? (Acc)x[c[k]] : (Acc)0;
     }
+    hold_raw(v);  // BF16: widened below the gathers
 #pragma unroll
     for (int k = 0; k < kBatch; k++) {
       const int64_t i = base + (int64_t)k * kRowBlockThreads;
-      if (i < end) sink(i, v[k] * xv[k]);
+      if (i < end) sink(i, vtraits<V>::widen(v[k]) * xv[k]);
     }
   }
 }
@@ -403,10 +404,26 @@ __device__ __forceinline__ XTSub uniform_sub(const XTSub& v) {
                __builtin_amdgcn_readfirstlane(v.p0), 0};
 }
 
+// Loaded values as held until their products: FP64 / FP32 as accumulated,
+// BF16 as its raw bits (widened at the product; widened at the load, the
+// compiler waited for each value before issuing the next load).
+template <typename V>
+struct XTHeld {
+  using T = typename vtraits<V>::acc;
+  __device__ __forceinline__ static T hold(V v) { return vtraits<V>::widen(v); }
+  __device__ __forceinline__ static typename vtraits<V>::acc use(T t) { return t; }
+};
+template <>
+struct XTHeld<bf16> {
+  using T = uint32_t;
+  __device__ __forceinline__ static T hold(bf16 v) { return v.bits; }
+  __device__ __forceinline__ static float use(T t) { return __uint_as_float(t << 16); }
+};
+
 template <typename V>
 struct XTLoad {
   uint32_t c[kBatch];
-  typename vtraits<V>::acc v[kBatch];
+  typename XTHeld<V>::T v[kBatch];
   uint32_t h;  // run of the lane's first item
 };
 
@@ -421,7 +438,7 @@ __device__ __forceinline__ void xtile_load(const XTArgs& t, const V* cval, const
     const int32_t j = jb + q;
     const int64_t slot = (int64_t)s0.p0 + (int64_t)q * S + lane;
     d.c[q] = j < n ? (uint32_t)ld_stream(t.col + slot) : 0u;
-    d.v[q] = j < n ? vtraits<V>::widen(ld_stream(cval + slot)) : (typename vtraits<V>::acc)0;
+    d.v[q] = j < n ? XTHeld<V>::hold(ld_stream(cval + slot)) : (typename XTHeld<V>::T)0;
   }
 }
 
@@ -534,7 +551,7 @@ __global__ __launch_bounds__(kXTileThreads) void xtile_kernel(RBArgs<V> a, XTArg
     for (int b = 0; b < kBatch; b++)
       // the copy holds columns inside the tile; the clamp keeps a corrupted
       // entry inside the LDS tile
-      p[b] = cur.v[b] * (Acc)xs[min(cur.c[b], (uint32_t)(nx - 1))];
+      p[b] = XTHeld<V>::use(cur.v[b]) * (Acc)xs[min(cur.c[b], (uint32_t)(nx - 1))];
     const int32_t hd = (int32_t)min(cur.h, (uint32_t)(nr - 1));
     auto put = [&](int32_t r, const S& v) {
       const int64_t o = (int64_t)tile * t.rows + i0 + r;
