@@ -110,14 +110,17 @@ struct Unroll {
   static constexpr int value = KR >= 8 ? 2 : (KR >= 4 ? 4 : 8);
 };
 
+// (values arrive in their stored type; BF16 is widened here, after its
+// loads have all been issued)
 template <typename V, int KR>
 __device__ __forceinline__ void consume(typename vtraits<V>::acc* acc,
-                                        const typename vtraits<V>::acc* v, const int64_t* c,
+                                        V (&v)[Unroll<KR>::value], const int64_t* c,
                                         const typename vtraits<V>::vec* __restrict__ x) {
   constexpr int kU = Unroll<KR>::value;
+  hold_raw(v);
 #pragma unroll
   for (int u = 0; u < kU; u++)
-    if (c[u] >= 0) XRow<V, KR>::fma(acc, v[u], x, c[u]);
+    if (c[u] >= 0) XRow<V, KR>::fma(acc, vtraits<V>::widen(v[u]), x, c[u]);
 }
 
 // DIA part of row r over diagonals d = d0, d0 + step, ...
@@ -130,18 +133,18 @@ __device__ __forceinline__ void dia_row(const BatchMat& m, int64_t r, int d0, in
   const V* dv = static_cast<const V*>(m.dval);
   const Vec* x = static_cast<const Vec*>(m.x);
   for (int d = d0; d < m.ndiag; d += kU * step) {
-    Acc v[kU];
+    V v[kU];
     int64_t c[kU];
 #pragma unroll
     for (int u = 0; u < kU; u++) {
       const int dd = d + u * step;
       c[u] = -1;
-      v[u] = 0;
+      v[u] = V{};
       if (dd < m.ndiag) {
         const int64_t col = r + m.doff[dd];
         if (col >= 0 && col < m.ncols) {
           c[u] = col;
-          v[u] = vtraits<V>::widen(ld_stream(dv + r + m.dpitch * dd));
+          v[u] = ld_stream(dv + r + m.dpitch * dd);
         }
       }
     }
@@ -165,11 +168,11 @@ __device__ __forceinline__ void ell_row(const BatchMat& m, int64_t r, int n0, in
       const int nn = n + u * step;
       c[u] = nn < m.eloop ? (int64_t)ld_stream(m.eci + r + m.epitch * nn) : -1;
     }
-    Acc v[kU];
+    V v[kU];
 #pragma unroll
     for (int u = 0; u < kU; u++) {
       const int nn = n + u * step;
-      v[u] = c[u] >= 0 ? vtraits<V>::widen(ld_stream(ev + r + m.epitch * nn)) : (Acc)0;
+      v[u] = c[u] >= 0 ? ld_stream(ev + r + m.epitch * nn) : V{};
     }
     consume<V, KR>(acc, v, c, x);
   }
@@ -188,12 +191,12 @@ __device__ void row_items(const BatchMat& m, int64_t r, int lane, int L,
   const int64_t st = m.ro[r], en = m.ro[r + 1];
   for (int64_t j = st + lane; j < en; j += (int64_t)kU * L) {
     int64_t c[kU];
-    Acc v[kU];
+    V v[kU];
 #pragma unroll
     for (int u = 0; u < kU; u++) {
       const int64_t jj = j + (int64_t)u * L;
       c[u] = jj < en ? (int64_t)ld_stream(m.ci + jj) : -1;
-      v[u] = jj < en ? vtraits<V>::widen(ld_stream(val + jj)) : (Acc)0;
+      v[u] = jj < en ? ld_stream(val + jj) : V{};
     }
     consume<V, KR>(acc, v, c, x);
   }
