@@ -183,19 +183,30 @@ __global__ __launch_bounds__(256) void dia_spmm_kernel(DiaArgs<V> a) {
   const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
   const int64_t row = (int64_t)b * (256 / KL) + threadIdx.x / KL;
   const int l = threadIdx.x % KL;
-  if (row >= a.nrows) return;
+  // rows past the end stay in the wave (their lanes may hold the batch's
+  // offsets read back below) and do no work
+  const bool live = row < a.nrows;
   Acc t[Q];
 #pragma unroll
   for (int q = 0; q < Q; q++) t[q] = 0;
   for (int d0 = 0; d0 < a.ndiag; d0 += 8) {
+    // the batch's eight offsets in one round trip: lane w loads offset
+    // d0 + w and the wave reads them back lane by lane (left to the
+    // compiler, the BF16 kernel loaded them one at a time, each waited on
+    // before the next)
+    const int lane = threadIdx.x % kWave;
+    const int32_t mo = lane < 8 && d0 + lane < a.ndiag ? a.off[d0 + lane] : 0;
+    int32_t o8[8];
+#pragma unroll
+    for (int w = 0; w < 8; w++) o8[w] = __builtin_amdgcn_readlane(mo, w);
     V v[8];
     int64_t c[8];
 #pragma unroll
     for (int w = 0; w < 8; w++) {
       const int d = d0 + w;
-      const int64_t o = d < a.ndiag ? (int64_t)a.off[d] : 0;
+      const int64_t o = o8[w];
       const int64_t col = row + o;
-      const bool ok = d < a.ndiag && col >= 0 && col < a.ncols;
+      const bool ok = live && d < a.ndiag && col >= 0 && col < a.ncols;
       c[w] = ok ? col : -1;
       v[w] = ok ? ld_stream(a.val + a.pitch * d + row) : V{};
     }
@@ -213,6 +224,7 @@ __global__ __launch_bounds__(256) void dia_spmm_kernel(DiaArgs<V> a) {
         for (int q = 0; q < Q; q++) t[q] += vw * (Acc)xv[w][q];
       }
   }
+  if (!live) return;
   typename P::T* yp = reinterpret_cast<typename P::T*>(a.y + row * a.ld + l * Q);
   typename P::T out;
   if (a.init) {
