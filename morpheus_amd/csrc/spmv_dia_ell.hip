@@ -31,6 +31,51 @@ struct DiaArgs {
   int init;
 };
 
+// t[u][q] += v[u][w] * x[c[u][w]][q] over a batch of 8 diagonals / slots,
+// skipping c < 0.  The x loads are unconditional (c < 0 reads x[0]) and the
+// sums are selected rather than branched, so the batch's gathers are all in
+// flight before the first product: under a branch each gather was waited on
+// before the next was issued.  Same fused multiply-adds in the same order
+// as t += v * x.  With many right-hand sides (KR * RPT > 2) the gathers of
+// one diagonal / slot are issued together instead, to bound registers.
+template <typename V, int KR, int RPT>
+__device__ __forceinline__ void batch_fma(typename vtraits<V>::acc (&t)[RPT][KR],
+                                          const V (&v)[RPT][8], const int64_t (&c)[RPT][8],
+                                          const typename vtraits<V>::vec* __restrict__ x,
+                                          int32_t ld) {
+  using Acc = typename vtraits<V>::acc;
+  if constexpr (KR * RPT <= 2) {
+    Acc xq[RPT][8][KR];
+#pragma unroll
+    for (int w = 0; w < 8; w++)
+#pragma unroll
+      for (int u = 0; u < RPT; u++)
+#pragma unroll
+        for (int q = 0; q < KR; q++) xq[u][w][q] = (Acc)x[(c[u][w] >= 0 ? c[u][w] : 0) * ld + q];
+#pragma unroll
+    for (int w = 0; w < 8; w++)
+#pragma unroll
+      for (int u = 0; u < RPT; u++)
+#pragma unroll
+        for (int q = 0; q < KR; q++) {
+          const Acc f = __builtin_fma(vtraits<V>::widen(v[u][w]), xq[u][w][q], t[u][q]);
+          t[u][q] = c[u][w] >= 0 ? f : t[u][q];
+        }
+  } else {
+#pragma unroll
+    for (int w = 0; w < 8; w++)
+#pragma unroll
+      for (int u = 0; u < RPT; u++) {
+        const int64_t xc = c[u][w] >= 0 ? c[u][w] : 0;
+#pragma unroll
+        for (int q = 0; q < KR; q++) {
+          const Acc xq = (Acc)x[xc * ld + q];
+          if (c[u][w] >= 0) t[u][q] += vtraits<V>::widen(v[u][w]) * xq;
+        }
+      }
+  }
+}
+
 template <typename V, int KR, int RPT>
 __global__ __launch_bounds__(256) void dia_kernel(DiaArgs<V> a) {
   using Acc = typename vtraits<V>::acc;
@@ -44,6 +89,9 @@ __global__ __launch_bounds__(256) void dia_kernel(DiaArgs<V> a) {
     for (int q = 0; q < KR; q++) t[u][q] = 0;
   // batches of 8 diagonals: offsets, values and x entries of the batch are
   // all requested before any of them is consumed
+  // (The branch-free loads of ell_kernel made this kernel slower: S7
+  // 31.6 -> 34.0 us, S27 210 -> 223 us in FP64; its value loads already
+  // stream at the read rate of the device.)
   for (int d0 = 0; d0 < a.ndiag; d0 += 8) {
     V v[RPT][8];
     int64_t c[RPT][8];
@@ -87,6 +135,7 @@ __global__ __launch_bounds__(256) void dia_kernel(DiaArgs<V> a) {
 template <typename V>
 struct EllArgs {
   int64_t nrows;
+  int64_t ncols;  // set for ell_kernel (its x loads read x[0] for empty slots)
   int32_t loop;  // slots visited per row
   int64_t pitch;
   const int32_t* __restrict__ ci;
@@ -110,33 +159,31 @@ __global__ __launch_bounds__(256) void ell_kernel(EllArgs<V> a) {
     for (int q = 0; q < KR; q++) t[u][q] = 0;
   // batches of 8 slots: indices first, then values and x entries
   for (int n0 = 0; n0 < a.loop; n0 += 8) {
+    // index and value loads are unconditional (slots outside the matrix read
+    // slot 0 and are discarded): a load under a branch is waited on where
+    // the branch joins, which left one index load in flight at a time
     int64_t c[RPT][8];
+    int64_t slot[RPT][8];
 #pragma unroll
     for (int w = 0; w < 8; w++)
 #pragma unroll
       for (int u = 0; u < RPT; u++) {
         const int64_t row = row0 + u * 256;
         const int n = n0 + w;
-        c[u][w] = (n < a.loop && row < a.nrows) ? (int64_t)ld_stream(a.ci + row + a.pitch * n) : -1;
+        const bool in = n < a.loop && row < a.nrows;
+        slot[u][w] = in ? row + a.pitch * n : 0;
+        const int32_t ci = ld_stream(a.ci + slot[u][w]);
+        c[u][w] = in ? (int64_t)ci : -1;
       }
     V v[RPT][8];
 #pragma unroll
     for (int w = 0; w < 8; w++)
 #pragma unroll
-      for (int u = 0; u < RPT; u++) {
-        const int64_t row = row0 + u * 256;
-        v[u][w] = c[u][w] >= 0 ? ld_stream(a.val + row + a.pitch * (n0 + w)) : V{};
-      }
+      for (int u = 0; u < RPT; u++) v[u][w] = ld_stream(a.val + slot[u][w]);
 #pragma unroll
     for (int u = 0; u < RPT; u++) hold_raw(v[u]);
-#pragma unroll
-    for (int w = 0; w < 8; w++)
-#pragma unroll
-      for (int u = 0; u < RPT; u++)
-        if (c[u][w] >= 0)
-#pragma unroll
-          for (int q = 0; q < KR; q++)
-            t[u][q] += vtraits<V>::widen(v[u][w]) * (Acc)a.x[c[u][w] * a.ld + q];
+    // x loads unconditional as well (padding slots read x[0], unused)
+    if (a.ncols > 0) batch_fma<V, KR, RPT>(t, v, c, a.x, a.ld);
   }
 #pragma unroll
   for (int u = 0; u < RPT; u++) {
@@ -392,6 +439,7 @@ int launch_ell_k(mph_matrix_impl* A, const void* x, void* y, int ld, int init,
                  hipStream_t s) {
   EllArgs<V> a{};
   a.nrows = A->nrows;
+  a.ncols = A->ncols;
   a.loop = (int32_t)A->ell_loop;
   a.pitch = A->ell_pitch;
   a.ci = A->ell_ci;
