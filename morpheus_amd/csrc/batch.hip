@@ -188,6 +188,7 @@ __device__ void row_items(const BatchMat& m, int64_t r, int lane, int L,
   const Vec* x = static_cast<const Vec*>(m.x);
   if (m.pre == BPRE_DIA) dia_row<V, KR>(m, r, lane, L, acc);
   else if (m.pre == BPRE_ELL) ell_row<V, KR>(m, r, lane, L, acc);
+  if (!m.ro) return;  // a DIA / ELL row taken as a whole-row tile
   const int64_t st = m.ro[r], en = m.ro[r + 1];
   for (int64_t j = st + lane; j < en; j += (int64_t)kU * L) {
     int64_t c[kU];
@@ -255,6 +256,71 @@ __device__ void tile_short(const BatchMat& m, const BatchTile& t, int init,
   for (int i = 0; i < nd; i++) {
     block_row<V, KR>(m, s_def[i], init, s_red);
     __syncthreads();
+  }
+}
+
+// One row reduced by the whole workgroup as Dot2 pairs per column (FP32 /
+// BF16 rows of more than kCompensateLen items: every such row is a
+// whole-row tile): lane-strided pairs, then a fixed tree over lanes and
+// waves; the old y joins through TwoSum when accumulating.
+template <typename V, int KR>
+__device__ void block_row_comp(const BatchMat& m, int64_t r, int init, float* s_red) {
+  using Vec = typename vtraits<V>::vec;
+  csum<float> cs[KR];
+#pragma unroll
+  for (int q = 0; q < KR; q++) cs[q] = csum<float>{0, 0};
+  const Vec* x = static_cast<const Vec*>(m.x);
+  if (m.pre == BPRE_DIA) {
+    const V* dv = static_cast<const V*>(m.dval);
+    for (int d = threadIdx.x; d < m.ndiag; d += kBT) {
+      const int64_t col = r + m.doff[d];
+      if (col >= 0 && col < m.ncols) {
+        const float v = vtraits<V>::widen(ld_stream(dv + r + m.dpitch * d));
+#pragma unroll
+        for (int q = 0; q < KR; q++) cs[q].add_prod(v, (float)x[col * KR + q]);
+      }
+    }
+  } else if (m.pre == BPRE_ELL) {
+    const V* ev = static_cast<const V*>(m.evalp);
+    for (int n = threadIdx.x; n < m.eloop; n += kBT) {
+      const int32_t c = ld_stream(m.eci + r + m.epitch * n);
+      if (c >= 0) {
+        const float v = vtraits<V>::widen(ld_stream(ev + r + m.epitch * n));
+#pragma unroll
+        for (int q = 0; q < KR; q++) cs[q].add_prod(v, (float)x[(int64_t)c * KR + q]);
+      }
+    }
+  }
+  if (m.ro) {
+    const V* val = static_cast<const V*>(m.val);
+    for (int64_t j = m.ro[r] + threadIdx.x; j < m.ro[r + 1]; j += kBT) {
+      const float v = vtraits<V>::widen(ld_stream(val + j));
+      const int64_t c = ld_stream(m.ci + j);
+#pragma unroll
+      for (int q = 0; q < KR; q++) cs[q].add_prod(v, (float)x[c * KR + q]);
+    }
+  }
+  constexpr int kW = kBT / kWave;
+  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+#pragma unroll
+  for (int q = 0; q < KR; q++) {
+    for (int sh = kWave >> 1; sh > 0; sh >>= 1) {
+      const float s2 = shfl_xor(cs[q].s, sh), c2 = shfl_xor(cs[q].c, sh);
+      cs[q].combine(s2, c2);
+    }
+    if (lane == 0) {
+      s_red[2 * (q * kW + wave)] = cs[q].s;
+      s_red[2 * (q * kW + wave) + 1] = cs[q].c;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < KR) {
+    const int q = threadIdx.x;
+    csum<float> t{s_red[2 * q * kW], s_red[2 * q * kW + 1]};
+    for (int w = 1; w < kW; w++) t.combine(s_red[2 * (q * kW + w)], s_red[2 * (q * kW + w) + 1]);
+    Vec* y = static_cast<Vec*>(m.y) + r * KR + q;
+    if (!init) t.combine((float)*y, 0.0f);
+    *y = (Vec)t.value();
   }
 }
 
@@ -385,6 +451,21 @@ __global__ __launch_bounds__(kBT) void batch_spmm_kernel(const BatchMat* __restr
   }
 }
 
+// FP32 / BF16 rows of more than kCompensateLen items (whole-row tiles kept
+// out of the queue): one Dot2 row per workgroup turn, in a launch of its own
+// so that the pairs do not raise the register count of the queue kernel.
+template <typename V, int KR>
+__global__ __launch_bounds__(kBT) void batch_comp_rows_kernel(const BatchMat* __restrict__ mats,
+                                                              const BatchTile* __restrict__ tiles,
+                                                              int32_t n, int init) {
+  __shared__ float s_red[2 * KR * kBT / kWave];
+  for (int32_t i = blockIdx.x; i < n; i += gridDim.x) {
+    const BatchTile t = tiles[i];
+    block_row_comp<V, KR>(mats[t.mat], t.r0, init, s_red);
+    __syncthreads();
+  }
+}
+
 }  // namespace mph
 
 // ------------------------------------------------------------------ host
@@ -401,6 +482,9 @@ struct mph_batch_s {
   int sched = 0;                          // 0 queues (default), 1 static lists
   int32_t grab = 4;                       // tiles per counter update
   int32_t* d_counter = nullptr;
+  // FP32 / BF16 rows of more than kCompensateLen items (Dot2 kernel)
+  mph::BatchTile* d_comp = nullptr;
+  int32_t ncomp = 0;
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t exec[2] = {nullptr, nullptr};
   std::vector<void*> owned;  // row offsets derived for COO members
@@ -411,6 +495,11 @@ namespace {
 
 // estimated fixed cost of one tile, in matrix entries
 constexpr int64_t kTileOverhead = 512;
+
+// the ELL / DIA slots a row of a HYB / HDC member adds to its tail / CSR part
+int64_t row_prefix_len(const mph_matrix_impl* A) {
+  return A->format == MPH_HDC ? A->ndiag : A->format == MPH_HYB ? A->ell_loop : 0;
+}
 
 int64_t tile_cost(const BatchTile& t, const std::vector<int32_t>* ro, const mph_matrix_impl* A) {
   if (t.kind == TILE_DIA) return (int64_t)(t.r1 - t.r0) * std::max<int64_t>(A->ndiag, 1);
@@ -423,6 +512,15 @@ int64_t tile_cost(const BatchTile& t, const std::vector<int32_t>* ro, const mph_
 
 template <typename V, int KR>
 int launch_batch(mph_batch_s* b, int init, hipStream_t s) {
+  if constexpr (!std::is_same<V, double>::value)
+    if (b->ncomp > 0)
+      hipLaunchKernelGGL((batch_comp_rows_kernel<V, KR>),
+                         dim3((unsigned)std::min<int32_t>(b->ncomp, 4 * device_cu_count())),
+                         dim3(kBT), 0, s, b->d_mats, b->d_comp, b->ncomp, init);
+  if (b->ntiles == 0) {
+    MPH_HIP_CHECK(hipGetLastError());
+    return MPH_OK;
+  }
   if (b->sched == 1) {
     hipLaunchKernelGGL((batch_spmm_static_kernel<V, KR>), dim3((unsigned)b->nwg), dim3(kBT), 0,
                        s, b->d_mats, b->d_tiles_lpt, b->d_woff, init);
@@ -448,7 +546,7 @@ int launch_k(mph_batch_s* b, int init, hipStream_t s) {
 }
 
 int batch_launch(mph_batch_s* b, int init, hipStream_t s) {
-  if (b->ntiles == 0) return MPH_OK;
+  if (b->ntiles == 0 && b->ncomp == 0) return MPH_OK;
   switch (b->vt) {
     case MPH_F64: return launch_k<double>(b, init, s);
     case MPH_F32: return launch_k<float>(b, init, s);
@@ -473,6 +571,7 @@ int mph_batch_create(mph_batch_t* out, int32_t nmat, const mph_matrix_t* mats, i
   std::vector<BatchMat> hm(nmat);
   std::vector<BatchTile> tiles;
   std::vector<int64_t> cost;
+  std::vector<int64_t> tlen;  // items of a whole-row tile's row (else 0)
   int vt = nmat > 0 ? mats[0]->vt : MPH_F32;
   // COO members are batched as the CSR view of their row-sorted entries: row
   // offsets derived on the device (owned by the batch) and the row blocks of
@@ -544,6 +643,7 @@ int mph_batch_create(mph_batch_t* out, int32_t nmat, const mph_matrix_t* mats, i
                       bl[q].sub < 0 ? bl[q + 1].row : bl[q].row + 1};
           tiles.push_back(t);
           cost.push_back(tile_cost(t, &ro, A));
+          tlen.push_back(t.kind == TILE_WHOLEROW ? (int64_t)ro[t.r0 + 1] - ro[t.r0] + row_prefix_len(A) : 0);
         }
       }
       release_plans(&view);
@@ -583,13 +683,37 @@ int mph_batch_create(mph_batch_t* out, int32_t nmat, const mph_matrix_t* mats, i
         MPH_HIP_CHECK(hipMemcpyAsync(bl.data(), A->rb.blocks, sizeof(RowBlock) * bl.size(),
                                      hipMemcpyDeviceToHost, s));
         MPH_HIP_CHECK(hipStreamSynchronize(s));
-        for (int32_t q = 0; q < A->rb.nblocks; q++) {
+        if (A->rb.prefix_apart) {
+          // FP32 / BF16 HYB / HDC with an ELL / DIA part of kCompensateLen / 2
+          // or more slots per row: every row a whole-row tile, so that the
+          // row's two parts are summed as one Dot2 sum
+          for (int64_t r = 0; r < A->nrows; r++) {
+            BatchTile t{i, TILE_WHOLEROW, (int32_t)r, (int32_t)r + 1};
+            tiles.push_back(t);
+            cost.push_back(tile_cost(t, &ro, A) + kTileOverhead);
+            tlen.push_back((int64_t)ro[t.r0 + 1] - ro[t.r0] + row_prefix_len(A));
+          }
+          bl.assign(1, RowBlock{0, -1});
+        }
+        for (int32_t q = 0; q + 1 < (int32_t)bl.size(); q++) {
           if (bl[q].sub > 0) continue;  // long rows: one whole-row tile
           BatchTile t{i, bl[q].sub < 0 ? TILE_SHORT : TILE_WHOLEROW, bl[q].row,
                       bl[q].sub < 0 ? bl[q + 1].row : bl[q].row + 1};
           tiles.push_back(t);
           cost.push_back(tile_cost(t, &ro, A));
+          tlen.push_back(t.kind == TILE_WHOLEROW ? (int64_t)ro[t.r0 + 1] - ro[t.r0] + row_prefix_len(A) : 0);
         }
+      }
+    } else if (vt != MPH_F64 && (as_dia ? A->ndiag : A->ell_loop) > kCompensateLen) {
+      // FP32 / BF16 rows of more than kCompensateLen diagonals / slots: one
+      // whole-row tile each (Dot2 pairs), with no CSR part
+      m.ro = nullptr;
+      m.pre = as_dia ? BPRE_DIA : BPRE_ELL;
+      for (int64_t r = 0; r < A->nrows; r++) {
+        BatchTile t{i, TILE_WHOLEROW, (int32_t)r, (int32_t)r + 1};
+        tiles.push_back(t);
+        cost.push_back((as_dia ? A->ndiag : A->ell_loop) + kTileOverhead);
+        tlen.push_back(as_dia ? A->ndiag : A->ell_loop);
       }
     } else {
       for (int64_t r = 0; r < A->nrows; r += kSlab) {
@@ -597,6 +721,7 @@ int mph_batch_create(mph_batch_t* out, int32_t nmat, const mph_matrix_t* mats, i
                     (int32_t)std::min<int64_t>(A->nrows, r + kSlab)};
         tiles.push_back(t);
         cost.push_back(tile_cost(t, nullptr, A));
+        tlen.push_back(0);
       }
     }
     m.ndiag = (int32_t)A->ndiag;
@@ -608,6 +733,24 @@ int mph_batch_create(mph_batch_t* out, int32_t nmat, const mph_matrix_t* mats, i
     m.eci = A->ell_ci;
     m.evalp = A->ell_v;
     hm[i] = m;
+  }
+  // FP32 / BF16 whole-row tiles of more than kCompensateLen items leave the
+  // queue for the Dot2 kernel
+  std::vector<BatchTile> comp;
+  if (vt != MPH_F64) {
+    std::vector<BatchTile> keep;
+    std::vector<int64_t> keep_cost;
+    for (size_t q = 0; q < tiles.size(); q++) {
+      const BatchTile& t = tiles[q];
+      if (t.kind == TILE_WHOLEROW && tlen[q] > kCompensateLen) {
+        comp.push_back(t);
+      } else {
+        keep.push_back(t);
+        keep_cost.push_back(cost[q]);
+      }
+    }
+    tiles.swap(keep);
+    cost.swap(keep_cost);
   }
   // largest tiles first: the dynamic queue then drains evenly
   std::vector<size_t> order(tiles.size());
@@ -678,6 +821,11 @@ int mph_batch_create(mph_batch_t* out, int32_t nmat, const mph_matrix_t* mats, i
   if (e == hipSuccess && b->ntiles > 0)
     e = hipMemcpyAsync(b->d_tiles, sorted.data(), sizeof(BatchTile) * b->ntiles,
                        hipMemcpyHostToDevice, s);
+  b->ncomp = (int32_t)comp.size();
+  if (e == hipSuccess && b->ncomp > 0) e = hipMalloc(&b->d_comp, sizeof(BatchTile) * b->ncomp);
+  if (e == hipSuccess && b->ncomp > 0)
+    e = hipMemcpyAsync(b->d_comp, comp.data(), sizeof(BatchTile) * b->ncomp,
+                       hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) {
     set_error(std::string("batch create: ") + hipGetErrorString(e));
@@ -700,6 +848,7 @@ int mph_batch_destroy(mph_batch_t b) {
   dfree(b->d_woff);
   dfree(b->d_qoff);
   dfree(b->d_counter);
+  dfree(b->d_comp);
   for (void* p : b->owned) dfree(p);
   delete b;
   return MPH_OK;

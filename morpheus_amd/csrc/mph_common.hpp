@@ -170,32 +170,61 @@ __device__ __forceinline__ A group_sum(A v, int L) {
   return v;
 }
 
-// Knuth's TwoSum: s + e == a + b exactly.
+// Knuth's TwoSum: s + e == a + b exactly.  Contraction is off in the error
+// free transformations: hipcc contracts a * b + c into an FMA by default,
+// which would fold a product into the sum and leave the error term wrong.
 template <typename A>
 __device__ __forceinline__ void two_sum(A a, A b, A& s, A& e) {
+#pragma clang fp contract(off)
   s = a + b;
   const A bb = s - a;
   e = (a - (s - bb)) + (b - bb);
 }
 
-// Compensated (sum, error) pair; combine() is associative up to rounding of
-// the error terms, which is what cross-lane reductions need.
+// TwoProduct through the FMA: p + e == a * b exactly (no underflow).
+template <typename A>
+__device__ __forceinline__ void two_prod(A a, A b, A& p, A& e) {
+#pragma clang fp contract(off)
+  p = a * b;
+  e = __builtin_fma(a, b, -p);
+}
+
+// Compensated (sum, error) pair.  add_prod() is one step of Dot2 (Ogita,
+// Rump, Oishi, "Accurate sum and dot product", SIAM J. Sci. Comput. 26(6),
+// 2005, Algorithm 5.3): the rounding errors of the product (TwoProduct) and
+// of the addition (TwoSum) both go into the correction c, so a row sum is as
+// accurate as if computed in twice the working precision and then rounded:
+// |result - exact| <= u |exact| + O(n^2 u^2) sum |a x| (in practice about
+// n u^2 sum |a x|).  combine() joins two pairs (cross-lane reductions).
 template <typename A>
 struct csum {
   A s, c;
   __device__ __forceinline__ void add(A v) {
+#pragma clang fp contract(off)
     A t, e;
     two_sum(s, v, t, e);
     s = t;
     c += e;
   }
+  __device__ __forceinline__ void add_prod(A a, A b) {
+#pragma clang fp contract(off)
+    A p, ep, t, e;
+    two_prod(a, b, p, ep);
+    two_sum(s, p, t, e);
+    s = t;
+    c += e + ep;
+  }
   __device__ __forceinline__ void combine(A s2, A c2) {
+#pragma clang fp contract(off)
     A t, e;
     two_sum(s, s2, t, e);
     s = t;
     c += e + c2;
   }
-  __device__ __forceinline__ A value() const { return s + c; }
+  __device__ __forceinline__ A value() const {
+#pragma clang fp contract(off)
+    return s + c;
+  }
 };
 
 // ---------------------------------------------------------------- XCDs

@@ -78,6 +78,16 @@ struct XTilePlan {
 // COO tail of HYB viewed through row offsets built here).
 struct RowBlockPlan {
   int32_t chunk = 0;
+  // rows longer than this leave the short blocks (chunks of `chunk` items,
+  // or the x-tiled pass): `chunk` in FP64, at most kCompensateLen in FP32 /
+  // BF16, so that every row summed with compensation is summed on a path
+  // that carries Dot2 pairs
+  int32_t long_len = 0;
+  // FP32 / BF16 HYB / HDC whose ELL / DIA part holds at least
+  // kCompensateLen / 2 slots per row: that part runs as its own pass (the
+  // DIA / ELL kernels compensate parts longer than kCompensateLen) and the
+  // tail / CSR part accumulates onto it; never fused into the row passes
+  bool prefix_apart = false;
   int32_t nblocks = 0;
   RowBlock* blocks = nullptr;    // nblocks + 1 (sentinel)
   void* partial = nullptr;       // per-block accumulators (long-row chunks)

@@ -40,7 +40,8 @@ template <typename A>
 struct Sum<A, false> {
   A s;
   __device__ static Sum zero() { return Sum{0}; }
-  __device__ void add(A v) { s += v; }
+  __device__ static Sum of(A v) { return Sum{v}; }
+  __device__ void add_prod(A a, A x) { s += a * x; }
   __device__ void merge(const Sum& o) { s += o.s; }
   __device__ A value() const { return s; }
   __device__ Sum up(int d) const { return Sum{shfl_up(s, d)}; }
@@ -49,23 +50,32 @@ struct Sum<A, false> {
   __device__ static Sum load(const A* p, int64_t i) { return Sum{p[i]}; }
 };
 
+// Dot2 pair (TwoSum on the additions, the FMA TwoProduct on the products;
+// Ogita, Rump, Oishi 2005)
 template <typename A>
 struct Sum<A, true> {
   A s, c;
   __device__ static Sum zero() { return Sum{0, 0}; }
-  __device__ void add(A v) {
-    A t, e;
-    two_sum(s, v, t, e);
+  __device__ static Sum of(A v) { return Sum{v, 0}; }
+  __device__ void add_prod(A a, A x) {
+#pragma clang fp contract(off)
+    A p, ep, t, e;
+    two_prod(a, x, p, ep);
+    two_sum(s, p, t, e);
     s = t;
-    c += e;
+    c += e + ep;
   }
   __device__ void merge(const Sum& o) {
+#pragma clang fp contract(off)
     A t, e;
     two_sum(s, o.s, t, e);
     s = t;
     c += e + o.c;
   }
-  __device__ A value() const { return s + c; }
+  __device__ A value() const {
+#pragma clang fp contract(off)
+    return s + c;
+  }
   __device__ Sum up(int d) const { return Sum{shfl_up(s, d), shfl_up(c, d)}; }
   __device__ Sum down(int d) const { return Sum{shfl_down(s, d), shfl_down(c, d)}; }
   __device__ void store(A* p, int64_t i) const {
@@ -105,9 +115,11 @@ __device__ __forceinline__ void chunk_range(const CooArgs<V>& a, int64_t c, int6
   }
 }
 
-template <typename Vec, typename A>
-__device__ __forceinline__ void put_y(Vec* y, int64_t r, A v, int init) {
-  y[r] = init ? (Vec)v : (Vec)((A)y[r] + v);
+// y[r] = v, or y[r] + v (the old y joining the sum before its final rounding)
+template <typename Vec, typename S>
+__device__ __forceinline__ void put_y(Vec* y, int64_t r, S v, int init) {
+  if (!init) v.merge(S::of((decltype(v.s))y[r]));
+  y[r] = (Vec)v.value();
 }
 
 __device__ __forceinline__ void zero_rows(double* y, int64_t a, int64_t b, int64_t ld) {
@@ -168,7 +180,7 @@ __device__ void coo_chunk(const CooArgs<V>& a, int64_t c, int lane) {
     } else if (row == last_row && continues_after) {
       v.store(a.last_val, c);
     } else {
-      put_y(a.y, row * a.ld, v.value(), a.init);
+      put_y(a.y, row * a.ld, v, a.init);
     }
   };
 
@@ -179,9 +191,9 @@ __device__ void coo_chunk(const CooArgs<V>& a, int64_t c, int lane) {
   load_val<V, T>(a.val + i0, n, vals);
 
   // all gathers of the lane are issued before the serial pass consumes them
-  A prod[T];
+  A xk[T];
 #pragma unroll
-  for (int k = 0; k < T; k++) prod[k] = k < n ? vals[k] * (A)a.x[(int64_t)cols[k] * a.ld] : (A)0;
+  for (int k = 0; k < T; k++) xk[k] = k < n ? (A)a.x[(int64_t)cols[k] * a.ld] : (A)0;
 
   // serial pass: head segment, complete middle segments, tail segment
   int32_t h = rows[0], cur = rows[0];
@@ -191,7 +203,6 @@ __device__ void coo_chunk(const CooArgs<V>& a, int64_t c, int lane) {
 #pragma unroll
   for (int k = 0; k < T; k++) {
     if (k < n) {
-      const A p = prod[k];
       if (rows[k] != cur) {
         if (single) {
           head = run;
@@ -203,7 +214,7 @@ __device__ void coo_chunk(const CooArgs<V>& a, int64_t c, int lane) {
         cur = rows[k];
         run = S::zero();
       }
-      run.add(p);
+      run.add_prod(vals[k], xk[k]);
     }
   }
   const int32_t t = cur;
@@ -268,7 +279,7 @@ __global__ void coo_fixup_kernel(CooArgs<V> a) {
     const bool more = e2 < a.nnz && a.ri[e2] == lr;
     if (!(whole && more)) break;
   }
-  put_y(a.y, lr * a.ld, tot.value(), a.init);
+  put_y(a.y, lr * a.ld, tot, a.init);
 }
 
 template <typename V, int T, bool COMP>

@@ -13,7 +13,8 @@
 // and skips the zeroing when ndiag == 0).  ELL visits entries_per_row slots,
 // not the 32-padded width, when the analysis has confirmed that the slots
 // beyond entries_per_row hold only the -1 padding.  FP32 / BF16 SpMV takes
-// two adjacent rows per thread (row-pair kernels below).
+// two adjacent rows per thread (row-pair kernels below); FP32 / BF16 rows of
+// more than kCompensateLen diagonals / slots take a Dot2-compensated kernel.
 #include "mph_internal.hpp"
 
 namespace mph {
@@ -410,7 +411,74 @@ __global__ __launch_bounds__(256) void ell_pair_kernel(EllArgs<V> a) {
   if (row + 1 < a.nrows) a.y[row + 1] = a.init ? t1 : a.y[row + 1] + t1;
 }
 
+// FP32 / BF16 rows of more than kCompensateLen diagonals / slots: thread
+// (row, column q) sums its row as Dot2 pairs (TwoSum on the additions, the
+// FMA TwoProduct on the products) and adds the old y through TwoSum when
+// accumulating.  Diagonals / slots in storage order, as the other kernels.
+template <typename V, bool DIA>
+__global__ __launch_bounds__(256) void dia_ell_comp_kernel(DiaArgs<V> d, EllArgs<V> e, int32_t k) {
+  using Acc = typename vtraits<V>::acc;
+  using Vec = typename vtraits<V>::vec;
+  const int64_t nrows = DIA ? d.nrows : e.nrows;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nrows * k) return;
+  const int64_t row = t / k, q = t % k;
+  csum<Acc> cs{0, 0};
+  if constexpr (DIA) {
+    for (int32_t n = 0; n < d.ndiag; n++) {
+      const int64_t col = row + d.off[n];
+      if (col >= 0 && col < d.ncols)
+        cs.add_prod(vtraits<V>::widen(ld_stream(d.val + d.pitch * n + row)),
+                    (Acc)d.x[col * d.ld + q]);
+    }
+  } else {
+    for (int32_t n = 0; n < e.loop; n++) {
+      const int64_t slot = row + e.pitch * n;
+      const int32_t c = ld_stream(e.ci + slot);
+      if (c >= 0) cs.add_prod(vtraits<V>::widen(ld_stream(e.val + slot)), (Acc)e.x[(int64_t)c * e.ld + q]);
+    }
+  }
+  Vec* yp = (DIA ? d.y : e.y) + row * (DIA ? d.ld : e.ld) + q;
+  if (!(DIA ? d.init : e.init)) cs.combine((Acc)*yp, (Acc)0);
+  *yp = (Vec)cs.value();
+}
+
 namespace {
+
+template <typename V, bool DIA>
+int launch_comp(mph_matrix_impl* A, int k, const void* x, void* y, int init, hipStream_t s) {
+  DiaArgs<V> d{};
+  EllArgs<V> e{};
+  if constexpr (DIA) {
+    d.nrows = A->nrows;
+    d.ncols = A->ncols;
+    d.ndiag = (int32_t)A->ndiag;
+    d.pitch = A->dia_pitch;
+    d.off = A->dia_off;
+    d.val = static_cast<const V*>(A->dia_v);
+    d.x = static_cast<const typename vtraits<V>::vec*>(x);
+    d.y = static_cast<typename vtraits<V>::vec*>(y);
+    d.ld = k;
+    d.init = init;
+  } else {
+    e.nrows = A->nrows;
+    e.ncols = A->ncols;
+    e.loop = (int32_t)A->ell_loop;
+    e.pitch = A->ell_pitch;
+    e.ci = A->ell_ci;
+    e.val = static_cast<const V*>(A->ell_v);
+    e.x = static_cast<const typename vtraits<V>::vec*>(x);
+    e.y = static_cast<typename vtraits<V>::vec*>(y);
+    e.ld = k;
+    e.init = init;
+  }
+  const int64_t n = A->nrows * (int64_t)k;
+  if (n == 0) return MPH_OK;
+  hipLaunchKernelGGL((dia_ell_comp_kernel<V, DIA>), dim3((unsigned)ceil_div(n, 256)), dim3(256), 0,
+                     s, d, e, (int32_t)k);
+  MPH_HIP_CHECK(hipGetLastError());
+  return MPH_OK;
+}
 
 template <typename V, int KR, int RPT>
 int launch_dia_k(mph_matrix_impl* A, const void* x, void* y, int ld, int init,
@@ -589,6 +657,9 @@ bool pair_ok(const mph_matrix_impl* A, int k) {
 template <bool DIA>
 int dispatch(mph_matrix_impl* A, int k, const void* x, void* y, int init,
              hipStream_t s) {
+  if (A->vt != MPH_F64 && (DIA ? A->ndiag : A->ell_loop) > kCompensateLen)
+    return A->vt == MPH_F32 ? launch_comp<float, DIA>(A, k, x, y, init, s)
+                            : launch_comp<bf16, DIA>(A, k, x, y, init, s);
   if (pair_ok<DIA>(A, k))
     return A->vt == MPH_F32 ? launch_pair<float, DIA>(A, x, y, init, s)
                             : launch_pair<bf16, DIA>(A, x, y, init, s);

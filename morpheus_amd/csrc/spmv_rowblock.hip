@@ -130,6 +130,33 @@ __device__ __forceinline__ void row_prefix(const RBArgs<V>& a, int64_t r,
   }
 }
 
+// The same prefix as a Dot2 sum per column (FP32 / BF16 long rows).
+template <typename V, int PRE, int KR>
+__device__ __forceinline__ void row_prefix_comp(const RBArgs<V>& a, int64_t r,
+                                                csum<typename vtraits<V>::acc>* cs) {
+  using Acc = typename vtraits<V>::acc;
+  if constexpr (PRE == PRE_DIA) {
+    for (int d = 0; d < a.ndiag; d++) {
+      const int64_t col = r + a.doff[d];
+      if (col >= 0 && col < a.ncols) {
+        const Acc v = vtraits<V>::widen(ld_stream(a.dval + r + a.dpitch * d));
+#pragma unroll
+        for (int k = 0; k < KR; k++) cs[k].add_prod(v, (Acc)a.x[col * a.ld + k]);
+      }
+    }
+  } else if constexpr (PRE == PRE_ELL) {
+    for (int n = 0; n < a.eloop; n++) {
+      const int64_t slot = r + a.epitch * n;
+      const int32_t c = ld_stream(a.eci + slot);
+      if (c != -1) {
+        const Acc v = vtraits<V>::widen(ld_stream(a.evalp + slot));
+#pragma unroll
+        for (int k = 0; k < KR; k++) cs[k].add_prod(v, (Acc)a.x[(int64_t)c * a.ld + k]);
+      }
+    }
+  }
+}
+
 __device__ __forceinline__ int choose_lanes(int64_t items, int64_t rows,
                                             int fixed) {
   if (fixed > 0) return fixed;
@@ -179,9 +206,19 @@ __device__ __forceinline__ void store_y(Vec* y, int64_t i, A v, int init) {
   y[i] = init ? (Vec)v : (Vec)((A)y[i] + v);
 }
 
-// Products a_j * x[c_j] for items j = base + tid + k*256, k < kBatch: all
-// index and value loads of the batch are issued before the dependent x
-// gathers, so each thread keeps kBatch gathers in flight.
+// A compensated row sum stored into y; when accumulating, the old y joins
+// the pair through TwoSum before the one final rounding.
+template <typename Vec, typename A>
+__device__ __forceinline__ void store_y_cs(Vec* y, int64_t i, csum<A> cs, int init) {
+  if (!init) cs.combine((A)y[i], (A)0);
+  y[i] = (Vec)cs.value();
+}
+
+// Items j = base + tid + k*256, k < kBatch, passed to sink(j, a_j, x[c_j])
+// (the value widened to the accumulation type): all index and value loads
+// of the batch are issued before the dependent x gathers, so each thread
+// keeps kBatch gathers in flight.  The sink forms the product (plainly, or
+// as a Dot2 step).
 constexpr int kBatch = 8;
 
 template <typename V, typename F>
@@ -210,7 +247,7 @@ __device__ __forceinline__ void batched_products(const int32_t* __restrict__ ci,
 #pragma unroll
     for (int k = 0; k < kBatch; k++) {
       const int64_t i = base + (int64_t)k * kRowBlockThreads;
-      if (i < end) sink(i, vtraits<V>::widen(v[k]) * xv[k]);
+      if (i < end) sink(i, vtraits<V>::widen(v[k]), xv[k]);
     }
   }
 }
@@ -227,7 +264,8 @@ __device__ void long_chunk_spmv(const RBArgs<V>& a, uint32_t b, RowBlock blk,
   const int64_t e = min(re, s + (int64_t)a.chunk);
   if constexpr (COMP) {
     csum<Acc> cs{0, 0};
-    batched_products<V>(a.ci, a.val, a.x, s, e, [&](int64_t, Acc p) { cs.add(p); });
+    batched_products<V>(a.ci, a.val, a.x, s, e,
+                        [&](int64_t, Acc av, Acc xv) { cs.add_prod(av, xv); });
     block_csum(cs, s_red);
     if (threadIdx.x == 0) {
       a.partial[2 * b] = cs.s;
@@ -235,7 +273,7 @@ __device__ void long_chunk_spmv(const RBArgs<V>& a, uint32_t b, RowBlock blk,
     }
   } else {
     Acc t = 0;
-    batched_products<V>(a.ci, a.val, a.x, s, e, [&](int64_t, Acc p) { t += p; });
+    batched_products<V>(a.ci, a.val, a.x, s, e, [&](int64_t, Acc av, Acc xv) { t += av * xv; });
     t = block_sum(t, s_red);
     if (threadIdx.x == 0) a.partial[b] = t;
   }
@@ -252,14 +290,15 @@ __global__ void rowblock_combine_kernel(RBArgs<V> a) {
   if (blk.sub != 0) return;
   const int64_t r = blk.row;
   const int64_t nsub = ((int64_t)a.ro[r + 1] - a.ro[r] + a.chunk - 1) / a.chunk;
-  Acc pre[1] = {0};
-  row_prefix<V, PRE, 1>(a, r, pre);
   if constexpr (COMP) {
-    csum<Acc> cs{pre[0], 0};
+    csum<Acc> cs{0, 0};
+    row_prefix_comp<V, PRE, 1>(a, r, &cs);
     for (int64_t q = 0; q < nsub; q++)
       cs.combine(a.partial[2 * (b + q)], a.partial[2 * (b + q) + 1]);
-    store_y(a.y, out_row(a, r), cs.value(), a.init);
+    store_y_cs(a.y, out_row(a, r), cs, a.init);
   } else {
+    Acc pre[1] = {0};
+    row_prefix<V, PRE, 1>(a, r, pre);
     Acc t = 0;
     for (int64_t q = 0; q < nsub; q++) t += a.partial[b + q];
     store_y(a.y, out_row(a, r), t + pre[0], a.init);
@@ -314,7 +353,7 @@ __global__ __launch_bounds__(kRowBlockThreads) void rowblock_spmv_kernel(
     rv[k] = a.ro[row0 + min(i, nr)];
   }
   batched_products<V>(a.ci + p0, a.val + p0, a.x, 0, n,
-                      [&](int64_t i, Acc p) { s_prod[i] = p; });
+                      [&](int64_t i, Acc av, Acc xv) { s_prod[i] = av * xv; });
 #pragma unroll
   for (int k = 0; k < kRoLoads; k++) {
     const int i = threadIdx.x + k * kRowBlockThreads;
@@ -332,26 +371,16 @@ __global__ __launch_bounds__(kRowBlockThreads) void rowblock_spmv_kernel(
   const int L = a.lanes_fixed > 0 ? a.lanes_fixed : lanes_b;
   const int groups = kRowBlockThreads / L;
   const int g = threadIdx.x / L, l = threadIdx.x % L;
+  // (a short block holds rows of at most long_len <= kCompensateLen items in
+  // FP32 / BF16: longer rows take the chunk or x-tiled paths, which carry
+  // Dot2 pairs, so the sums here are plain)
   for (int r = g; r < nr; r += groups) {
     const int st = s_ro[r], en = s_ro[r + 1];
-    Acc out;
-    if (COMP && en - st > kCompensateLen) {
-      csum<Acc> cs{0, 0};
-      for (int j = st + l; j < en; j += L) cs.add(s_prod[j]);
-      for (int m = L >> 1; m > 0; m >>= 1) {
-        const Acc s2 = shfl_xor(cs.s, m), c2 = shfl_xor(cs.c, m);
-        cs.combine(s2, c2);
-      }
-      if constexpr (PRE != PRE_NONE) cs.add(s_pre[r]);
-      out = cs.value();
-    } else {
-      Acc t = 0;
-      for (int j = st + l; j < en; j += L) t += s_prod[j];
-      t = group_sum(t, L);
-      if constexpr (PRE != PRE_NONE) t += s_pre[r];
-      out = t;
-    }
-    if (l == 0) store_y(a.y, out_row(a, row0 + r), out, a.init);
+    Acc t = 0;
+    for (int j = st + l; j < en; j += L) t += s_prod[j];
+    t = group_sum(t, L);
+    if constexpr (PRE != PRE_NONE) t += s_pre[r];
+    if (l == 0) store_y(a.y, out_row(a, row0 + r), t, a.init);
   }
 }
 
@@ -442,23 +471,27 @@ __device__ __forceinline__ void xtile_load(const XTArgs& t, const V* cval, const
   }
 }
 
-// A running sum that is plain (FP64) or TwoSum-compensated (FP32).
+// A running sum of products a * x: plain (FP64) or Dot2-compensated (FP32).
 template <typename A, bool C>
 struct XSum {
   A s, c;
   __device__ static XSum zero() { return XSum{0, 0}; }
-  __device__ void add(A v) {
+  __device__ void add(A p) { s += p; }  // plain form only
+  __device__ void add_prod(A a, A x) {
     if constexpr (C) {
-      A t, e;
-      two_sum(s, v, t, e);
+#pragma clang fp contract(off)
+      A p, ep, t, e;
+      two_prod(a, x, p, ep);
+      two_sum(s, p, t, e);
       s = t;
-      c += e;
+      c += e + ep;
     } else {
-      s += v;
+      s += a * x;
     }
   }
   __device__ void merge(const XSum& o) {
     if constexpr (C) {
+#pragma clang fp contract(off)
       A t, e;
       two_sum(s, o.s, t, e);
       s = t;
@@ -546,12 +579,21 @@ __global__ __launch_bounds__(kXTileThreads) void xtile_kernel(RBArgs<V> a, XTArg
     const int32_t nr = s1.i0 - s0.i0, n = s1.e0 - s0.e0;
     const int32_t i0 = s0.i0, e0 = s0.e0;
     const int32_t* ro = s_lro + (i0 - gstart);  // ro[r] - e0: first item of run r
-    Acc p[kBatch];
+    // FP64: the products; FP32 / BF16: value and x kept apart for Dot2
+    Acc pa[kBatch], px[kBatch];
 #pragma unroll
-    for (int b = 0; b < kBatch; b++)
+    for (int b = 0; b < kBatch; b++) {
       // the copy holds columns inside the tile; the clamp keeps a corrupted
       // entry inside the LDS tile
-      p[b] = XTHeld<V>::use(cur.v[b]) * (Acc)xs[min(cur.c[b], (uint32_t)(nx - 1))];
+      const Acc xv = (Acc)xs[min(cur.c[b], (uint32_t)(nx - 1))];
+      if constexpr (COMP) {
+        pa[b] = XTHeld<V>::use(cur.v[b]);
+        px[b] = xv;
+      } else {
+        pa[b] = XTHeld<V>::use(cur.v[b]) * xv;
+        px[b] = (Acc)1;
+      }
+    }
     const int32_t hd = (int32_t)min(cur.h, (uint32_t)(nr - 1));
     auto put = [&](int32_t r, const S& v) {
       const int64_t o = (int64_t)tile * t.rows + i0 + r;
@@ -593,7 +635,10 @@ __global__ __launch_bounds__(kXTileThreads) void xtile_kernel(RBArgs<V> a, XTArg
           } while (b >= next_rel);
           run = S::zero();
         }
-        run.add(p[b]);
+        if constexpr (COMP)
+          run.add_prod(pa[b], px[b]);
+        else
+          run.add(pa[b]);
       }
     }
     const int32_t tl = cur_r;
@@ -712,13 +757,14 @@ __global__ __launch_bounds__(256) void xtile_combine_kernel(RBArgs<V> a, XTArgs 
   __syncthreads();
   if (rg != 0 || i >= t.rows) return;
   const int64_t r = t.row[i];
-  Acc pre[1] = {0};
-  row_prefix<V, PRE, 1>(a, r, pre);
   if constexpr (COMP) {
-    csum<Acc> tot{pre[0], 0};
+    csum<Acc> tot{0, 0};
+    row_prefix_comp<V, PRE, 1>(a, r, &tot);
     for (int q = 0; q < kXCombRanges; q++) tot.combine(s_sum[q][ri], s_err[q][ri]);
-    store_y(a.y, r * a.ld, tot.value(), a.init);
+    store_y_cs(a.y, r * a.ld, tot, a.init);
   } else {
+    Acc pre[1] = {0};
+    row_prefix<V, PRE, 1>(a, r, pre);
     Acc tot = 0;
     for (int q = 0; q < kXCombRanges; q++) tot += s_sum[q][ri];
     store_y(a.y, r * a.ld, tot + pre[0], a.init);
@@ -772,15 +818,72 @@ __global__ void xtile_bounds_kernel(const int32_t* __restrict__ ro,
 // Each row is reduced by a group of lanes reading the row's items directly
 // (coalesced within the row); every item feeds KR FMAs from one contiguous
 // load of the x row.  The DIA / ELL prefix is split across the group's lanes.
+// A long-row chunk of the SpMM in FP32 / BF16: Dot2 pairs per column, the
+// block's pair per column stored at partial[2 * (b * KR + q)].
+template <typename V, int KR>
+__device__ void spmm_chunk_comp(const RBArgs<V>& a, uint32_t b, int64_t s, int64_t e,
+                                typename vtraits<V>::acc* s_red) {
+  using Acc = typename vtraits<V>::acc;
+  csum<Acc> t[KR];
+#pragma unroll
+  for (int q = 0; q < KR; q++) t[q] = csum<Acc>{0, 0};
+  for (int64_t i = s + threadIdx.x; i < e; i += kRowBlockThreads) {
+    const Acc v = vtraits<V>::widen(ld_stream(a.val + i));
+    const int64_t c = ld_stream(a.ci + i);
+#pragma unroll
+    for (int q = 0; q < KR; q++) t[q].add_prod(v, (Acc)a.x[c * a.ld + q]);
+  }
+#pragma unroll
+  for (int q = 0; q < KR; q++)
+    for (int m = kWave >> 1; m > 0; m >>= 1) {
+      const Acc s2 = shfl_xor(t[q].s, m), c2 = shfl_xor(t[q].c, m);
+      t[q].combine(s2, c2);
+    }
+  constexpr int kW = kRowBlockThreads / kWave;
+  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < KR; q++) {
+      s_red[2 * (q * kW + wave)] = t[q].s;
+      s_red[2 * (q * kW + wave) + 1] = t[q].c;
+    }
+  __syncthreads();
+  if (threadIdx.x < KR) {
+    const int q = threadIdx.x;
+    csum<Acc> u{s_red[2 * q * kW], s_red[2 * q * kW + 1]};
+    for (int w = 1; w < kW; w++) u.combine(s_red[2 * (q * kW + w)], s_red[2 * (q * kW + w) + 1]);
+    a.partial[2 * ((int64_t)b * KR + q)] = u.s;
+    a.partial[2 * ((int64_t)b * KR + q) + 1] = u.c;
+  }
+}
+
+// FP32 / BF16: the long-row chunks of the SpMM in a launch of their own, so
+// that the Dot2 pairs do not raise the register count of the short-row
+// kernel (every record but the chunks returns at once).
+template <typename V, int KR>
+__global__ __launch_bounds__(kRowBlockThreads) void rowblock_spmm_chunk_kernel(RBArgs<V> a) {
+  using Acc = typename vtraits<V>::acc;
+  __shared__ Acc s_red[2 * KR * kRowBlockThreads / kWave];
+  const uint32_t b = blockIdx.x;
+  const RowBlock blk = a.blocks[b];
+  if (blk.sub < 0) return;
+  const int64_t r = blk.row;
+  const int64_t rs = a.ro[r], re = a.ro[r + 1];
+  const int64_t s = rs + (int64_t)blk.sub * a.chunk;
+  spmm_chunk_comp<V, KR>(a, b, s, min(re, s + (int64_t)a.chunk), s_red);
+}
+
 template <typename V, int PRE, int KR>
 __global__ __launch_bounds__(kRowBlockThreads) void rowblock_spmm_kernel(
     RBArgs<V> a) {
   using Acc = typename vtraits<V>::acc;
+  constexpr bool COMP = std::is_same<Acc, float>::value;
   __shared__ Acc s_red[KR * kRowBlockThreads / kWave];
   const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
   const RowBlock blk = a.blocks[b];
   if (blk.sub < -1) return;  // a long row handled by the x-tiled pass
   if (blk.sub >= 0) {
+    if constexpr (COMP) return;  // rowblock_spmm_chunk_kernel
     const int64_t r = blk.row;
     const int64_t rs = a.ro[r], re = a.ro[r + 1];
     const int64_t s = rs + (int64_t)blk.sub * a.chunk;
@@ -872,6 +975,19 @@ __global__ void rowblock_spmm_combine_kernel(RBArgs<V> a) {
   if (blk.sub != 0) return;
   const int64_t r = blk.row;
   const int64_t nsub = ((int64_t)a.ro[r + 1] - a.ro[r] + a.chunk - 1) / a.chunk;
+  if constexpr (std::is_same<Acc, float>::value) {
+    csum<Acc> cs[KR];
+#pragma unroll
+    for (int q = 0; q < KR; q++) cs[q] = csum<Acc>{0, 0};
+    row_prefix_comp<V, PRE, KR>(a, r, cs);
+    for (int64_t j = 0; j < nsub; j++)
+#pragma unroll
+      for (int q = 0; q < KR; q++)
+        cs[q].combine(a.partial[2 * ((b + j) * KR + q)], a.partial[2 * ((b + j) * KR + q) + 1]);
+#pragma unroll
+    for (int q = 0; q < KR; q++) store_y_cs(a.y, r * a.ld + q, cs[q], a.init);
+    return;
+  }
   Acc tot[KR];
 #pragma unroll
   for (int q = 0; q < KR; q++) tot[q] = 0;
@@ -884,29 +1000,34 @@ __global__ void rowblock_spmm_combine_kernel(RBArgs<V> a) {
 }
 
 // --------------------------------------------------------------- analysis
+// A row longer than long_len (<= chunk) is long: ceil(len / chunk) chunk
+// blocks of its own.  Short blocks start at multiples of kRowBlockMaxRows
+// rows, where the item window of `chunk` items changes, and after a long row.
 __device__ __forceinline__ bool rb_is_start(const int32_t* ro, int64_t r,
-                                            int32_t chunk, bool& is_long,
-                                            int64_t& len) {
+                                            int32_t chunk, int32_t long_len,
+                                            bool& is_long, int64_t& len) {
   len = (int64_t)ro[r + 1] - ro[r];
-  is_long = len > chunk;
+  is_long = len > long_len;
   if (r == 0 || is_long) return true;
   const int64_t plen = (int64_t)ro[r] - ro[r - 1];
   return (r % kRowBlockMaxRows) == 0 || (ro[r] / chunk) != (ro[r - 1] / chunk) ||
-         plen > chunk;
+         plen > long_len;
 }
 
 __global__ void rb_count_kernel(const int32_t* __restrict__ ro, int64_t nrows,
-                                int32_t chunk, int32_t* __restrict__ cnt) {
+                                int32_t chunk, int32_t long_len,
+                                int32_t* __restrict__ cnt) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrows) return;
   bool lg;
   int64_t len;
-  const bool st = rb_is_start(ro, r, chunk, lg, len);
+  const bool st = rb_is_start(ro, r, chunk, long_len, lg, len);
   cnt[r] = st ? (lg ? (int32_t)((len + chunk - 1) / chunk) : 1) : 0;
 }
 
 __global__ void rb_fill_kernel(const int32_t* __restrict__ ro, int64_t nrows,
-                               int32_t chunk, const int32_t* __restrict__ first,
+                               int32_t chunk, int32_t long_len,
+                               const int32_t* __restrict__ first,
                                RowBlock* __restrict__ blocks) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrows) return;
@@ -916,7 +1037,7 @@ __global__ void rb_fill_kernel(const int32_t* __restrict__ ro, int64_t nrows,
   if (c == 0) return;
   bool lg;
   int64_t len;
-  rb_is_start(ro, r, chunk, lg, len);
+  rb_is_start(ro, r, chunk, long_len, lg, len);
   for (int32_t q = 0; q < c; q++) blocks[f + q] = RowBlock{(int32_t)r, lg ? q : -1};
 }
 
@@ -1444,7 +1565,7 @@ int split_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
   // part: no prefix pass)
   const bool prefixed = (A->format == MPH_HYB && !p.m_ro && A->ell_loop > 0) ||
                         (A->format == MPH_HDC && A->ndiag > 0);
-  const bool split = prefixed && (xt_on || nrows >= kSplitMinRows);
+  const bool split = prefixed && (xt_on || nrows >= kSplitMinRows || p.prefix_apart);
   if ((!xt_on && !split) || nrows == 0 || A->ncols == 0) return MPH_OK;
   const int32_t* d_ci = p.ci;
   const void* d_val = p.val;
@@ -1477,21 +1598,21 @@ int split_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
   auto is_tiled = [&](int64_t i) { return tiled[split ? map[i] : i] != 0; };
   auto len = [&](int64_t i) { return (int64_t)L[i + 1] - L[i]; };
 
-  const int64_t chunk = p.chunk;
+  const int64_t chunk = p.chunk, long_len = p.long_len;
   std::vector<RowBlock> sb;
   int32_t nchunked = 0;
   for (int64_t i = 0; i < m;) {
     if (is_tiled(i)) {
       sb.push_back(RowBlock{(int32_t)i, -2});
       i++;
-    } else if (len(i) > chunk) {
+    } else if (len(i) > long_len) {
       for (int64_t q = 0; q * chunk < len(i); q++) sb.push_back(RowBlock{(int32_t)i, (int32_t)q});
       nchunked++;
       i++;
     } else {
       const int64_t i0 = i;
       int64_t items = 0;
-      while (i < m && !is_tiled(i) && len(i) <= chunk && i - i0 < kRowBlockMaxRows &&
+      while (i < m && !is_tiled(i) && len(i) <= long_len && i - i0 < kRowBlockMaxRows &&
              items + len(i) < 2 * chunk) {
         items += len(i);
         i++;
@@ -1514,13 +1635,13 @@ int split_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
       if (k.sub == -1) {
         const int64_t i1 = sb[b + 1].row;
         ok = ok && i1 > k.row && i1 - k.row <= kRowBlockMaxRows && (int64_t)L[i1] - L[k.row] < 2 * chunk;
-        for (int64_t i = k.row; ok && i < i1; i++) ok = !is_tiled(i) && len(i) <= chunk;
+        for (int64_t i = k.row; ok && i < i1; i++) ok = !is_tiled(i) && len(i) <= long_len;
         next = i1;
       } else if (k.sub == -2) {
         ok = ok && is_tiled(k.row);
         next = k.row + 1;
       } else {
-        ok = ok && !is_tiled(k.row) && (int64_t)k.sub * chunk < len(k.row) &&
+        ok = ok && !is_tiled(k.row) && len(k.row) > long_len && (int64_t)k.sub * chunk < len(k.row) &&
              (k.sub == 0 || (sb[b - 1].row == k.row && sb[b - 1].sub == k.sub - 1));
         next = k.row + 1;
       }
@@ -1549,7 +1670,7 @@ int split_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
   const bool grow = (int64_t)sb.size() > (int64_t)p.nblocks + 1;
   bool fail = hipMalloc(&d_sb, sizeof(RowBlock) * sb.size()) != hipSuccess ||
               hipMalloc(&d_kr, sizeof(int32_t) * kr.size()) != hipSuccess ||
-              (grow && hipMalloc(&d_part, (size_t)accs * 16 * sb.size()) != hipSuccess);
+              (grow && hipMalloc(&d_part, (size_t)accs * 32 * sb.size()) != hipSuccess);
   if (!fail && split)
     fail = hipMalloc(&d_map, sizeof(int32_t) * (size_t)std::max<int64_t>(m, 1)) != hipSuccess ||
            hipMalloc(&d_lro, sizeof(int32_t) * (size_t)(m + 1)) != hipSuccess;
@@ -1742,6 +1863,16 @@ int rowblock_analyse(mph_matrix_impl* A, hipStream_t s) {
     }
   }
   p.chunk = A->tuning.rows_chunk > 0 ? A->tuning.rows_chunk : default_chunk(A->vt);
+  {
+    // the ELL / DIA slots a row adds in a fused pass
+    const int64_t prefix = (A->format == MPH_HYB && !p.m_ro) ? A->ell_loop
+                           : A->format == MPH_HDC            ? A->ndiag
+                                                             : 0;
+    p.prefix_apart = A->vt != MPH_F64 && prefix >= kCompensateLen / 2;
+    p.long_len = A->vt == MPH_F64 ? p.chunk
+                                  : std::min<int64_t>(p.chunk, p.prefix_apart ? kCompensateLen
+                                                                              : kCompensateLen - prefix);
+  }
   if (nrows == 0) {
     p.nblocks = 0;
     return MPH_OK;
@@ -1754,7 +1885,7 @@ int rowblock_analyse(mph_matrix_impl* A, hipStream_t s) {
   MPH_HIP_CHECK(hipMalloc(&tmp, sizeof(int64_t) * scan_tmp_elems(nrows)));
   const unsigned g = (unsigned)ceil_div(nrows, 256);
   hipLaunchKernelGGL(rb_count_kernel, dim3(g), dim3(256), 0, s, ro, nrows,
-                     p.chunk, cnt);
+                     p.chunk, p.long_len, cnt);
   MPH_HIP_CHECK(hipGetLastError());
   int st = exclusive_scan_i32(cnt, first, nrows, tmp, s);
   int32_t nb = 0;
@@ -1764,12 +1895,12 @@ int rowblock_analyse(mph_matrix_impl* A, hipStream_t s) {
     MPH_HIP_CHECK(hipStreamSynchronize(s));
     p.nblocks = nb;
     const int accs = (A->vt == MPH_F64 ? 8 : 4);
-    // partials: up to 16 RHS per block (SpMM) or a compensated pair (SpMV)
+    // partials: up to 16 RHS per block (SpMM), as Dot2 pairs in FP32 / BF16
     MPH_HIP_CHECK(hipMalloc(&p.blocks, sizeof(RowBlock) * (nb + 1)));
-    MPH_HIP_CHECK(hipMalloc(&p.partial, (size_t)accs * 16 * (nb + 1)));
+    MPH_HIP_CHECK(hipMalloc(&p.partial, (size_t)accs * 32 * (nb + 1)));
     {
       hipLaunchKernelGGL(rb_fill_kernel, dim3(g), dim3(256), 0, s, ro, nrows,
-                         p.chunk, first, p.blocks);
+                         p.chunk, p.long_len, first, p.blocks);
       MPH_HIP_CHECK(hipGetLastError());
       MPH_HIP_CHECK(hipStreamSynchronize(s));
       std::vector<int32_t> krec;
@@ -1900,6 +2031,10 @@ int launch_spmm_k(mph_matrix_impl* A, const void* x, void* y, int ld, int init,
   hipLaunchKernelGGL((rowblock_spmm_kernel<V, PRE, KR>),
                      dim3((unsigned)A->rb.nblocks), dim3(kRowBlockThreads), 0, s,
                      a);
+  if constexpr (!std::is_same<V, double>::value)
+    if (A->rb.nlong > 0)
+      hipLaunchKernelGGL((rowblock_spmm_chunk_kernel<V, KR>), dim3((unsigned)A->rb.nblocks),
+                         dim3(kRowBlockThreads), 0, s, a);
   if (A->rb.nlong > 0)
     hipLaunchKernelGGL((rowblock_spmm_combine_kernel<V, PRE, KR>),
                        dim3((unsigned)ceil_div(A->rb.nblocks, 256)), dim3(256), 0, s, a);
@@ -1920,6 +2055,10 @@ int launch_split_spmm_k(mph_matrix_impl* A, const void* x, void* y, int ld, int 
   if (p.nsblocks > 0)
     hipLaunchKernelGGL((rowblock_spmm_kernel<V, PRE_NONE, KR>), dim3((unsigned)p.nsblocks),
                        dim3(kRowBlockThreads), 0, s, a);
+  if constexpr (!std::is_same<V, double>::value)
+    if (p.nchunked > 0)
+      hipLaunchKernelGGL((rowblock_spmm_chunk_kernel<V, KR>), dim3((unsigned)p.nsblocks),
+                         dim3(kRowBlockThreads), 0, s, a);
   if (p.nchunked > 0)
     hipLaunchKernelGGL((rowblock_spmm_combine_kernel<V, PRE_NONE, KR>),
                        dim3((unsigned)ceil_div(p.nsblocks, 256)), dim3(256), 0, s, a);
@@ -1995,6 +2134,12 @@ int dispatch_pre(mph_matrix_impl* A, int k, const void* x, void* y, int init,
     if (pre == PRE_DIA) return launch_spmv<V, PRE_DIA>(A, x, y, init, s);
     if (pre == PRE_ELL) return launch_spmv<V, PRE_ELL>(A, x, y, init, s);
     return launch_spmv<V, PRE_NONE>(A, x, y, init, s);
+  }
+  if (pre != PRE_NONE && A->rb.prefix_apart) {
+    // the ELL / DIA part in its own pass, the tail / CSR part added to it
+    MPH_CHECK_STATUS(pre == PRE_ELL ? ell_multiply(A, k, x, y, init, s)
+                                    : dia_multiply(A, k, x, y, init, s));
+    return launch_spmm<V, PRE_NONE>(A, k, x, y, 0, s);
   }
   if (pre == PRE_DIA) return launch_spmm<V, PRE_DIA>(A, k, x, y, init, s);
   if (pre == PRE_ELL) return launch_spmm<V, PRE_ELL>(A, k, x, y, init, s);

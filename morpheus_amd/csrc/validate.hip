@@ -114,9 +114,9 @@ int validate_rowblock_plan(const int32_t* d_ro, int64_t nrows, RowBlockPlan& p,
       if (r1 - k.row > kRowBlockMaxRows) return fail("short block row count");
       if ((int64_t)ro[r1] - ro[k.row] >= 2 * (int64_t)p.chunk) return fail("short block items");
       for (int64_t r = k.row; r < r1; r++)
-        if ((int64_t)ro[r + 1] - ro[r] > p.chunk) return fail("long row in short block");
+        if ((int64_t)ro[r + 1] - ro[r] > p.long_len) return fail("long row in short block");
     } else {
-      if (len <= p.chunk) return fail("chunked row is short");
+      if (len <= p.long_len) return fail("chunked row is short");
       const int64_t nsub = (len + p.chunk - 1) / p.chunk;
       if (k.sub >= nsub || b - k.sub < 0) return fail("chunk index");
       const RowBlock f = bl[b - k.sub];

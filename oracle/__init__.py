@@ -53,6 +53,7 @@ _SIGS = {
     "orc_ell_spmv": (None, [i64, i64, i64, p, p, p, p, I]),
     "orc_csr_absrow": (None, [i64, p, p, p, p, p]),
     "orc_csr_spmv_compensated": (None, [i64, p, p, p, p, p]),
+    "orc_csr_spmv_f32": (None, [i64, p, p, p, p, p, I]),
     "orc_exceeds_tolerance": (I, [i64, i64, i64]),
     "orc_pad_size": (i64, [i64, i64]),
     "orc_coo_is_sorted": (I, [i64, p, p]),
@@ -182,6 +183,21 @@ def spmv_compensated(A: Csr, x) -> np.ndarray:
     out = _f64(A.nrows)
     lib().orc_csr_spmv_compensated(A.nrows, _ptr(A.row_offsets), _ptr(A.column_indices),
                                    _ptr(A.values), _ptr(x), _ptr(out))
+    return out
+
+
+F32_PLAIN, F32_LANES, F32_TWOSUM, F32_DOT2 = 0, 1, 2, 3
+
+
+def spmv_f32(A: Csr, x, mode: int) -> np.ndarray:
+    """FP32 row sums of FP32-rounded values and x in one of four schemes
+    (plain serial, 64 lanes + tree, TwoSum only, Dot2): the witnesses of the
+    FP32 accuracy check, not a reference restatement."""
+    v = np.ascontiguousarray(A.values, dtype=np.float32)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.empty(A.nrows, dtype=np.float32)
+    lib().orc_csr_spmv_f32(A.nrows, _ptr(A.row_offsets), _ptr(A.column_indices),
+                           _ptr(v), _ptr(x), _ptr(out), int(mode))
     return out
 
 

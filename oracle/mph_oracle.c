@@ -98,6 +98,58 @@ void orc_csr_spmv_compensated(int64_t nrows, const int32_t* ro,
   }
 }
 
+/* FP32 row sums over FP32 values and x, in four accumulation schemes.  They
+ * do not restate a reference function (the reference accumulates in the
+ * value type, core/src/impl/Csr/Serial/Morpheus_Multiply_Impl.hpp:51-57);
+ * they are the witnesses that show the FP32 accuracy check of the tests
+ * tells a plain FP32 accumulator from a compensated one:
+ *   mode 0  serial sum of rounded products (the Serial loop in float);
+ *   mode 1  64 lane-strided partial sums, then a pairwise tree (the shape of
+ *           a plain GPU row reduction);
+ *   mode 2  TwoSum (Knuth) on the rounded products: the additions are
+ *           compensated, the products are not;
+ *   mode 3  Dot2 (Ogita, Rump, Oishi, "Accurate sum and dot product", SIAM J.
+ *           Sci. Comput. 26(6), 2005, Algorithm 5.3): TwoSum on the additions
+ *           and the FMA form of TwoProduct (e = fma(a, x, -p)) on the
+ *           products, both error terms added into one FP32 correction.
+ * Built with -ffp-contract=off, so every float operation rounds once. */
+static void two_sum_f(float a, float b, float* s, float* e) {
+  const float t = a + b;
+  const float bb = t - a;
+  *e = (a - (t - bb)) + (b - bb);
+  *s = t;
+}
+
+void orc_csr_spmv_f32(int64_t nrows, const int32_t* ro, const int32_t* ci,
+                      const float* v, const float* x, float* y, int mode) {
+  for (int64_t i = 0; i < nrows; i++) {
+    const int32_t b = ro[i], e = ro[i + 1];
+    if (mode == 1) {
+      float lane[64];
+      for (int l = 0; l < 64; l++) lane[l] = 0.0f;
+      for (int32_t jj = b; jj < e; jj++) lane[(jj - b) % 64] += v[jj] * x[ci[jj]];
+      for (int w = 32; w > 0; w >>= 1)
+        for (int l = 0; l < w; l++) lane[l] += lane[l + w];
+      y[i] = lane[0];
+      continue;
+    }
+    float s = 0.0f, c = 0.0f;
+    for (int32_t jj = b; jj < e; jj++) {
+      const float a = v[jj], xv = x[ci[jj]];
+      const float p = a * xv;
+      if (mode == 0) {
+        s += p;
+      } else {
+        float t, es;
+        two_sum_f(s, p, &t, &es);
+        s = t;
+        c += (mode == 3) ? es + fmaf(a, xv, -p) : es;
+      }
+    }
+    y[i] = (mode == 0) ? s : s + c;
+  }
+}
+
 /* ========================================================================
  * Helpers
  * ===================================================================== */

@@ -58,6 +58,11 @@ void orc_csr_absrow(int64_t nrows, const int32_t* ro, const int32_t* ci,
 void orc_csr_spmv_compensated(int64_t nrows, const int32_t* ro,
                               const int32_t* ci, const double* v,
                               const double* x, double* y);
+/* FP32 row sums in four accumulation schemes (0 plain serial, 1 64 lanes +
+ * tree, 2 TwoSum on rounded products, 3 Dot2): the witnesses that the FP32
+ * accuracy check separates plain from compensated accumulation. */
+void orc_csr_spmv_f32(int64_t nrows, const int32_t* ro, const int32_t* ci,
+                      const float* v, const float* x, float* y, int mode);
 
 /* ----------------------------------------------------------------- helpers */
 int orc_exceeds_tolerance(int64_t num_rows, int64_t num_entries,

@@ -166,3 +166,67 @@ def dense_of(A: Csr) -> np.ndarray:
     r = np.repeat(np.arange(A.nrows), np.diff(A.row_offsets))
     D[r, A.column_indices] = A.values
     return D
+
+
+U32 = 2.0 ** -24
+
+
+def round_sig(v, bits: int):
+    """Round to `bits` significant bits (24: FP32, 8: BF16), to nearest."""
+    m, e = np.frexp(np.asarray(v, dtype=np.float64))
+    return np.ldexp(np.round(m * 2.0 ** bits) / 2.0 ** bits, e)
+
+
+def cancellation_csr(nrows: int, ncols: int, long_lengths, seed: int, bits: int = 24,
+                     rho: float = 1e-3, short_max: int = 8) -> tuple:
+    """A CSR matrix and x whose long rows cancel heavily, with every value
+    exact in `bits` significant bits (24: FP32, 8: BF16) and x exact in FP32,
+    so every product is exact in FP64.
+
+    Each row holds its diagonal entry.  Rows listed in `long_lengths` (placed
+    at random rows) have that many entries: in column order the first half
+    of the products are positive and the second half negative, scaled so the
+    row sum is about rho * (sum of the positive products); sum |a x| is then
+    about 2 / rho times |sum a x|, and every partial sum in column order is
+    large.  Other rows hold 1..short_max entries with values in [-1, 1].
+    Returns (A, x, long_rows)."""
+    from morpheus_amd.formats import Csr
+    rng = np.random.default_rng(seed)
+    x = round_sig(rng.uniform(0.5, 2.0, ncols), 24)
+    long_rows = rng.choice(nrows, size=len(long_lengths), replace=False)
+    lens = rng.integers(1, short_max + 1, nrows)
+    lens[long_rows] = long_lengths
+    cols, vals = [], []
+    for r in range(nrows):
+        n = int(lens[r])
+        c = rng.choice(ncols - 1, size=n - 1, replace=False)
+        c = np.sort(np.concatenate([c + (c >= r), [r]])) if r < ncols else np.sort(
+            rng.choice(ncols, size=n, replace=False))
+        if r in long_rows:
+            a = round_sig(rng.uniform(1.0, 2.0, n) * 2.0 ** rng.integers(-6, 6, n), bits)
+            h = n // 2
+            p = a * x[c]
+            scale = p[:h].sum() * (1.0 - rho) / p[h:].sum()
+            a[h:] = -round_sig(a[h:] * scale, bits)
+        else:
+            a = round_sig(rng.uniform(-1.0, 1.0, n), bits)
+            a[a == 0] = 0.5
+        cols.append(c)
+        vals.append(a)
+    ro = np.zeros(nrows + 1, dtype=np.int64)
+    np.cumsum(lens, out=ro[1:])
+    A = Csr(nrows, ncols, ro, np.concatenate(cols), np.concatenate(vals))
+    return A, x, np.sort(long_rows)
+
+
+def compensated_bound(nnz_row, absrow, y_exact):
+    """The accuracy of a Dot2-compensated FP32 row sum over exact inputs:
+    |y - y_exact| <= 4 u |y_exact| + n u^2 sum_j |a_ij x_j| (u = 2^-24)."""
+    n = np.asarray(nnz_row, dtype=np.float64)
+    return 4.0 * U32 * np.abs(y_exact) + n * U32 * U32 * np.asarray(absrow) + 1e-300
+
+
+def plain_fp32_bound(nnz_row, absrow):
+    """What any FP32 summation order guarantees: (n + 2) u sum_j |a_ij x_j|."""
+    n = np.asarray(nnz_row, dtype=np.float64)
+    return (n + 2.0) * U32 * np.asarray(absrow) + 1e-300
