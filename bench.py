@@ -230,7 +230,7 @@ def run_large(reps=10):
         log(f"large: generated {mname} (nnz {A.nnz}) in {time.time() - t0:.1f}s")
         x = gen.random_vector(A.ncols, 31)
         ref = oracle.spmv(A, x)
-        bound = 4.0 * (np.diff(A.row_offsets) + 1) * 2.0 ** -53 * oracle.absrow(A, x)
+        bound = 4.0 * np.maximum(np.diff(A.row_offsets), 1) * 2.0 ** -53 * oracle.absrow(A, x)
         xt = torch.from_numpy(x).cuda()
         y = torch.empty(A.nrows, dtype=torch.float64, device="cuda")
         base = device.to_device(A, "f64")
@@ -334,7 +334,7 @@ def run_large_autotuned(A, mname, ks=(1, 4, 16), reps=5):
                  "max_rel_err_col0": err}
             if dtype == "f64":
                 r["within_fp64_bound_col0"] = bool(np.all(
-                    np.abs(y0 - ref[k]) <= 4.0 * (nrow + 1) * 2.0 ** -53 * scale[k]))
+                    np.abs(y0 - ref[k]) <= 4.0 * np.maximum(nrow, 1) * 2.0 ** -53 * scale[k]))
             rec[f"k{k}"] = r
             del Xt, Y
         if M is not D:
