@@ -195,6 +195,25 @@ int mph_coo_multiply(int64_t nrows, int64_t ncols, int64_t nnz,
                      const int32_t* row_indices, const int32_t* column_indices,
                      const void* values, int value_type, const void* x,
                      void* y, int init, mph_stream_t stream);
+
+/* mph_coo_multiply without any allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Runs the
+ * deterministic wavefront-segmented COO kernel (8 entries per lane) with its
+ * per-chunk carries in a fixed device array of 8 MiB per GPU (held until the
+ * process ends), so it calls no hipMalloc / hipFree and never synchronises.
+ * Limits: nnz <= 268,435,456 (larger sizes return MPH_ERR_INVALID_INPUT with
+ * the limit in mph_last_error()); rows must be sorted and every index in
+ * range -- the preconditions of the reference HIP COO multiply
+ * (core/src/impl/Coo/HIP/Morpheus_Multiply_Impl.hpp:51-179), not checked
+ * here.  Every call on one GPU shares that GPU's array: calls that overlap on
+ * one GPU (other streams, other threads) race and must be serialised by the
+ * caller.  Same arguments and result as mph_coo_multiply (init = 1: y = A x,
+ * rows without entries 0; init = 0: y += A x). */
+int mph_coo_multiply_noalloc(int64_t nrows, int64_t ncols, int64_t nnz,
+                             const int32_t* row_indices, const int32_t* column_indices,
+                             const void* values, int value_type, const void* x, void* y,
+                             int init, mph_stream_t stream);
 int mph_dia_multiply(int64_t nrows, int64_t ncols, int64_t nnz, int64_t ndiag,
                      int64_t pitch, const int32_t* diagonal_offsets,
                      const void* values, int value_type, const void* x,

@@ -103,6 +103,7 @@ _SIGS = {
     "mph_multiply_multi": (I, [p, i32, p, p, I, p]),
     "mph_csr_multiply": (I, [i64, i64, i64, p, p, p, I, p, p, I, p]),
     "mph_coo_multiply": (I, [i64, i64, i64, p, p, p, I, p, p, I, p]),
+    "mph_coo_multiply_noalloc": (I, [i64, i64, i64, p, p, p, I, p, p, I, p]),
     "mph_dia_multiply": (I, [i64, i64, i64, i64, i64, p, p, I, p, p, I, p]),
     "mph_ell_multiply": (I, [i64, i64, i64, i64, i64, i64, p, p, I, p, p, I, p]),
     "mph_convert_csr_to_coo": (I, [i64, i64, p, p, p, I, p, p, p, p]),

@@ -234,7 +234,8 @@ int mph_initialize(int device) {
     return MPH_ERR_HIP;
   }
   device_cu_count();
-  return MPH_OK;
+  void* scratch = nullptr;  // the no-allocation entry point's scratch address
+  return noalloc_scratch(&scratch);
 }
 
 int mph_finalize(void) { return MPH_OK; }

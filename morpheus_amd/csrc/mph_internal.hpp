@@ -228,6 +228,9 @@ int xtile_multiply(mph_matrix_impl* A, const XTilePlan& xp, const void* x, void*
 // HYB / COO: row offsets of a row-sorted COO entry list (ro has nrows + 1 entries)
 int coo_row_offsets(const int32_t* ri, int64_t nnz, int64_t nrows, int32_t* ro, hipStream_t s);
 int default_chunk(int vt);
+// the device address of the current device's mph_coo_multiply_noalloc
+// scratch (spmv_coo.hip), looked up on first use per device
+int noalloc_scratch(void** out);
 
 }  // namespace mph
 

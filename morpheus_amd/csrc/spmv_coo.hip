@@ -27,6 +27,8 @@
 // Unsorted COO (valid for the reference Serial backend) falls back to an
 // atomic scatter, which is correct but not bitwise reproducible.
 #include <algorithm>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "mph_internal.hpp"
@@ -502,6 +504,79 @@ int coo_run(mph_matrix_impl* A, const void* x, void* y, int init, int64_t ld,
 
 }  // namespace
 
+// ---------------------------------------------------------- no allocation
+// The scratch of mph_coo_multiply_noalloc: the per-chunk carries (first and
+// last row partials, pairs in FP32) of the segmented kernel, in one
+// file-scope device array per GPU.  16 bytes per chunk of 64 * 8 entries in
+// every value type, so kNoAllocScratchBytes bounds the entries to
+// kNoAllocScratchBytes / 16 * 512 (268,435,456 for 8 MiB).
+constexpr size_t kNoAllocScratchBytes = 8u << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kNoAllocItems = 8;
+constexpr int kMaxDevices = 64;
+// device address of `scratch` on each GPU (looked up once per device: each
+// GPU holds its own copy of the array at its own address)
+static void* g_scratch_ptr[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+// Looks up (once per device, with that device current) and returns the
+// address of this device's copy of `scratch`.  Called by mph_initialize and,
+// for a device first used without it, by the entry point's first call there.
+int noalloc_scratch(void** out) {
+  int dev = 0;
+  MPH_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return runtime_error("device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    MPH_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return runtime_error("no device address for the no-allocation scratch");
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return MPH_OK;
+}
+
+template <typename V>
+int coo_noalloc_run(int64_t nrows, int64_t nnz, const int32_t* ri, const int32_t* ci,
+                    const void* v, const void* x, void* y, int init, void* scratch_ptr,
+                    hipStream_t s) {
+  using Acc = typename vtraits<V>::acc;
+  constexpr bool comp = std::is_same<Acc, float>::value;
+  constexpr int T = kNoAllocItems;
+  CooArgs<V> a{};
+  a.nrows = nrows;
+  a.nnz = nnz;
+  a.nchunks = ceil_div(nnz, (int64_t)kWave * T);
+  a.ri = ri;
+  a.ci = ci;
+  a.val = static_cast<const V*>(v);
+  a.x = static_cast<const typename vtraits<V>::vec*>(x);
+  a.y = static_cast<typename vtraits<V>::vec*>(y);
+  a.init = init;
+  a.gapfill = 0;  // y is cleared by a fill pass instead
+  a.vec_ok = (reinterpret_cast<uintptr_t>(ri) % 16 == 0) && (reinterpret_cast<uintptr_t>(ci) % 16 == 0);
+  // scratch: first partials of every chunk, then the last partials
+  const size_t half = (size_t)a.nchunks * (comp ? 2 : 1);
+  a.first_val = static_cast<Acc*>(scratch_ptr);
+  a.last_val = a.first_val + half;
+  a.chunks = nullptr;
+  a.ld = 1;
+  if (init && nrows > 0)
+    hipLaunchKernelGGL(coo_zero_kernel<V>, dim3((unsigned)ceil_div(nrows, 256)), dim3(256), 0, s,
+                       a.y, nrows, (int64_t)1);
+  if (nnz > 0) {
+    const dim3 block(256);
+    hipLaunchKernelGGL((coo_segmented_kernel<V, T, comp>),
+                       dim3((unsigned)ceil_div(a.nchunks, 256 / kWave)), block, 0, s, a);
+    hipLaunchKernelGGL((coo_fixup_kernel<V, T, comp>), dim3((unsigned)ceil_div(a.nchunks, 256)),
+                       block, 0, s, a);
+  }
+  MPH_HIP_CHECK(hipGetLastError());
+  return MPH_OK;
+}
+
 // k right-hand sides (row-major X / Y): one segmented pass per column, each
 // reading x and y with element stride k.
 int coo_multiply(mph_matrix_impl* A, int k, const void* x, void* y, int init,
@@ -524,3 +599,27 @@ int coo_multiply(mph_matrix_impl* A, int k, const void* x, void* y, int init,
 }
 
 }  // namespace mph
+
+extern "C" int mph_coo_multiply_noalloc(int64_t nrows, int64_t ncols, int64_t nnz,
+                                        const int32_t* ri, const int32_t* ci, const void* v,
+                                        int vt, const void* x, void* y, int init,
+                                        mph_stream_t stream) {
+  using namespace mph;
+  if (nrows < 0 || ncols < 0 || nnz < 0) return invalid("negative size");
+  if (vt != MPH_F64 && vt != MPH_F32 && vt != MPH_BF16) return invalid("unknown value type");
+  const int64_t max_nnz = (int64_t)(kNoAllocScratchBytes / 16) * kWave * kNoAllocItems;
+  if (nnz > max_nnz)
+    return invalid("mph_coo_multiply_noalloc: " + std::to_string(nnz) +
+                   " entries exceed its scratch limit of " + std::to_string(max_nnz));
+  if (nnz > INT32_MAX || nrows > INT32_MAX) return invalid("sizes beyond int32 indices");
+  if ((nnz > 0 && (!ri || !ci || !v)) || (nnz > 0 && !x) || (nrows > 0 && !y))
+    return invalid("null array");
+  void* sp = nullptr;
+  MPH_CHECK_STATUS(noalloc_scratch(&sp));
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  switch (vt) {
+    case MPH_F64: return coo_noalloc_run<double>(nrows, nnz, ri, ci, v, x, y, init ? 1 : 0, sp, s);
+    case MPH_F32: return coo_noalloc_run<float>(nrows, nnz, ri, ci, v, x, y, init ? 1 : 0, sp, s);
+    default: return coo_noalloc_run<bf16>(nrows, nnz, ri, ci, v, x, y, init ? 1 : 0, sp, s);
+  }
+}
