@@ -412,12 +412,19 @@ int mph_i32_stats(const int32_t* v, int64_t n, double mean, int32_t threshold,
 
 /* ------------------------------------------------------------ batched */
 /* Many matrices of uneven size and structure, each with k right-hand sides,
- * in one persistent launch over a work queue of (matrix, row block) tiles.
- * No reference counterpart (the reference multiplies one matrix by one
- * vector per call).  Matrices are given as handles; their formats may differ.
- * A COO member must have its entries sorted by row (its row offsets are
- * derived on the device and kept by the batch); an unsorted one is refused.
- * X[i] is ncols_i x k and Y[i] nrows_i x k, row-major. */
+ * in one launch over a list of (matrix, row block) tiles: by default one
+ * workgroup per tile, the hardware dispatcher handing the tiles out; the
+ * environment variable MPH_BATCH_SCHEDULE=1 (per-workgroup tile lists) or
+ * =0 (an atomic work queue) selects a persistent launch instead, read at
+ * mph_batch_create.  (FP32 / BF16 rows of more than 1024 entries are summed
+ * with Dot2 compensation in a second launch.)  No reference counterpart (the
+ * reference multiplies one matrix by one vector per call).  Matrices are
+ * given as handles; their formats may differ.  A COO member must have its
+ * entries sorted by row (its row offsets are derived on the device and kept
+ * by the batch); an unsorted one is refused.  X[i] is ncols_i x k and Y[i]
+ * nrows_i x k, row-major, aligned to min(16, k * element size) bytes.  Runs
+ * of one batch must not overlap (on several streams or threads): the queue
+ * form's counters are per batch. */
 typedef struct mph_batch_s* mph_batch_t;
 int mph_batch_create(mph_batch_t* out, int32_t nmat, const mph_matrix_t* mats,
                      int32_t k, const void* const* X, void* const* Y,
@@ -427,7 +434,8 @@ int mph_batch_destroy(mph_batch_t b);
 int mph_batch_run(mph_batch_t b, int init, mph_stream_t stream);
 /* Captures mph_batch_run into a hipGraph once and replays it. */
 int mph_batch_run_graph(mph_batch_t b, int init, mph_stream_t stream);
-/* Number of (matrix, row block) tiles and persistent workgroups. */
+/* Number of (matrix, row block) tiles and of workgroups of the persistent
+ * forms. */
 int mph_batch_info(mph_batch_t b, int64_t* ntiles, int32_t* nworkgroups);
 
 /* ------------------------------------------------------------ autotuner */

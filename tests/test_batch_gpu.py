@@ -94,6 +94,29 @@ def test_batch_accumulate(gpu):
             assert np.all(np.abs(Y[:, j] - ref) <= bound)
 
 
+@pytest.mark.parametrize("k", [1, 2, 4, 16])
+def test_batch_schedules_agree(gpu, k, monkeypatch):
+    """The three launch forms (one workgroup per tile, the default; per-
+    workgroup lists; the atomic queue), with and without the per-XCD matrix
+    affinity, compute every tile the same way: identical bits, each within
+    the FP64 bound."""
+    mats = small_batch(n=18, seed=13)
+    outs = {}
+    for sched, aff in (("2", "0"), ("1", "0"), ("0", "0"), ("1", "1"), ("0", "1")):
+        monkeypatch.setenv("MPH_BATCH_SCHEDULE", sched)
+        monkeypatch.setenv("MPH_BATCH_AFFINITY", aff)
+        Xs, o = run_batch(mats, k, "f64")
+        outs[(sched, aff)] = o[0]
+    ref = outs[("2", "0")]
+    for key, o in outs.items():
+        for a, b in zip(ref, o):
+            assert np.array_equal(a, b), key
+    for (A, _), X, Y in zip(mats, Xs, ref):
+        for j in range(k):
+            bound = fp64_bound(np.diff(A.row_offsets), oracle.absrow(A, X[:, j]))
+            assert np.all(np.abs(Y[:, j] - oracle.spmv(A, X[:, j])) <= bound)
+
+
 def test_batch_graph_replays_match_eager(gpu):
     """The captured graph is replayed several times with eager work and
     synchronisations in between; every replay equals the eager launch."""
