@@ -128,6 +128,13 @@ int mph_matrix_destroy(mph_matrix_t A);
  * sparsity structure in place. */
 int mph_matrix_analyse(mph_matrix_t A, mph_stream_t stream);
 int mph_matrix_format(mph_matrix_t A);
+/* Bytes of the arrays one product y = A x (k = 1; init as given) streams
+ * under A's analysed plan, each counted once: what the plan reads where it
+ * differs from the format's minimal representation (a HYB over its merged
+ * CSR copy, row-sorted COO through derived row offsets, the x-tiled copy of
+ * long rows and its partials, the split passes of HYB / HDC).  No reference
+ * counterpart: for roofline reporting.  Analyses A if needed. */
+int mph_matrix_plan_bytes(mph_matrix_t A, int init, int64_t* bytes);
 
 /* Launch-shape knobs the autotuner searches (0 = library default).
  *   rows_chunk   : nnz budget of a CSR row block (short rows) / long-row chunk
@@ -437,6 +444,17 @@ int mph_batch_run_graph(mph_batch_t b, int init, mph_stream_t stream);
 /* Number of (matrix, row block) tiles and of workgroups of the persistent
  * forms. */
 int mph_batch_info(mph_batch_t b, int64_t* ntiles, int32_t* nworkgroups);
+
+/* ------------------------------------------------------------ probes */
+/* Device rate probes for the benchmark's ceilings (no reference
+ * counterpart).  mph_probe_read: one pass of 16-byte non-temporal loads over
+ * nbytes of `data` (16-byte aligned), nothing written (`sink`, one double,
+ * is written only if the data sum to a NaN).  mph_probe_gather: for items
+ * j < n, columns[j] read in order and x[columns[j]] gathered, eight items per
+ * thread; out[t] (ceil(n / 8) doubles) = thread t's sum. */
+int mph_probe_read(const void* data, int64_t nbytes, void* sink, mph_stream_t stream);
+int mph_probe_gather(const int32_t* columns, int64_t n, const double* x, double* out,
+                     mph_stream_t stream);
 
 /* ------------------------------------------------------------ autotuner */
 /* Times each admissible launch shape of A on `stream` (A's own x/y shapes,

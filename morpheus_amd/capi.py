@@ -104,6 +104,9 @@ _SIGS = {
     "mph_csr_multiply": (I, [i64, i64, i64, p, p, p, I, p, p, I, p]),
     "mph_coo_multiply": (I, [i64, i64, i64, p, p, p, I, p, p, I, p]),
     "mph_coo_multiply_noalloc": (I, [i64, i64, i64, p, p, p, I, p, p, I, p]),
+    "mph_matrix_plan_bytes": (I, [p, I, pi64]),
+    "mph_probe_read": (I, [p, i64, p, p]),
+    "mph_probe_gather": (I, [p, i64, p, p, p]),
     "mph_dia_multiply": (I, [i64, i64, i64, i64, i64, p, p, I, p, p, I, p]),
     "mph_ell_multiply": (I, [i64, i64, i64, i64, i64, i64, p, p, I, p, p, I, p]),
     "mph_convert_csr_to_coo": (I, [i64, i64, p, p, p, I, p, p, p, p]),

@@ -200,6 +200,115 @@ int multiply_impl(mph_matrix_t A, int k, const void* x, void* y, int init,
   return invalid("unknown format");
 }
 
+// Bytes of the arrays one k = 1 product streams under the analysed plan,
+// each counted once (x once, y written once, or read and written when
+// accumulating or when a second pass adds to it).
+int64_t csr_plan_bytes(const mph_matrix_impl* A, const RowBlockPlan& p, int64_t list_rows,
+                       int64_t items) {
+  const int64_t si = 4, sv = (int64_t)value_size(A->vt);
+  const int64_t acc = 8;  // a partial: an FP64 sum, or an FP32 Dot2 pair
+  int64_t b = (list_rows + 1) * si;
+  const int64_t tiled = p.xt.rows > 0 ? p.xt.items : 0;
+  b += (items - tiled) * (si + sv);
+  if (p.xt.rows > 0) {
+    // tile-major copy (16-bit column + value per slot), the (tile, row)
+    // partials written and read back, the tiled rows' ids
+    b += tiled * (2 + sv) + 2 * (int64_t)p.xt.ntiles * p.xt.rows * acc + (int64_t)p.xt.rows * si;
+  }
+  if (p.split) b += list_rows * si;  // the list's row map
+  return b;
+}
+
+int plan_bytes(mph_matrix_t A, int init, int64_t* out) {
+  const int64_t si = 4, sv = (int64_t)value_size(A->vt), sx = (int64_t)vector_size(A->vt);
+  const int64_t x = A->ncols * sx, y = A->nrows * sx * (init ? 1 : 2);
+  int64_t b = 0;
+  auto dia_part = [&]() -> int64_t {
+    std::vector<int32_t> off((size_t)A->ndiag);
+    if (A->ndiag > 0 &&
+        hipMemcpy(off.data(), A->dia_off, sizeof(int32_t) * A->ndiag, hipMemcpyDeviceToHost) !=
+            hipSuccess)
+      return -1;
+    int64_t v = A->ndiag * si;
+    for (int32_t o : off)
+      v += std::max<int64_t>(0, std::min<int64_t>(A->nrows - std::max<int64_t>(0, -(int64_t)o),
+                                                  A->ncols - std::max<int64_t>(0, (int64_t)o))) * sv;
+    return v;
+  };
+  const RowBlockPlan& p = A->rb;
+  switch (A->format) {
+    case MPH_DIA: {
+      const int64_t d = dia_part();
+      if (d < 0) return runtime_error("DIA offsets unreadable");
+      b = d;
+      break;
+    }
+    case MPH_ELL:
+      b = A->ell_loop * A->nrows * (si + sv);
+      break;
+    case MPH_COO:
+      if (A->coo.rowplan) {
+        b = csr_plan_bytes(A, p, A->nrows, A->nnz);  // derived row offsets, no row indices
+      } else if (A->coo.sorted) {
+        const int64_t tiled = A->coo.xt.rows > 0 ? A->coo.xt.items : 0;
+        b = (A->nnz - tiled) * (2 * si + sv) + A->coo.nchunks * 2 * 8;
+        if (tiled) b += tiled * (2 + sv) + 2 * (int64_t)A->coo.xt.ntiles * A->coo.xt.rows * 8;
+        if (A->coo.prezero) b += A->nrows * sx;
+      } else {
+        b = A->nnz * (2 * si + sv) + A->nrows * sx;  // clear, then atomic adds
+      }
+      break;
+    case MPH_CSR:
+      b = csr_plan_bytes(A, p, A->nrows, A->nnz);
+      break;
+    case MPH_HYB:
+    case MPH_HDC: {
+      const bool hyb = A->format == MPH_HYB;
+      const int64_t tail = hyb ? A->coo_nnz : A->csr_nnz;
+      if (tail == 0) {  // the ELL / DIA part alone
+        if (hyb) {
+          b = A->ell_loop * A->nrows * (si + sv);
+        } else {
+          b = dia_part();
+          if (b < 0) return runtime_error("DIA offsets unreadable");
+        }
+        break;
+      }
+      if (hyb && p.m_ro) {  // merged copy: the CSR plan over it
+        int32_t mnnz = 0;
+        if (hipMemcpy(&mnnz, p.m_ro + A->nrows, sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess)
+          return runtime_error("merged row offsets unreadable");
+        b = csr_plan_bytes(A, p, A->nrows, mnnz);
+        break;
+      }
+      int64_t pre = 0;
+      if (hyb) {
+        pre = A->ell_loop * A->nrows * (si + sv);
+      } else {
+        pre = dia_part();
+        if (pre < 0) return runtime_error("DIA offsets unreadable");
+      }
+      if (p.split) {
+        // the ELL / DIA pass writes y; the tail rows' list reads and writes it again
+        int64_t listed = 0;
+        if (p.nsblocks > 0 &&
+            hipMemcpy(&listed, p.sblocks + p.nsblocks, sizeof(int32_t), hipMemcpyDeviceToHost) !=
+                hipSuccess)
+          return runtime_error("split plan unreadable");
+        listed = (int32_t)listed;
+        b = pre + csr_plan_bytes(A, p, listed, tail) + 2 * listed * sx;
+      } else {
+        b = pre + csr_plan_bytes(A, p, A->nrows, tail);
+      }
+      break;
+    }
+    default:
+      return invalid("unknown format");
+  }
+  *out = b + x + y;
+  return MPH_OK;
+}
+
 }  // namespace
 
 }  // namespace mph
@@ -383,6 +492,20 @@ int mph_matrix_analyse(mph_matrix_t A, mph_stream_t stream) {
 }
 
 int mph_matrix_format(mph_matrix_t A) { return A ? A->format : -1; }
+
+int mph_matrix_plan_bytes(mph_matrix_t A, int init, int64_t* bytes) {
+  if (!A || !bytes) return invalid("null argument");
+  int st = analyse(A, nullptr);
+  if (st == MPH_ERR_NOT_IMPLEMENTED && A->format == MPH_HYB) {
+    // unsorted tail: the ELL pass, then the COO tail by atomics
+    const int64_t si = 4, sv = (int64_t)value_size(A->vt), sx = (int64_t)vector_size(A->vt);
+    *bytes = A->ell_loop * A->nrows * (si + sv) + A->coo_nnz * (2 * si + sv) +
+             A->ncols * sx + A->nrows * sx * (init ? 2 : 3);
+    return MPH_OK;
+  }
+  MPH_CHECK_STATUS(st);
+  return plan_bytes(A, init ? 1 : 0, bytes);
+}
 
 int mph_matrix_set_tuning(mph_matrix_t A, const mph_tuning* t) {
   if (!A || !t) return invalid("null argument");
