@@ -54,6 +54,9 @@ def parse():
                          "(2M rows, power-law(1.2), mean 30), the large parity configs")
     ap.add_argument("--no-sweep", action="store_true",
                     help="skip the format x precision sweep (FP64 / FP32 / BF16 on S7, Z15, B9)")
+    ap.add_argument("--only", default="",
+                    help="run one section alone and print its JSON (profiling): batch, "
+                         "large (FP64 formats on S27 and Z12), large-S27, large-Z12, p2")
     ap.add_argument("--formats", default="",
                     help="comma-separated subset of the step's formats (profiling one "
                          "format at a time; the JSON line then covers that subset)")
@@ -170,7 +173,7 @@ def run_gpu(mats, vecs, order, steps, warmup):
 def cold_and_graph(mats, vecs, order, reps):
     """Per format, two more figures for the same product: the median over
     `reps` products each started after the 256 MB MALL is flushed (a 1 GiB
-    read sweep between products, outside the events: clean lines, so the
+    read probe pass between products, outside the events: clean lines, so the
     product does not also pay for write-backs), and the mean of `reps`
     back-to-back products replayed from one hipGraph (warm caches, no host
     launch gaps)."""
@@ -178,14 +181,14 @@ def cold_and_graph(mats, vecs, order, reps):
     import torch
     stream = torch.cuda.current_stream()
     scratch = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
-    sink = torch.empty((), dtype=torch.float64, device="cuda")
+    sink = torch.zeros(1, dtype=torch.float64, device="cuda")
     cold, graph = {}, {}
     for n in order:
         D, _ = mats[n]
         x, y = vecs[n]
         evs = []
         for r in range(reps):
-            torch.sum(scratch, dim=0, out=sink)
+            probe_read(scratch, scratch.numel() * 8, sink, stream)
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record(stream)
             D.multiply(x, y, True, stream)
@@ -210,12 +213,15 @@ def cold_and_graph(mats, vecs, order, reps):
     return cold, graph
 
 
-def run_large(reps=10):
+def run_large(reps=10, autotune=True, matrices=("S27", "Z12")):
     """FP64 products on the two large configurations (S27: 3D 27-point
     stencil on 160^3, 4.1M rows, 109M nnz; Z12: 2M rows, power-law(1.2) row
     lengths, mean 30, cap 200k), each format the reference admits: time
-    (median of `reps` back-to-back products), fraction of 8 TB/s and of a
-    same-size read stream, and the FP64 per-row bound against the oracle."""
+    (median of `reps` back-to-back products), fraction of 8 TB/s on the
+    algorithmic and on the plan's bytes and of a same-size read stream, the
+    FP64 per-row bound against the oracle, and the cost of switching: the
+    on-GPU Convert from CSR and the analysis of the new handle (wall clock,
+    synchronised, allocations included)."""
     import numpy as np
     import torch
     import oracle  # the oracle: FP64 reference products and row scales
@@ -225,6 +231,8 @@ def run_large(reps=10):
     stream = torch.cuda.current_stream()
     out = {}
     for mname, make in (("S27", gen.S27), ("Z12", gen.Z12)):
+        if mname not in matrices:
+            continue
         t0 = time.time()
         A = make()
         log(f"large: generated {mname} (nnz {A.nnz}) in {time.time() - t0:.1f}s")
@@ -235,13 +243,32 @@ def run_large(reps=10):
         y = torch.empty(A.nrows, dtype=torch.float64, device="cuda")
         base = device.to_device(A, "f64")
         out[mname] = {"nnz": int(A.nnz)}
+        torch.cuda.synchronize()
+        w0 = time.perf_counter()
+        base.analyse()
+        torch.cuda.synchronize()
+        csr_analyse_ms = (time.perf_counter() - w0) * 1e3
         for f, fname in enumerate(FORMAT_NAMES):
+            convert_ms = 0.0
             try:
-                D = base if f == base.format else device.convert(base, f)
+                if f == base.format:
+                    D = base
+                else:
+                    torch.cuda.synchronize()
+                    w0 = time.perf_counter()
+                    D = device.convert(base, f)
+                    torch.cuda.synchronize()
+                    convert_ms = (time.perf_counter() - w0) * 1e3
             except capi.FormatConversionException:
                 out[mname][fname] = "FormatConversionException"
                 continue
-            D.analyse()
+            if D is base:
+                analyse_ms = csr_analyse_ms
+            else:
+                w0 = time.perf_counter()
+                D.analyse()
+                torch.cuda.synchronize()
+                analyse_ms = (time.perf_counter() - w0) * 1e3
             D.multiply(xt, y, True, stream)
             torch.cuda.synchronize()
             ok = bool(np.all(np.abs(y.cpu().numpy() - ref) <= bound))
@@ -255,23 +282,31 @@ def run_large(reps=10):
                 ts.append(e0.elapsed_time(e1))
             ms = float(np.median(ts))
             nbytes = D.algorithmic_bytes()
+            pbytes = D.plan_bytes()
             rs = read_stream_flushed(nbytes, 5)
             out[mname][fname] = {
                 "ms": round(ms, 4),
                 "gflops": round(2.0 * A.nnz / (ms * 1e-3) / 1e9, 2),
                 "bytes_alg": int(nbytes),
                 "hbm_frac": round(data_gbs(nbytes, ms) / HBM_PEAK_GBS, 4),
+                "bytes_plan": int(pbytes),
+                "hbm_frac_plan": round(data_gbs(pbytes, ms) / HBM_PEAK_GBS, 4),
                 "read_stream_same_size_gbs": round(rs, 1),
                 "frac_of_read_stream": round(data_gbs(nbytes, ms) / rs, 4),
+                "convert_from_csr_ms": round(convert_ms, 3),
+                "analyse_ms": round(analyse_ms, 3),
                 "within_fp64_bound": ok,
             }
             log(f"large {mname} {fname}: {ms * 1e3:.0f} us, "
-                f"{out[mname][fname]['hbm_frac']:.2f} of 8 TB/s, "
-                f"{out[mname][fname]['frac_of_read_stream']:.2f} of a read stream, bound {ok}")
+                f"{out[mname][fname]['hbm_frac']:.2f} of 8 TB/s "
+                f"({out[mname][fname]['hbm_frac_plan']:.2f} on the plan's bytes), "
+                f"{out[mname][fname]['frac_of_read_stream']:.2f} of a read stream, bound {ok}; "
+                f"convert {convert_ms:.1f} ms, analyse {analyse_ms:.1f} ms")
             if D is not base:
                 D.close()
         base.close()
-        out[mname]["autotuned"] = run_large_autotuned(A, mname)
+        if autotune:
+            out[mname]["autotuned"] = run_large_autotuned(A, mname)
         del A
     return out
 
@@ -353,31 +388,71 @@ def run_large_autotuned(A, mname, ks=(1, 4, 16), reps=5):
     return res
 
 
+def probe_read(buf, nbytes, sink, stream):
+    """One pass of the library's read probe (16-byte non-temporal loads,
+    nothing written) over the first `nbytes` of `buf`."""
+    from morpheus_amd import capi, device
+    capi.call("mph_probe_read", buf.data_ptr(), int(nbytes), sink.data_ptr(),
+              device.stream_handle(stream))
+
+
 def read_stream_flushed(nbytes, reps=20):
     """The device's read rate at a given size, as the flushed products see
-    it: a one-pass sum over `nbytes` of FP64 data (torch's reduction kernel:
-    read once, nothing written but the result), median of `reps` runs each
-    after the same 1 GiB MALL-flushing sweep, outside the events.  GB/s."""
+    it: one pass of the library's read probe (a hand-written kernel, 16-byte
+    non-temporal loads, nothing written) over `nbytes`, median of `reps` runs
+    each after the same 1 GiB MALL-flushing read sweep, outside the events.
+    GB/s."""
     import numpy as np
     import torch
     stream = torch.cuda.current_stream()
     scratch = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
-    data = torch.ones(max(1, nbytes // 8), dtype=torch.float64, device="cuda")
-    sink = torch.empty((), dtype=torch.float64, device="cuda")
-    out = torch.empty((), dtype=torch.float64, device="cuda")
-    torch.sum(data, dim=0, out=out)
+    data = torch.ones(max(2, (nbytes + 15) // 8), dtype=torch.float64, device="cuda")
+    sink = torch.zeros(1, dtype=torch.float64, device="cuda")
+    probe_read(data, nbytes, sink, stream)
     evs = []
     for _ in range(reps):
-        torch.sum(scratch, dim=0, out=sink)
+        probe_read(scratch, scratch.numel() * 8, sink, stream)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
-        torch.sum(data, dim=0, out=out)
+        probe_read(data, nbytes, sink, stream)
         e1.record(stream)
         evs.append((e0, e1))
     torch.cuda.synchronize()
     ms = float(np.median([a.elapsed_time(b) for a, b in evs]))
     del scratch, data
     return data_gbs(nbytes, ms)
+
+
+def gather_ceiling(z15, reps=20):
+    """The x-gather floor of Z15's short-row pass: the library's gather probe
+    over the column indices of the rows of at most 512 entries (3.3M items;
+    the longer rows take the x-tiled pass) into an 8 MB x, index loads
+    coalesced, eight gathers in flight per thread, no row bookkeeping;
+    median of `reps` runs back to back (steady state).  Returns us and the
+    item count."""
+    import numpy as np
+    import torch
+    from morpheus_amd import capi, device
+    stream = torch.cuda.current_stream()
+    L = np.diff(z15.row_offsets)
+    keep = np.repeat(L <= 512, L)
+    ci = torch.from_numpy(np.ascontiguousarray(z15.column_indices[keep], dtype=np.int32)).cuda()
+    x = torch.empty(z15.ncols, dtype=torch.float64, device="cuda").uniform_(-1.0, 1.0)
+    n = ci.numel()
+    out = torch.empty((n + 7) // 8, dtype=torch.float64, device="cuda")
+    run = lambda: capi.call("mph_probe_gather", ci.data_ptr(), n, x.data_ptr(), out.data_ptr(),
+                            device.stream_handle(stream))
+    run()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        run()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    return float(np.median(ts)) * 1e3, int(n)
 
 
 def data_gbs(nbytes, ms):
@@ -462,10 +537,20 @@ def run_batch(steps):
     """Batched SpMM: 512 matrices (log-uniform 1k-64k rows; 2D 5-point
     stencils, power-law(1.5) rows with mean 12, banded with bandwidth 5-33),
     k = 8 row-major right-hand sides U[-1, 1]; formats chosen per matrix by
-    the autotuner; one persistent launch per product, replayed from a
-    hipGraph.  Compared with the FP64 forms in matrices*RHS/s."""
+    the autotuner; one launch per product, replayed from a hipGraph.
+
+    Per value type: time, algorithmic bytes (each member's stored arrays in
+    its chosen format read once, X read once, Y written once; and the same
+    at CSR-equivalent storage), GB/s and fraction of 8 TB/s, and the error of
+    column 0 of every member against the oracle (FP64: the per-row bound
+    4 n_row u sum_j |a_ij x_j|; FP32 / BF16: max_i |y_i - y_ref_i| /
+    sum_j |a_ij x_j| against the compensated FP64 oracle).  The primary
+    ratios are FP32 / BF16 against the same batched launch in FP64; the plain
+    FP64 forms (one SpMM per matrix, one SpMV per matrix and column, eager
+    and graph-replayed) are reported beside them."""
     import numpy as np
     import torch
+    import oracle  # the oracle: FP64 reference products and row scales
     from morpheus_amd import device
     from morpheus_amd import generators as gen
     from morpheus_amd.formats import FORMAT_NAMES
@@ -474,6 +559,7 @@ def run_batch(steps):
     t0 = time.time()
     mats = [A for _, A in gen.batch_set(512)]
     nnz = sum(A.nnz for A in mats)
+    rows = sum(A.nrows for A in mats)
     log(f"batch: generated {len(mats)} matrices, {nnz} nnz in {time.time() - t0:.1f}s")
     rng = np.random.default_rng(88)
     Xh = [rng.uniform(-1.0, 1.0, (A.ncols, K)) for A in mats]
@@ -490,7 +576,16 @@ def run_batch(steps):
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / reps
 
-    # FP64 CSR copies: the plain drop-in form and the reference for errors
+    # oracle column 0 of every member, and its row scales
+    t0 = time.time()
+    x0 = [np.ascontiguousarray(x[:, 0]) for x in Xh]
+    ref64 = [oracle.spmv(A, x) for A, x in zip(mats, x0)]
+    refc = [oracle.spmv_compensated(A, x) for A, x in zip(mats, x0)]
+    scale = [oracle.absrow(A, x) for A, x in zip(mats, x0)]
+    bound = [4.0 * np.maximum(np.diff(A.row_offsets), 1) * 2.0 ** -53 * sc
+             for A, sc in zip(mats, scale)]
+    log(f"batch: oracle column 0 of {len(mats)} members in {time.time() - t0:.1f}s")
+
     t0 = time.time()
     D64 = [device.to_device(A, "f64") for A in mats]
     X64 = [torch.from_numpy(x).cuda() for x in Xh]
@@ -502,8 +597,8 @@ def run_batch(steps):
     hist = {FORMAT_NAMES[f]: choice.count(f) for f in sorted(set(choice))}
     log(f"batch: autotuned formats {hist} in {time.time() - t0:.1f}s")
 
-    res = {"matrices": len(mats), "k": K, "nnz": int(nnz), "formats_chosen": hist}
-    Yref = None
+    res = {"matrices": len(mats), "k": K, "rows": int(rows), "nnz": int(nnz),
+           "formats_chosen": hist}
     for dtype in ("f64", "f32", "bf16"):
         base = D64 if dtype == "f64" else (Dcsr32 if dtype == "f32" else
                                             [device.to_device(A, "bf16") for A in mats])
@@ -512,41 +607,55 @@ def run_batch(steps):
         X = [x.to(vdt) for x in X64]
         Y = [torch.empty((D.nrows, K), dtype=vdt, device="cuda") for D in F]
         B = device.Batch(F, K, X, Y)
-        ntiles, nwg = B.info()
+        ntiles, ncomp = B.info()
         ms = timed(lambda: B.run_graph(True, stream), steps)
-        res[f"batched_{dtype}_ms"] = round(ms, 4)
-        res[f"batched_{dtype}_matrhs_per_s"] = round(len(mats) * K / (ms * 1e-3), 1)
+        nbytes = sum(D.algorithmic_bytes(K) for D in F)
+        csr_bytes = sum(D.algorithmic_bytes(K) for D in base)
+        gbs = data_gbs(nbytes, ms)
+        rec = {"ms": round(ms, 4),
+               "matrhs_per_s": round(len(mats) * K / (ms * 1e-3), 1),
+               "gflops": round(2.0 * nnz * K / (ms * 1e-3) / 1e9, 1),
+               "bytes_alg": int(nbytes), "gbs": round(gbs, 1),
+               "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+               "bytes_csr_equiv": int(csr_bytes),
+               "hbm_frac_csr_equiv": round(data_gbs(csr_bytes, ms) / HBM_PEAK_GBS, 4),
+               "tiles": int(ntiles), "dot2_rows": int(ncomp)}
+        worst, outside = 0.0, 0
+        for A, y, r64, rc, sc, bd in zip(mats, Y, ref64, refc, scale, bound):
+            y0 = y[:, 0].double().cpu().numpy()
+            if dtype == "f64":
+                outside += int(np.count_nonzero(~(np.abs(y0 - r64) <= bd)))
+            m = sc > 0
+            if m.any():
+                worst = max(worst, float(np.max(np.abs(y0[m] - rc[m]) / sc[m])))
+        rec["max_rel_err_col0"] = worst
         if dtype == "f64":
-            Yref = [y.clone() for y in Y]
-            res["tiles"], res["workgroups"] = ntiles, nwg
-        else:
-            # max-row relative error vs the FP64 product, scaled by sum |a||x|
-            err = 0.0
-            for D, y, yr, x in zip(D64, Y, Yref, X64):
-                absD = device.DeviceMatrix(D.format, D.nrows, D.ncols, D.nnz, "f64",
-                                           dict(ro=D.arrays["ro"], ci=D.arrays["ci"],
-                                                v=D.arrays["v"].abs()), {})
-                sc = torch.empty_like(yr)
-                absD.multiply_multi(x.abs(), sc)
-                e = ((y.double() - yr).abs() / sc.clamp_min(1e-300))[sc > 0]
-                if e.numel():
-                    err = max(err, float(e.max()))
-                absD.close()
-            res[f"batched_{dtype}_max_rel_err"] = err
+            rec["rows_outside_fp64_bound_col0"] = outside
+        res[dtype] = rec
         B.close()
-        log(f"batch {dtype}: {ms * 1e3:.1f} us per product of {len(mats)} matrices x {K} RHS")
+        for D, Db in zip(F, base):
+            if D is not Db:
+                D.close()
+        if dtype != "f64":
+            for D in base:
+                D.close()
+        log(f"batch {dtype}: {ms * 1e3:.1f} us per product of {len(mats)} matrices x {K} RHS, "
+            f"{gbs:.0f} GB/s ({rec['hbm_frac']:.3f} of 8 TB/s), max rel err {worst:.1e}")
+    for dtype in ("f32", "bf16"):
+        res[f"speedup_{dtype}_vs_batched_f64"] = round(
+            res[dtype]["matrhs_per_s"] / res["f64"]["matrhs_per_s"], 3)
+    res["tolerance"] = {"f32": 1e-5, "bf16": 1e-2}
     # plain FP64 GPU forms: one SpMM per matrix, and one SpMV per matrix and RHS
     Y64 = [torch.empty((D.nrows, K), dtype=torch.float64, device="cuda") for D in D64]
+    plain = {}
     ms = timed(lambda: [D.multiply_multi(x, y, True, stream) for D, x, y in zip(D64, X64, Y64)], 3)
-    res["per_matrix_spmm_f64_ms"] = round(ms, 4)
-    res["per_matrix_spmm_f64_matrhs_per_s"] = round(len(mats) * K / (ms * 1e-3), 1)
+    plain["per_matrix_spmm_ms"] = round(ms, 4)
     xcols = [[x[:, j].contiguous() for j in range(K)] for x in X64]
     ycols = [[torch.empty(D.nrows, dtype=torch.float64, device="cuda") for _ in range(K)]
              for D in D64]
     ms = timed(lambda: [D.multiply(xc[j], yc[j], True, stream)
                         for D, xc, yc in zip(D64, xcols, ycols) for j in range(K)], 2)
-    res["plain_f64_spmv_ms"] = round(ms, 4)
-    res["plain_f64_spmv_matrhs_per_s"] = round(len(mats) * K / (ms * 1e-3), 1)
+    plain["per_column_spmv_ms"] = round(ms, 4)
     # the same launches captured in hipGraphs (no host overhead per launch)
     g1, g2 = torch.cuda.CUDAGraph(), torch.cuda.CUDAGraph()
     torch.cuda.synchronize()
@@ -557,24 +666,20 @@ def run_batch(steps):
         for D, xc, yc in zip(D64, xcols, ycols):
             for j in range(K):
                 D.multiply(xc[j], yc[j], True)
-    ms = timed(g1.replay, 5)
-    res["per_matrix_spmm_f64_graph_ms"] = round(ms, 4)
-    res["per_matrix_spmm_f64_graph_matrhs_per_s"] = round(len(mats) * K / (ms * 1e-3), 1)
-    ms = timed(g2.replay, 5)
-    res["plain_f64_spmv_graph_ms"] = round(ms, 4)
-    res["plain_f64_spmv_graph_matrhs_per_s"] = round(len(mats) * K / (ms * 1e-3), 1)
+    plain["per_matrix_spmm_graph_ms"] = round(timed(g1.replay, 5), 4)
+    plain["per_column_spmv_graph_ms"] = round(timed(g2.replay, 5), 4)
     del g1, g2
-    # the target ratio is taken against the faster (graph-replayed) plain form
-    plain = max(res["plain_f64_spmv_matrhs_per_s"], res["plain_f64_spmv_graph_matrhs_per_s"])
-    res["plain_f64_reference"] = "plain_f64_spmv_graph" if plain == res[
-        "plain_f64_spmv_graph_matrhs_per_s"] else "plain_f64_spmv"
+    for D in D64:
+        D.close()
+    best = min(plain.values())
+    plain["fastest_matrhs_per_s"] = round(len(mats) * K / (best * 1e-3), 1)
+    res["plain_f64"] = plain
     for dtype in ("f32", "bf16"):
-        res[f"speedup_{dtype}_vs_plain_f64"] = round(res[f"batched_{dtype}_matrhs_per_s"] / plain, 2)
-        res[f"speedup_{dtype}_vs_batched_f64"] = round(
-            res[f"batched_{dtype}_matrhs_per_s"] / res["batched_f64_matrhs_per_s"], 3)
-    log(f"batch: plain FP64 {res['plain_f64_spmv_ms']:.2f} ms eager / "
-        f"{res['plain_f64_spmv_graph_ms']:.2f} ms graph, per-matrix SpMM FP64 "
-        f"{res['per_matrix_spmm_f64_ms']:.2f} ms eager / {res['per_matrix_spmm_f64_graph_ms']:.2f} ms graph")
+        res[f"speedup_{dtype}_vs_fastest_plain_f64"] = round(
+            res[dtype]["matrhs_per_s"] / plain["fastest_matrhs_per_s"], 3)
+    log(f"batch: FP32 {res['speedup_f32_vs_batched_f64']:.2f}x, BF16 "
+        f"{res['speedup_bf16_vs_batched_f64']:.2f}x the batched FP64 launch; plain FP64 forms "
+        + ", ".join(f"{k} {v:.2f}" for k, v in plain.items() if k.endswith("_ms")) + " ms")
     return res
 
 
@@ -656,6 +761,126 @@ def run_sweep(z15, s7, steps):
     return out
 
 
+def run_one_shot(z15, reps=20):
+    """The one-shot C-ABI multiply (mph_csr_multiply: a transient handle,
+    its analysis, the product, a stream synchronisation and the release, per
+    call) on Z15 in FP64, wall clock per call, median of `reps`, beside the
+    handle's product."""
+    import numpy as np
+    import torch
+    from morpheus_amd import capi, device
+    ro = torch.from_numpy(z15.row_offsets.astype(np.int32)).cuda()
+    ci = torch.from_numpy(z15.column_indices.astype(np.int32)).cuda()
+    v = torch.from_numpy(z15.values).cuda()
+    x = torch.empty(z15.ncols, dtype=torch.float64, device="cuda").uniform_(-1.0, 1.0)
+    y = torch.empty(z15.nrows, dtype=torch.float64, device="cuda")
+    h = device.stream_handle(torch.cuda.current_stream())
+
+    def call():
+        capi.call("mph_csr_multiply", z15.nrows, z15.ncols, z15.nnz, ro.data_ptr(),
+                  ci.data_ptr(), v.data_ptr(), 0, x.data_ptr(), y.data_ptr(), 1, h)
+
+    call()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        w0 = time.perf_counter()
+        call()
+        torch.cuda.synchronize()
+        ts.append((time.perf_counter() - w0) * 1e3)
+    ms = float(np.median(ts))
+    log(f"one-shot mph_csr_multiply on Z15: {ms:.3f} ms per call (analysis, product, sync, release)")
+    return {"ms_per_call": round(ms, 4), "reps": reps,
+            "note": "transient analysis + product + stream sync + release, wall clock"}
+
+
+def run_p2(reps=200):
+    """The CPU-reference configuration: the 2D 5-point Poisson matrix on a
+    256x256 grid (65,536 rows, 326,656 nnz, values 4 / -1) in FP64, x ~
+    U[-1, 1].  The OpenMP restatement of the reference CSR multiply on the
+    host cores and the Serial restatement of Convert CSR -> COO -> DIA -> CSR
+    (checked bit-exact), timed on the host; beside them the same product and
+    round trip through the library on the GPU (checked against the oracle:
+    bit-exact round trip, the FP64 per-row bound)."""
+    import numpy as np
+    import torch
+    import oracle  # the oracle: CPU-reference leg and checks
+    from morpheus_amd import device
+    from morpheus_amd import generators as gen
+    from morpheus_amd.formats import COO_FORMAT, CSR_FORMAT, DIA_FORMAT
+    A = gen.P2()
+    x = gen.random_vector(A.ncols, 7)
+    ref = oracle.spmv(A, x)
+    out = {"matrix": "P2 (2D 5-point Poisson, 256x256)", "nrows": int(A.nrows), "nnz": int(A.nnz)}
+    fn, _, _ = oracle.omp_spmv_fn(A)
+    for _ in range(5):
+        fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    t = float(np.median(ts))
+    out["cpu_omp_spmv"] = {"ms": round(t * 1e3, 4), "gflops": round(2.0 * A.nnz / t / 1e9, 3),
+                           "cores": oracle.omp_threads(), "reps": reps}
+    t0 = time.perf_counter()
+    back = oracle.convert(oracle.convert(oracle.to_coo(A), DIA_FORMAT), CSR_FORMAT)
+    rt = (time.perf_counter() - t0) * 1e3
+    same = bool(np.array_equal(back.row_offsets, A.row_offsets)
+                and np.array_equal(back.column_indices, A.column_indices)
+                and np.array_equal(back.values.view(np.uint64), A.values.view(np.uint64)))
+    out["cpu_serial_roundtrip_csr_coo_dia_csr"] = {"ms": round(rt, 3), "bit_exact": same}
+    # the GPU side of the same configuration
+    D = device.to_device(A, "f64")
+    xt = torch.from_numpy(x).cuda()
+    y = torch.empty(A.nrows, dtype=torch.float64, device="cuda")
+    D.analyse()
+    D.multiply(xt, y)
+    torch.cuda.synchronize()
+    yh = y.cpu().numpy()
+    bound = 4.0 * np.maximum(np.diff(A.row_offsets), 1) * 2.0 ** -53 * oracle.absrow(A, x)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        D.multiply(xt, y)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    C = device.convert(D, COO_FORMAT)
+    Dd = device.convert(C, DIA_FORMAT)
+    back = device.convert(Dd, CSR_FORMAT)
+    torch.cuda.synchronize()
+    grt = (time.perf_counter() - w0) * 1e3
+    bh = back.to_host()
+    gsame = bool(np.array_equal(bh.row_offsets, A.row_offsets)
+                 and np.array_equal(bh.column_indices, A.column_indices)
+                 and np.array_equal(bh.values.view(np.uint64), A.values.view(np.uint64)))
+    for M in (back, Dd, C, D):
+        M.close()
+    out["gpu_csr_spmv"] = {"ms": round(ms, 5), "gflops": round(2.0 * A.nnz / (ms * 1e-3) / 1e9, 2),
+                           "within_fp64_bound": bool(np.all(np.abs(yh - ref) <= bound))}
+    out["gpu_roundtrip_csr_coo_dia_csr"] = {"ms": round(grt, 3), "bit_exact": gsame}
+    log(f"P2: CPU OpenMP {out['cpu_omp_spmv']['gflops']:.2f} GFLOP/s on "
+        f"{out['cpu_omp_spmv']['cores']} threads, Serial round trip {rt:.1f} ms (bit-exact {same}); "
+        f"GPU {out['gpu_csr_spmv']['gflops']:.1f} GFLOP/s, round trip {grt:.1f} ms (bit-exact {gsame})")
+    return out
+
+
+def calibrate_copy():
+    """An elementwise kernel reading 1 GiB and writing 1 GiB, three times
+    (the known-size reference for the FETCH_SIZE / WRITE_SIZE counters; a
+    plain copy_ may be served by a DMA engine, invisible to kernel
+    counters)."""
+    import torch
+    a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -678,6 +903,21 @@ def main():
     from morpheus_amd import capi
 
     capi.call("mph_initialize", 0)
+    if args.only:
+        sec = args.only
+        if sec == "batch":
+            res = run_batch(args.batch_steps)
+        elif sec.startswith("large"):
+            res = run_large(autotune=False,
+                            matrices=("S27", "Z12") if sec == "large" else (sec.split("-")[1],))
+        elif sec == "p2":
+            res = run_p2()
+        else:
+            sys.exit(f"unknown section {sec}")
+        if args.calibrate_copy:
+            calibrate_copy()
+        print(json.dumps({"section": sec, "result": res}), flush=True)
+        return
     only = [f.strip().upper() for f in args.formats.split(",") if f.strip()]
     mats, vecs, order, ell_z15, (z15, s7), tuned = build_workload(only, not args.no_tune)
     total_ms, wall, per = run_gpu(mats, vecs, order, args.steps, args.warmup)
@@ -691,6 +931,7 @@ def main():
         t = float(np.mean(per[n]))
         bytes_alg = D.algorithmic_bytes()
         gbs = bytes_alg / (t * 1e-3) / 1e9
+        bytes_plan = D.plan_bytes()
         formats[n] = {
             "matrix": "Z15" if host is z15 else "S7",
             "nnz": int(host.nnz),
@@ -700,6 +941,8 @@ def main():
             "bytes_alg": int(bytes_alg),
             "gbs": round(gbs, 1),
             "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+            "bytes_plan": int(bytes_plan),
+            "hbm_frac_plan": round(data_gbs(bytes_plan, t) / HBM_PEAK_GBS, 4),
         }
         log(f"{n} on {formats[n]['matrix']}: {t * 1e3:.1f} us  {formats[n]['gflops']} GFLOP/s  "
             f"{gbs:.0f} GB/s  ({100 * gbs / HBM_PEAK_GBS:.1f}% of 8 TB/s)")
@@ -732,6 +975,8 @@ def main():
         "unit": "GB/s",
         "frac": round(formats[dom]["gbs"] / HBM_PEAK_GBS, 4),
         "traffic": None,
+        "bytes_alg": formats[dom]["bytes_alg"],
+        "bytes_plan": formats[dom]["bytes_plan"],
     }
     # HBM traffic per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE
     # passes on this workload (scaled by a copy-kernel calibration), committed
@@ -772,7 +1017,21 @@ def main():
         log(f"{n}: a {rec['bytes_alg'] / 1e6:.0f} MB read stream runs at {rs:.0f} GB/s; "
             f"the product (MALL flushed) at {rec['frac_of_read_stream_mall_flushed']:.2f} of it")
 
+    # ceilings: the read probe over 1 GiB (MALL flushed), and the x-gather
+    # floor of Z15's short rows
+    ceilings = {"read_1gib_gbs": round(read_stream_flushed(1 << 30, 10), 1)}
+    if "CSR" in order:
+        g_us, g_items = gather_ceiling(z15)
+        ceilings["gather_z15_short_rows"] = {
+            "us": round(g_us, 2), "items": g_items,
+            "note": ("coalesced int32 column loads + 8-byte x gathers from an 8 MB x, "
+                     "8 in flight per thread, no row bookkeeping")}
+        log(f"ceilings: 1 GiB read {ceilings['read_1gib_gbs']:.0f} GB/s; Z15 short-row gather "
+            f"floor {g_us:.1f} us over {g_items} items")
+        formats["CSR"]["one_shot"] = run_one_shot(z15)
+
     cg = run_cg(mats, 100)
+    p2 = run_p2()
     batch = None if args.no_batch else run_batch(args.batch_steps)
     sweep = None if (args.no_sweep or only) else run_sweep(z15, s7, 20)
     large = None if (args.no_large or only) else run_large()
@@ -793,13 +1052,7 @@ def main():
         }
 
     if args.calibrate_copy:
-        # an elementwise kernel reading 1 GiB and writing 1 GiB (a plain
-        # copy_ may be served by a DMA engine, invisible to kernel counters)
-        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
-        b = torch.empty_like(a)
-        for _ in range(3):
-            torch.mul(a, 1.0, out=b)
-        torch.cuda.synchronize()
+        calibrate_copy()
 
     line = {
         "metric": METRIC,
@@ -827,6 +1080,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "formats": formats,
+        "ceilings": ceilings,
+        "p2": p2,
         "batched_spmm": batch,
         "cg_iteration": cg,
         "precision_sweep": sweep,

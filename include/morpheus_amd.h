@@ -419,19 +419,23 @@ int mph_i32_stats(const int32_t* v, int64_t n, double mean, int32_t threshold,
 
 /* ------------------------------------------------------------ batched */
 /* Many matrices of uneven size and structure, each with k right-hand sides,
- * in one launch over a list of (matrix, row block) tiles: by default one
- * workgroup per tile, the hardware dispatcher handing the tiles out; the
- * environment variable MPH_BATCH_SCHEDULE=1 (per-workgroup tile lists) or
- * =0 (an atomic work queue) selects a persistent launch instead, read at
- * mph_batch_create.  (FP32 / BF16 rows of more than 1024 entries are summed
- * with Dot2 compensation in a second launch.)  No reference counterpart (the
- * reference multiplies one matrix by one vector per call).  Matrices are
- * given as handles; their formats may differ.  A COO member must have its
- * entries sorted by row (its row offsets are derived on the device and kept
- * by the batch); an unsorted one is refused.  X[i] is ncols_i x k and Y[i]
- * nrows_i x k, row-major, aligned to min(16, k * element size) bytes.  Runs
- * of one batch must not overlap (on several streams or threads): the queue
- * form's counters are per batch. */
+ * in one launch over a list of (matrix, row block) tiles, one workgroup per
+ * tile.  FP32 / BF16 rows of more than 1024 entries are summed with Dot2
+ * compensation by a second kernel on a stream of the batch's own, forked
+ * from and joined to the caller's stream (both kernels are captured by
+ * mph_batch_run_graph).  No reference counterpart (the reference multiplies
+ * one matrix by one vector per call).  Matrices are given as handles; their
+ * formats may differ.  A COO member must have its entries sorted by row (its
+ * row offsets are derived on the device and kept by the batch); an unsorted
+ * one is refused.  X[i] is ncols_i x k and Y[i] nrows_i x k, row-major,
+ * aligned to min(16, k * element size) bytes.  Runs of one batch must not
+ * overlap (on several streams or threads): they share the batch's second
+ * stream and its events.  Tile shapes, read at mph_batch_create from the
+ * environment (the result does not depend on them): MPH_BATCH_ROWS_PER_GROUP
+ * and MPH_BATCH_CSR_ROWS_PER_GROUP (4 by default, or 1: rows a lane group
+ * of a DIA / ELL resp. CSR-shaped row tile takes), MPH_BATCH_STAGE (1 by
+ * default: narrow DIA members stage X rows in LDS), MPH_BATCH_WAVE_ROWS (1
+ * by default: rows per wave of a one-wave tile). */
 typedef struct mph_batch_s* mph_batch_t;
 int mph_batch_create(mph_batch_t* out, int32_t nmat, const mph_matrix_t* mats,
                      int32_t k, const void* const* X, void* const* Y,
@@ -441,9 +445,8 @@ int mph_batch_destroy(mph_batch_t b);
 int mph_batch_run(mph_batch_t b, int init, mph_stream_t stream);
 /* Captures mph_batch_run into a hipGraph once and replays it. */
 int mph_batch_run_graph(mph_batch_t b, int init, mph_stream_t stream);
-/* Number of (matrix, row block) tiles and of workgroups of the persistent
- * forms. */
-int mph_batch_info(mph_batch_t b, int64_t* ntiles, int32_t* nworkgroups);
+/* Number of tiles of the tile kernel, and of rows of the Dot2 kernel. */
+int mph_batch_info(mph_batch_t b, int64_t* ntiles, int32_t* ndot2_rows);
 
 /* ------------------------------------------------------------ probes */
 /* Device rate probes for the benchmark's ceilings (no reference

@@ -148,6 +148,16 @@ class DeviceMatrix:
         vec = k * (self.ncols * sx + self.nrows * sx * (1 if init else 2))
         return int(mat + vec)
 
+    def plan_bytes(self, init: bool = True) -> int:
+        """Bytes the analysed plan of one k = 1 product moves (mph_matrix_plan_bytes):
+        what its kernels read of the stored arrays or of the analysis' copies
+        (the tiled copy of x-tiled long rows, HYB's merged copy), the
+        partials written and read back, x once, y once (twice when
+        accumulating); ELL padding past the last filled slot is not read."""
+        b = C.c_int64(0)
+        capi.call("mph_matrix_plan_bytes", self.handle, int(bool(init)), C.byref(b))
+        return int(b.value)
+
     # -------------------------------------------------------------- host
     def to_host(self):
         a, m = self.arrays, self.meta
@@ -512,6 +522,7 @@ class Batch:
         capi.call("mph_batch_run_graph", self.handle, int(bool(init)), stream_handle(stream))
 
     def info(self):
+        """(tiles of the tile kernel, rows of the Dot2 kernel)"""
         nt, nw = C.c_int64(), C.c_int32()
         capi.call("mph_batch_info", self.handle, C.byref(nt), C.byref(nw))
         return int(nt.value), int(nw.value)

@@ -94,27 +94,35 @@ def test_batch_accumulate(gpu):
             assert np.all(np.abs(Y[:, j] - ref) <= bound)
 
 
-@pytest.mark.parametrize("k", [1, 2, 4, 16])
-def test_batch_schedules_agree(gpu, k, monkeypatch):
-    """The three launch forms (one workgroup per tile, the default; per-
-    workgroup lists; the atomic queue), with and without the per-XCD matrix
-    affinity, compute every tile the same way: identical bits, each within
-    the FP64 bound."""
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+@pytest.mark.parametrize("k", [1, 2, 4, 8, 16])
+def test_batch_rows_per_group_agree(gpu, k, dtype, monkeypatch):
+    """Row tiles whose lane groups take 1 or 4 rows (8 or 2 entries of each
+    per batch), for DIA / ELL members and for CSR-shaped ones, with and
+    without the LDS-staged X rows of narrow DIA members, add every row's
+    entries in the same order: identical bits, and within the FP64 bound /
+    the FP32 tolerance of the oracle."""
     mats = small_batch(n=18, seed=13)
     outs = {}
-    for sched, aff in (("2", "0"), ("1", "0"), ("0", "0"), ("1", "1"), ("0", "1")):
-        monkeypatch.setenv("MPH_BATCH_SCHEDULE", sched)
-        monkeypatch.setenv("MPH_BATCH_AFFINITY", aff)
-        Xs, o = run_batch(mats, k, "f64")
-        outs[(sched, aff)] = o[0]
-    ref = outs[("2", "0")]
+    for rows, csr_rows, stage in (("4", "1", "1"), ("1", "1", "1"), ("4", "4", "1"),
+                                  ("1", "4", "1"), ("4", "1", "0"), ("1", "1", "0")):
+        monkeypatch.setenv("MPH_BATCH_ROWS_PER_GROUP", rows)
+        monkeypatch.setenv("MPH_BATCH_CSR_ROWS_PER_GROUP", csr_rows)
+        monkeypatch.setenv("MPH_BATCH_STAGE", stage)
+        Xs, o = run_batch(mats, k, dtype)
+        outs[(rows, csr_rows, stage)] = o[0]
+    ref = outs[("4", "1", "1")]
     for key, o in outs.items():
         for a, b in zip(ref, o):
             assert np.array_equal(a, b), key
     for (A, _), X, Y in zip(mats, Xs, ref):
         for j in range(k):
-            bound = fp64_bound(np.diff(A.row_offsets), oracle.absrow(A, X[:, j]))
-            assert np.all(np.abs(Y[:, j] - oracle.spmv(A, X[:, j])) <= bound)
+            scale = oracle.absrow(A, X[:, j])
+            if dtype == "f64":
+                bound = fp64_bound(np.diff(A.row_offsets), scale)
+                assert np.all(np.abs(Y[:, j] - oracle.spmv(A, X[:, j])) <= bound)
+            else:
+                assert max_rel_err(Y[:, j], oracle.spmv_compensated(A, X[:, j]), scale) <= 1e-5
 
 
 def test_batch_graph_replays_match_eager(gpu):
