@@ -794,6 +794,43 @@ def run_one_shot(z15, reps=20):
             "note": "transient analysis + product + stream sync + release, wall clock"}
 
 
+def run_no_copies(mats, vecs, order, reps=20):
+    """The products of the Morpheus drop-in containers: the same matrices
+    through handles with MPH_POLICY_NO_VALUE_COPIES (no x-tiled copy, no HYB
+    merged copy: every product reads the caller's value arrays), for the
+    formats whose default plan keeps a copy on these matrices; mean of `reps`
+    back-to-back products, and the plan's bytes."""
+    import numpy as np
+    import torch
+    from morpheus_amd import capi, device
+    stream = torch.cuda.current_stream()
+    out = {}
+    for n in order:
+        D, _ = mats[n]
+        if n not in ("CSR", "COO", "HYB"):
+            continue
+        M = device.DeviceMatrix(D.format, D.nrows, D.ncols, D.nnz, D.dtype, D.arrays, D.meta)
+        M.set_policy(capi.POLICY_NO_VALUE_COPIES)
+        M.analyse()
+        x, y = vecs[n]
+        M.multiply(x, y, True, stream)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            M.multiply(x, y, True, stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        pb = M.plan_bytes()
+        out[n] = {"ms": round(ms, 5), "bytes_plan": int(pb),
+                  "hbm_frac_alg": round(data_gbs(D.algorithmic_bytes(), ms) / HBM_PEAK_GBS, 4),
+                  "hbm_frac_plan": round(data_gbs(pb, ms) / HBM_PEAK_GBS, 4)}
+        log(f"{n} without value copies (drop-in policy): {ms * 1e3:.1f} us, plan bytes {pb / 1e6:.0f} MB")
+        M.close()
+    return out
+
+
 def run_p2(reps=200):
     """The CPU-reference configuration: the 2D 5-point Poisson matrix on a
     256x256 grid (65,536 rows, 326,656 nnz, values 4 / -1) in FP64, x ~
@@ -1029,6 +1066,8 @@ def main():
         log(f"ceilings: 1 GiB read {ceilings['read_1gib_gbs']:.0f} GB/s; Z15 short-row gather "
             f"floor {g_us:.1f} us over {g_items} items")
         formats["CSR"]["one_shot"] = run_one_shot(z15)
+    for n, rec in run_no_copies(mats, vecs, order).items():
+        formats[n]["no_value_copies"] = rec
 
     cg = run_cg(mats, 100)
     p2 = run_p2()

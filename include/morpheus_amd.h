@@ -125,8 +125,20 @@ int mph_matrix_create_hdc(mph_matrix_t* out, int64_t nrows, int64_t ncols,
 int mph_matrix_destroy(mph_matrix_t A);
 /* Builds the analysis now (multiply builds it lazily on first use).  Must be
  * called again (or the handle recreated) after the caller changes the
- * sparsity structure in place. */
+ * sparsity structure in place -- and, unless the handle's policy is
+ * MPH_POLICY_NO_VALUE_COPIES, after the caller writes values in place: by
+ * default the analysis may keep copies of values (the x-tiled long rows'
+ * tile-major copy, a HYB's merged CSR copy), which products then read. */
 int mph_matrix_analyse(mph_matrix_t A, mph_stream_t stream);
+
+/* Analysis policy (no reference counterpart: reference containers hold no
+ * derived state, core/src/Morpheus_CsrMatrix.hpp:148-163).
+ * MPH_POLICY_NO_VALUE_COPIES: the analysis keeps no copy of any value (no
+ * x-tiled pass, no HYB merged copy); every product reads the caller's value
+ * arrays, so values written in place are always seen.  The Morpheus drop-in
+ * containers set it.  Changing the policy releases the analysis. */
+#define MPH_POLICY_NO_VALUE_COPIES 1
+int mph_matrix_set_policy(mph_matrix_t A, int flags);
 int mph_matrix_format(mph_matrix_t A);
 /* Bytes of the arrays one product y = A x (k = 1; init as given) streams
  * under A's analysed plan, each counted once: what the plan reads where it

@@ -18,6 +18,9 @@ HEADER = HERE.parent / "include" / "morpheus_amd.h"
 # status codes (enum mph_status)
 OK, ERR_NOT_IMPLEMENTED, ERR_INVALID_INPUT, ERR_FORMAT_CONVERSION, ERR_HIP, \
     ERR_OUT_OF_MEMORY, ERR_RUNTIME = range(7)
+# analysis policy flags (mph_matrix_set_policy)
+POLICY_NO_VALUE_COPIES = 1
+
 # value types (enum mph_value_type)
 F64, F32, BF16 = 0, 1, 2
 VALUE_TYPES = {"f64": F64, "f32": F32, "bf16": BF16}
@@ -99,6 +102,7 @@ _SIGS = {
     "mph_matrix_format": (I, [p]),
     "mph_matrix_set_tuning": (I, [p, C.POINTER(Tuning)]),
     "mph_matrix_get_tuning": (I, [p, C.POINTER(Tuning)]),
+    "mph_matrix_set_policy": (I, [p, I]),
     "mph_multiply": (I, [p, p, p, I, p]),
     "mph_multiply_multi": (I, [p, i32, p, p, I, p]),
     "mph_csr_multiply": (I, [i64, i64, i64, p, p, p, I, p, p, I, p]),

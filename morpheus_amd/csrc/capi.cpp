@@ -526,6 +526,16 @@ int mph_matrix_set_tuning(mph_matrix_t A, const mph_tuning* t) {
   return MPH_OK;
 }
 
+int mph_matrix_set_policy(mph_matrix_t A, int flags) {
+  if (!A) return invalid("null matrix");
+  if (flags & ~MPH_POLICY_NO_VALUE_COPIES) return invalid("unknown policy flag");
+  if (flags != A->policy) {
+    A->policy = flags;
+    release_plans(A);
+  }
+  return MPH_OK;
+}
+
 int mph_matrix_get_tuning(mph_matrix_t A, mph_tuning* t) {
   if (!A || !t) return invalid("null argument");
   *t = A->tuning;

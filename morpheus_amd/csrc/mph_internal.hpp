@@ -166,6 +166,7 @@ struct mph_matrix_impl {
   const void* ell_v = nullptr;
 
   mph_tuning tuning{0, 0, 0, 0};
+  int32_t policy = 0;  // MPH_POLICY_* flags
   bool analysed = false;
   RowBlockPlan rb;
   CooPlan coo;
@@ -231,6 +232,12 @@ int default_chunk(int vt);
 // the device address of the current device's mph_coo_multiply_noalloc
 // scratch (spmv_coo.hip), looked up on first use per device
 int noalloc_scratch(void** out);
+
+// the handle's x-tiling mode (tuning.xtile: 0 auto, 1 off, 2 on); off when
+// its policy forbids value copies
+inline int xtile_mode(const mph_matrix_impl* A) {
+  return (A->policy & MPH_POLICY_NO_VALUE_COPIES) ? 1 : A->tuning.xtile;
+}
 
 }  // namespace mph
 

@@ -333,7 +333,7 @@ int coo_xtile_analyse(mph_matrix_impl* A, hipStream_t s) {
   CooPlan& p = A->coo;
   const int64_t nrows = A->nrows, nnz = A->nnz;
   const size_t vec_bytes = A->vt == MPH_F64 ? 8 : 4;
-  const int mode = A->tuning.xtile;
+  const int mode = xtile_mode(A);
   const bool on = mode == 2 || (mode == 0 && (int64_t)A->ncols * (int64_t)vec_bytes >= kXTileMinXBytes);
   if (!on || nnz == 0 || nrows == 0 || A->ncols == 0) return MPH_OK;
   int32_t* d_ro = nullptr;

@@ -1559,7 +1559,7 @@ int split_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
   RowBlockPlan& p = A->rb;
   const int64_t nrows = A->nrows;
   const size_t vec_bytes = A->vt == MPH_F64 ? 8 : 4;
-  const int mode = A->tuning.xtile;
+  const int mode = xtile_mode(A);
   const bool xt_on = mode == 2 || (mode == 0 && (int64_t)A->ncols * (int64_t)vec_bytes >= kXTileMinXBytes);
   // (an HDC without diagonals / a HYB without ELL slots is its CSR-shaped
   // part: no prefix pass)
@@ -1733,9 +1733,10 @@ int hyb_merge(mph_matrix_impl* A, hipStream_t s) {
   RowBlockPlan& p = A->rb;
   const int64_t nrows = A->nrows, loop = A->ell_loop;
   const size_t vec_bytes = A->vt == MPH_F64 ? 8 : 4;
-  const int mode = A->tuning.xtile;
+  const int mode = xtile_mode(A);
   const bool xt_on = mode == 2 || (mode == 0 && (int64_t)A->ncols * (int64_t)vec_bytes >= kXTileMinXBytes);
   if (A->coo_nnz == 0 || nrows == 0 || loop == 0) return MPH_OK;
+  if (A->policy & MPH_POLICY_NO_VALUE_COPIES) return MPH_OK;  // the ELL / tail passes instead
   if (!xt_on && nrows < kSplitMinRows) return MPH_OK;
   if (nrows * loop + A->coo_nnz >= (int64_t)INT32_MAX) return MPH_OK;  // int32 offsets
   int32_t* cnt = nullptr;

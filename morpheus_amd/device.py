@@ -101,6 +101,12 @@ class DeviceMatrix:
         t = capi.Tuning(rows_chunk, lanes, coo_items, rows_per_thread, xtile)
         capi.call("mph_matrix_set_tuning", self.handle, C.byref(t))
 
+    def set_policy(self, flags: int):
+        """mph_matrix_set_policy: capi.POLICY_NO_VALUE_COPIES keeps every
+        product on the caller's value arrays (no x-tiled copy, no HYB merged
+        copy)."""
+        capi.call("mph_matrix_set_policy", self.handle, int(flags))
+
     def get_tuning(self):
         t = capi.Tuning()
         capi.call("mph_matrix_get_tuning", self.handle, C.byref(t))

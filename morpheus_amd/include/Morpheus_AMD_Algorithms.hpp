@@ -14,8 +14,8 @@
 // Execution spaces.  Morpheus::HIP and Morpheus::Generic::HIP run the
 // MI355X kernels on HIPSpace containers.  Morpheus::Serial and
 // Morpheus::OpenMP on HostSpace containers run the same kernels: the matrix
-// is staged into HBM once (kept with the container's analysis until it is
-// modified) and x / y travel per call.  Conversion on the device succeeds for
+// structure is staged into HBM once (kept with the container's analysis
+// until it is modified), its values and x / y travel per call.  Conversion on the device succeeds for
 // every format pair (the reference HIP converters throw) and is bit-exact
 // with the reference Serial converters.
 //
@@ -182,7 +182,31 @@ inline mph_matrix_t create_handle(const M& A) {
                                c.ccolumn_indices().data(), c.cvalues().data(), vt);
   }
   check(st, "Morpheus::multiply");
+  // the analysis keeps no copy of any value: products always read the
+  // container's value arrays, so values written in place through a View
+  // kept from an earlier accessor call are seen by the next multiply (as
+  // with the reference's containers, which hold no derived state)
+  st = mph_matrix_set_policy(h, MPH_POLICY_NO_VALUE_COPIES);
+  if (st != MPH_OK) mph_matrix_destroy(h);
+  check(st, "Morpheus::multiply");
   return h;
+}
+
+// The value arrays of a host container copied into its staged device copy
+// (same format and shape).
+template <class S, class D>
+inline void copy_values(const S& src, D& dst) {
+  if constexpr (S::format_id == COO_FORMAT || S::format_id == CSR_FORMAT) {
+    copy_array(src.cvalues().const_view(), dst.values().view(), "values");
+  } else if constexpr (S::format_id == DIA_FORMAT || S::format_id == ELL_FORMAT) {
+    copy_dense_matrix(src.cvalues(), dst.values());
+  } else if constexpr (S::format_id == HYB_FORMAT) {
+    copy_values(src.cell(), dst.ell());
+    copy_values(src.ccoo(), dst.coo());
+  } else {
+    copy_values(src.cdia(), dst.dia());
+    copy_values(src.ccsr(), dst.csr());
+  }
 }
 
 template <class M>
@@ -196,7 +220,15 @@ template <class M>
 inline mph_matrix_t matrix_handle(const M& A) {
   PlanCache& pc = A.plan_cache();
   auto sig = signature(A);
-  if (pc.handle && !pc.stale && pc.signature == sig) return pc.handle;
+  if (pc.handle && !pc.stale && pc.signature == sig) {
+    if constexpr (!M::memory_space::is_device) {
+      // the host values may have been written through a kept reference: the
+      // staged copy takes them on every call (the structure stays cached)
+      using DM = device_type_of<M>;
+      copy_values(A, *std::static_pointer_cast<DM>(pc.device_copy));
+    }
+    return pc.handle;
+  }
   pc.reset();
   if constexpr (M::memory_space::is_device) {
     pc.handle = create_handle(A);

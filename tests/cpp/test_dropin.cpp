@@ -521,6 +521,122 @@ TEST(Multiply_PlanInvalidation_DeviceValues) {
   EXPECT_TRUE(within(y, dense_ref(c, x, nullptr)));
 }
 
+// Values written through a pointer kept from an accessor call made BEFORE
+// the first product (no accessor call afterwards, so the container is never
+// told): the analysis keeps no copy of any value, so the second product sees
+// them.  CSR with a 2000-entry row and x of 2.4 MB (x-tiled by default
+// through the C-ABI), and a 70,000-row HYB whose ELL slots are mostly
+// padding (merged into a CSR copy by default).
+TEST(Multiply_KeptPointer_DeviceValues) {
+  const size_t nr = 4, nc = 300000, nlong = 2000;
+  const size_t nnz = nlong + 3 * 5;
+  Csr<Host> A(nr, nc, nnz);
+  size_t k = 0;
+  A.row_offsets(0) = 0;
+  for (size_t r = 0; r < nr; r++) {
+    const size_t len = r == 0 ? nlong : 5;
+    for (size_t j = 0; j < len; j++, k++) {
+      A.column_indices(k) = (int)(j * (nc / len) + r);
+      A.values(k) = 1.0 + 0.001 * (double)k;
+    }
+    A.row_offsets(r + 1) = (int)k;
+  }
+  auto Ad = Morpheus::create_mirror_container<Dev>(A);
+  Morpheus::copy(A, Ad);
+  double* kept = Ad.values().data();  // kept across the products
+  const auto x = random_x(nc, 9);
+  Vec<Dev> xd(nc), yd(nr, 0.0);
+  Morpheus::copy(x, xd);
+  Morpheus::multiply<HIPExec>(Ad, xd, yd);
+  for (size_t q = 0; q < nnz; q++) A.values(q) *= -2.0;
+  const bool copied =
+      hipMemcpy(kept, A.cvalues().data(), sizeof(double) * nnz, hipMemcpyHostToDevice) == hipSuccess;
+  EXPECT_TRUE(copied);
+  Morpheus::multiply<HIPExec>(Ad, xd, yd);
+  Vec<Host> y(nr, 0.0);
+  Morpheus::copy(yd, y);
+  Coo<Host> c;
+  Morpheus::convert<Serial>(A, c);
+  EXPECT_TRUE(within(y, dense_ref(c, x, nullptr)));
+}
+
+TEST(Multiply_KeptPointer_HybValues) {
+  // 70,000 rows: row r has 1 + (r % 97 == 0 ? 60 : 0) entries, so the ELL
+  // part (entries_per_row from the reference's HYB split) is mostly padding
+  const size_t nr = 70000, nc = 70000;
+  std::vector<int> ri, ci;
+  std::vector<double> v;
+  for (size_t r = 0; r < nr; r++) {
+    const size_t len = 1 + (r % 97 == 0 ? 60 : 0);
+    for (size_t j = 0; j < len; j++) {
+      ri.push_back((int)r);
+      ci.push_back((int)((r + j * 1013) % nc));
+      v.push_back(1.0 + 0.0001 * (double)(ri.size() % 1000));
+    }
+  }
+  // rows sorted, columns sorted within each row
+  Coo<Host> A(nr, nc, ri.size());
+  {
+    std::vector<size_t> order(ri.size());
+    for (size_t i = 0; i < order.size(); i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+      return ri[a] != ri[b] ? ri[a] < ri[b] : ci[a] < ci[b];
+    });
+    for (size_t i = 0; i < order.size(); i++) {
+      A.row_indices(i) = ri[order[i]];
+      A.column_indices(i) = ci[order[i]];
+      A.values(i) = v[order[i]];
+    }
+  }
+  Hyb<Host> H;
+  Morpheus::convert<Serial>(A, H);
+  auto Hd = Morpheus::create_mirror_container<Dev>(H);
+  Morpheus::copy(H, Hd);
+  double* ell_kept = Hd.ell().values().data();
+  double* coo_kept = Hd.coo().values().data();
+  const auto x = random_x(nc, 5);
+  Vec<Dev> xd(nc), yd(nr, 0.0);
+  Morpheus::copy(x, xd);
+  Morpheus::multiply<HIPExec>(Hd, xd, yd);
+  // scale every value by 3 on the host copy and write it through the kept pointers
+  const size_t ell_n = H.cell().cvalues().nrows() * H.cell().cvalues().ncols();
+  for (size_t i = 0; i < H.cell().cvalues().nrows(); i++)
+    for (size_t j = 0; j < H.cell().cvalues().ncols(); j++) H.ell().values(i, j) *= 3.0;
+  for (size_t q = 0; q < H.ccoo().nnnz(); q++) H.coo().values(q) *= 3.0;
+  // (through a device mirror of the scaled matrix: the device layout of the
+  // ELL values is the one the kept pointer addresses)
+  auto Hs = Morpheus::create_mirror_container<Dev>(H);
+  Morpheus::copy(H, Hs);
+  bool ok = hipMemcpy(ell_kept, Hs.cell().cvalues().data(), sizeof(double) * ell_n,
+                      hipMemcpyDeviceToDevice) == hipSuccess;
+  ok = ok && (H.ccoo().nnnz() == 0 ||
+              hipMemcpy(coo_kept, Hs.ccoo().cvalues().data(), sizeof(double) * H.ccoo().nnnz(),
+                        hipMemcpyDeviceToDevice) == hipSuccess);
+  EXPECT_TRUE(ok);
+  Morpheus::multiply<HIPExec>(Hd, xd, yd);
+  Vec<Host> y(nr, 0.0);
+  Morpheus::copy(yd, y);
+  Coo<Host> c;
+  Morpheus::convert<Serial>(H, c);
+  EXPECT_TRUE(within(y, dense_ref(c, x, nullptr)));
+}
+
+// A host container written through a reference kept from before the first
+// product: the staged device copy takes the values on every call.
+TEST(Multiply_KeptReference_HostValues) {
+  auto A = small_csr();
+  double& kept = A.values(0);
+  const auto x = random_x(A.ncols(), 3);
+  Vec<Host> y(A.nrows());
+  Morpheus::multiply<Serial>(A, x, y);
+  kept = 100.0;
+  Coo<Host> c;
+  Morpheus::convert<Serial>(A, c);
+  const Ref ref = dense_ref(c, x, nullptr);
+  Morpheus::multiply<Serial>(A, x, y);
+  EXPECT_TRUE(within(y, ref));
+}
+
 // ============================================================ convert
 template <int S, int D, class ExecSpace>
 static void convert_pair() {
