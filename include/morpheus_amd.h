@@ -47,7 +47,12 @@ enum mph_status {
   MPH_ERR_RUNTIME = 6            /* Morpheus::RuntimeException          */
 };
 
-enum mph_value_type { MPH_F64 = 0, MPH_F32 = 1, MPH_BF16 = 2 };
+/* MPH_I32: 32-bit integer values (Morpheus containers of int).  Accepted by
+ * the conversions, the COO sort and mph_values_cast, which move values as
+ * raw words (the zero test of DIA -> COO is on every bit); not by the
+ * products, which compute in FP64 / FP32 (the drop-in multiplies int
+ * containers exactly through FP64, see mph_values_cast). */
+enum mph_value_type { MPH_F64 = 0, MPH_F32 = 1, MPH_BF16 = 2, MPH_I32 = 3 };
 
 /* Same order as Morpheus::formats_e (core/src/Morpheus_FormatsRegistry.hpp:56-64) */
 enum mph_format {
@@ -459,6 +464,20 @@ int mph_batch_run(mph_batch_t b, int init, mph_stream_t stream);
 int mph_batch_run_graph(mph_batch_t b, int init, mph_stream_t stream);
 /* Number of tiles of the tile kernel, and of rows of the Dot2 kernel. */
 int mph_batch_info(mph_batch_t b, int64_t* ntiles, int32_t* ndot2_rows);
+
+/* ------------------------------------------------------------ casts */
+/* Element casts for containers whose index / value types the kernels do not
+ * take (core/src/Morpheus_ContainerTraits.hpp:59-84 admits any integral index
+ * type and any arithmetic value type; the reference's tests use long long
+ * indices and int values, core/tests/utils/Utils.hpp:50-60).
+ * mph_index_narrow: int64 -> int32, MPH_ERR_INVALID_INPUT when an entry does
+ * not fit (synchronises the stream).  mph_index_widen: int32 -> int64.
+ * mph_values_cast: between MPH_F64, MPH_F32 and MPH_I32 (floating point to
+ * integer rounds to nearest).  No reference counterpart. */
+int mph_index_narrow(const int64_t* src, int64_t n, int32_t* dst, mph_stream_t stream);
+int mph_index_widen(const int32_t* src, int64_t n, int64_t* dst, mph_stream_t stream);
+int mph_values_cast(const void* src, int src_type, void* dst, int dst_type, int64_t n,
+                    mph_stream_t stream);
 
 /* ------------------------------------------------------------ probes */
 /* Device rate probes for the benchmark's ceilings (no reference

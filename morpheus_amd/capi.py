@@ -22,7 +22,7 @@ OK, ERR_NOT_IMPLEMENTED, ERR_INVALID_INPUT, ERR_FORMAT_CONVERSION, ERR_HIP, \
 POLICY_NO_VALUE_COPIES = 1
 
 # value types (enum mph_value_type)
-F64, F32, BF16 = 0, 1, 2
+F64, F32, BF16, I32 = 0, 1, 2, 3
 VALUE_TYPES = {"f64": F64, "f32": F32, "bf16": BF16}
 
 
@@ -103,6 +103,9 @@ _SIGS = {
     "mph_matrix_set_tuning": (I, [p, C.POINTER(Tuning)]),
     "mph_matrix_get_tuning": (I, [p, C.POINTER(Tuning)]),
     "mph_matrix_set_policy": (I, [p, I]),
+    "mph_index_narrow": (I, [p, i64, p, p]),
+    "mph_index_widen": (I, [p, i64, p, p]),
+    "mph_values_cast": (I, [p, I, p, I, i64, p]),
     "mph_multiply": (I, [p, p, p, I, p]),
     "mph_multiply_multi": (I, [p, i32, p, p, I, p]),
     "mph_csr_multiply": (I, [i64, i64, i64, p, p, p, I, p, p, I, p]),

@@ -355,17 +355,20 @@ int finish_coo(int64_t nnz, int64_t nrows, int64_t ncols, int32_t* ri, int32_t* 
 // DIA part: flags over (diagonal d, row r) in diagonal-major order.
 // (row r, diagonal d) lies on the stored diagonal iff its column is inside
 // [0, ncols): rows i_start + n, n < N of :97-101
+// (full_mask: integer values, where every bit counts; floating-point values
+// drop the sign bit, so that -0.0 is a zero too)
 template <typename W>
 __global__ void dia_flag_kernel(int64_t nrows, int64_t ncols, int64_t ndiag, int64_t pitch,
                                 const int32_t* __restrict__ off, const void* __restrict__ val_,
-                                int32_t* __restrict__ flag) {
+                                int32_t* __restrict__ flag, int full_mask) {
   const W* val = static_cast<const W*>(val_);
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nrows * ndiag) return;
   const int64_t d = t / nrows, r = t % nrows;
   const int64_t c = r + off[d];
   const bool in = c >= 0 && c < ncols;
-  flag[t] = (in && (val[r + pitch * d] & RawT<sizeof(W)>::mag) != 0) ? 1 : 0;
+  const W mag = full_mask ? (W)~(W)0 : RawT<sizeof(W)>::mag;
+  flag[t] = (in && (val[r + pitch * d] & mag) != 0) ? 1 : 0;
 }
 
 template <typename W>
@@ -635,8 +638,10 @@ __global__ void hdc_sizes_kernel(const int32_t* __restrict__ cnt, int64_t n, int
       hipLaunchKernelGGL(kernel<uint16_t>, grid, dim3(kT), 0, s, __VA_ARGS__);      \
   } while (0)
 
+// (conversion moves values as raw words: 32-bit integer values too)
 int check_vt(int vt) {
-  if (vt != MPH_F64 && vt != MPH_F32 && vt != MPH_BF16) return invalid("unknown value type");
+  if (vt != MPH_F64 && vt != MPH_F32 && vt != MPH_BF16 && vt != MPH_I32)
+    return invalid("unknown value type");
   return MPH_OK;
 }
 
@@ -682,8 +687,9 @@ int row_counts(const int32_t* ri, int64_t nnz, int64_t nrows, Tmp& t, int32_t** 
 
 // DIA-part emission shared by DIA -> COO and HDC -> COO; returns the count
 int emit_dia_part(int64_t nrows, int64_t ncols, int64_t ndiag, int64_t pitch,
-                  const int32_t* off, const void* val, int vsize, int64_t cap, int32_t* ri,
+                  const int32_t* off, const void* val, int vt, int64_t cap, int32_t* ri,
                   int32_t* ci, void* v, int64_t* count, hipStream_t s) {
+  const int vsize = (int)value_size(vt);
   *count = 0;
   const int64_t T = nrows * ndiag;
   if (T == 0) return MPH_OK;
@@ -694,7 +700,7 @@ int emit_dia_part(int64_t nrows, int64_t ncols, int64_t ndiag, int64_t pitch,
   MPH_CHECK_STATUS(t.alloc(&pos, T + 1));
   MPH_CHECK_STATUS(t.alloc(&tmp, scan_tmp_elems(T)));
   MPH_LAUNCH_W(dia_flag_kernel, vsize, dim3(blocks_for(T)), s, nrows, ncols, ndiag, pitch, off,
-               val, flag);
+               val, flag, vt == MPH_I32 ? 1 : 0);
   MPH_HIP_CHECK(hipGetLastError());
   MPH_CHECK_STATUS(exclusive_scan_i32(flag, pos, T, tmp, s));
   int32_t tot = 0;
@@ -859,7 +865,7 @@ int mph_convert_dia_to_coo(int64_t nrows, int64_t ncols, int64_t nnz, int64_t nd
   if (ndiag > 0 && pitch < nrows) return invalid("pitch must be >= nrows");
   const int vs = (int)value_size(vt);
   int64_t cnt = 0;
-  MPH_CHECK_STATUS(emit_dia_part(nrows, ncols, ndiag, pitch, off, val, vs, nnz, ri_out, ci_out,
+  MPH_CHECK_STATUS(emit_dia_part(nrows, ncols, ndiag, pitch, off, val, vt, nnz, ri_out, ci_out,
                                  v_out, &cnt, s));
   MPH_CHECK_STATUS(zero_tail(cnt, nnz, ri_out, ci_out, v_out, vs, s));
   return finish_coo(nnz, nrows, ncols, ri_out, ci_out, v_out, vs, s);
@@ -1214,7 +1220,7 @@ int mph_convert_hdc_to_coo(int64_t nrows, int64_t ncols, int64_t nnz, int64_t nd
   MPH_CHECK_STATUS(check_row_offsets(csr_ro, nrows, csr_nnz, s, "HDC CSR row_offsets"));
   const int vs = (int)value_size(vt);
   int64_t nd = 0;
-  MPH_CHECK_STATUS(emit_dia_part(nrows, ncols, ndiag, pitch, doff, dval, vs, nnz, ri_out, ci_out,
+  MPH_CHECK_STATUS(emit_dia_part(nrows, ncols, ndiag, pitch, doff, dval, vt, nnz, ri_out, ci_out,
                                  v_out, &nd, s));
   if (nd + csr_nnz > nnz) return invalid("HDC holds more entries than its nnz");
   if (csr_nnz > 0) {

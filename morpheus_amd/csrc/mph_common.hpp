@@ -51,7 +51,7 @@ struct vtraits<bf16> {
 };
 
 inline size_t value_size(int vt) {
-  return vt == MPH_F64 ? 8 : vt == MPH_F32 ? 4 : 2;
+  return vt == MPH_F64 ? 8 : (vt == MPH_F32 || vt == MPH_I32) ? 4 : 2;
 }
 inline size_t vector_size(int vt) { return vt == MPH_F64 ? 8 : 4; }
 

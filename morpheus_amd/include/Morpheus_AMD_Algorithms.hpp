@@ -111,6 +111,121 @@ inline void copy_sparse(const S& src, D& dst) {
   dst.touch();
 }
 
+// ------------------------------------------------------------ int64 / int staging
+// The kernels take int32 indices and FP64 / FP32 / BF16 values.  A
+// long long-indexed container reaches them through int32 copies of its index
+// arrays (every index and size below 2^31, checked), an int-valued one
+// through FP64 copies of its values, recast before every use (exact: int32
+// values are FP64 numbers, and so are the products and sums while they stay
+// below 2^53 in magnitude).
+
+// device scratch, released with its owner
+struct DevBuf {
+  void* p = nullptr;
+  explicit DevBuf(size_t bytes) {
+    if (bytes) hip_check(hipMalloc(&p, bytes), "Morpheus: device scratch");
+  }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+};
+
+template <class I>
+inline constexpr bool wide_index_v = std::is_same_v<std::remove_const_t<I>, long long>;
+
+// The buffers a handle is built over, and the value casts around its use.
+struct Stage {
+  std::vector<std::shared_ptr<DevBuf>> bufs;
+  std::vector<std::function<void()>> refresh, write_back;
+};
+
+// int32 view of n device indices of type I (the array itself for int)
+template <class I>
+inline const int* stage_index(Stage& st, const I* p, size_t n) {
+  if constexpr (wide_index_v<I>) {
+    auto b = std::make_shared<DevBuf>(sizeof(int32_t) * std::max<size_t>(n, 1));
+    check(mph_index_narrow(reinterpret_cast<const int64_t*>(p), (int64_t)n,
+                           static_cast<int32_t*>(b->p), nullptr),
+          "Morpheus: indices");
+    st.bufs.push_back(b);
+    return static_cast<const int*>(b->p);
+  } else {
+    return p;
+  }
+}
+
+// The values the kernels read for n device values of type V (FP64 copies
+// for int, recast by refresh and cast back by write_back)
+template <class V>
+inline const void* stage_values(Stage& st, const V* p, size_t n) {
+  if constexpr (std::is_same_v<std::remove_const_t<V>, int>) {
+    auto b = std::make_shared<DevBuf>(sizeof(double) * std::max<size_t>(n, 1));
+    st.bufs.push_back(b);
+    double* d = static_cast<double*>(b->p);
+    st.refresh.push_back([p, d, n]() {
+      check(mph_values_cast(p, MPH_I32, d, MPH_F64, (int64_t)n, nullptr), "Morpheus: values");
+    });
+    st.write_back.push_back([p, d, n]() {
+      check(mph_values_cast(d, MPH_F64, const_cast<V*>(p), MPH_I32, (int64_t)n, nullptr),
+            "Morpheus: values");
+    });
+    return d;
+  } else {
+    return p;
+  }
+}
+
+// the value type a handle computes in
+template <class V>
+inline constexpr int handle_code_v = std::is_same_v<V, int> ? MPH_F64 : value_code<V>::value;
+
+// An int32 device array that the kernels write n entries of, widened into
+// dst by done() (dst itself for int).  in_place: dst's entries are read too.
+template <class I>
+struct IdxOut {
+  int* p = nullptr;
+  I* dst;
+  size_t n;
+  std::shared_ptr<DevBuf> buf;
+  IdxOut(I* d, size_t count, bool in_place = false) : dst(d), n(count) {
+    if constexpr (wide_index_v<I>) {
+      buf = std::make_shared<DevBuf>(sizeof(int32_t) * std::max<size_t>(n, 1));
+      p = static_cast<int*>(buf->p);
+      if (in_place)
+        check(mph_index_narrow(reinterpret_cast<const int64_t*>(dst), (int64_t)n, p, nullptr),
+              "Morpheus: indices");
+    } else {
+      p = d;
+    }
+  }
+  void done() {
+    if constexpr (wide_index_v<I>)
+      check(mph_index_widen(p, (int64_t)n, reinterpret_cast<int64_t*>(dst), nullptr),
+            "Morpheus: indices");
+  }
+};
+
+template <class I>
+inline IdxOut<I> idx_out(I* d, size_t n, bool in_place = false) {
+  return IdxOut<I>(d, n, in_place);
+}
+
+// n entries of a device vector of V as FP64 (int) or as itself: fn(pointer)
+// runs on them; `in`: the entries are read, `out`: written back
+template <class V, class Fn>
+inline void as_compute_vector(V* p, size_t n, bool in, bool out, Fn&& fn) {
+  if constexpr (std::is_same_v<V, int>) {
+    DevBuf d(sizeof(double) * std::max<size_t>(n, 1));
+    if (in) check(mph_values_cast(p, MPH_I32, d.p, MPH_F64, (int64_t)n, nullptr), "Morpheus: vector");
+    fn(d.p);
+    if (out) check(mph_values_cast(d.p, MPH_F64, p, MPH_I32, (int64_t)n, nullptr), "Morpheus: vector");
+  } else {
+    fn(static_cast<void*>(p));
+  }
+}
+
 // ------------------------------------------------------------ handles
 template <class M>
 inline std::vector<const void*> signature(const M& A) {
@@ -143,52 +258,71 @@ inline std::vector<const void*> signature(const M& A) {
   return s;
 }
 
-// C-ABI handle over the arrays of a HIPSpace container.
+// C-ABI handle over the arrays of a HIPSpace container (through the int32
+// / FP64 copies of `st` for long long indices / int values).
 template <class M>
-inline mph_matrix_t create_handle(const M& A) {
+inline mph_matrix_t create_handle(const M& A, Stage& st) {
   static_assert(M::memory_space::is_device, "handle over host arrays");
-  constexpr int vt = value_code<typename M::non_const_value_type>::value;
+  using V = typename M::non_const_value_type;
+  constexpr int vt = handle_code_v<V>;
   const int64_t nr = (int64_t)A.nrows(), nc = (int64_t)A.ncols();
+  auto dm_size = [](const auto& d) { return (size_t)d.nrows() * d.ncols(); };
   mph_matrix_t h = nullptr;
-  int st = MPH_OK;
+  int status = MPH_OK;
   if constexpr (M::format_id == COO_FORMAT) {
-    st = mph_matrix_create_coo(&h, nr, nc, (int64_t)A.nnnz(), A.crow_indices().data(),
-                               A.ccolumn_indices().data(), A.cvalues().data(), vt);
+    status = mph_matrix_create_coo(&h, nr, nc, (int64_t)A.nnnz(),
+                                   stage_index(st, A.crow_indices().data(), A.nnnz()),
+                                   stage_index(st, A.ccolumn_indices().data(), A.nnnz()),
+                                   stage_values(st, A.cvalues().data(), A.nnnz()), vt);
   } else if constexpr (M::format_id == CSR_FORMAT) {
-    st = mph_matrix_create_csr(&h, nr, nc, (int64_t)A.nnnz(), A.crow_offsets().data(),
-                               A.ccolumn_indices().data(), A.cvalues().data(), vt);
+    status = mph_matrix_create_csr(&h, nr, nc, (int64_t)A.nnnz(),
+                                   stage_index(st, A.crow_offsets().data(), A.nrows() + 1),
+                                   stage_index(st, A.ccolumn_indices().data(), A.nnnz()),
+                                   stage_values(st, A.cvalues().data(), A.nnnz()), vt);
   } else if constexpr (M::format_id == DIA_FORMAT) {
-    st = mph_matrix_create_dia(&h, nr, nc, (int64_t)A.nnnz(),
-                               (int64_t)A.cdiagonal_offsets().size(), (int64_t)A.cvalues().nrows(),
-                               A.cdiagonal_offsets().data(), A.cvalues().data(), vt);
+    status = mph_matrix_create_dia(
+        &h, nr, nc, (int64_t)A.nnnz(), (int64_t)A.cdiagonal_offsets().size(),
+        (int64_t)A.cvalues().nrows(),
+        stage_index(st, A.cdiagonal_offsets().data(), A.cdiagonal_offsets().size()),
+        stage_values(st, A.cvalues().data(), dm_size(A.cvalues())), vt);
   } else if constexpr (M::format_id == ELL_FORMAT) {
-    st = mph_matrix_create_ell(&h, nr, nc, (int64_t)A.nnnz(), (int64_t)A.entries_per_row(),
-                               (int64_t)A.cvalues().ncols(), (int64_t)A.cvalues().nrows(),
-                               A.ccolumn_indices().data(), A.cvalues().data(), vt);
+    status = mph_matrix_create_ell(
+        &h, nr, nc, (int64_t)A.nnnz(), (int64_t)A.entries_per_row(), (int64_t)A.cvalues().ncols(),
+        (int64_t)A.cvalues().nrows(),
+        stage_index(st, A.ccolumn_indices().data(), dm_size(A.ccolumn_indices())),
+        stage_values(st, A.cvalues().data(), dm_size(A.cvalues())), vt);
   } else if constexpr (M::format_id == HYB_FORMAT) {
     const auto& e = A.cell();
     const auto& c = A.ccoo();
-    st = mph_matrix_create_hyb(&h, nr, nc, (int64_t)e.nnnz(), (int64_t)e.entries_per_row(),
-                               (int64_t)e.cvalues().ncols(), (int64_t)e.cvalues().nrows(),
-                               e.ccolumn_indices().data(), e.cvalues().data(), (int64_t)c.nnnz(),
-                               c.crow_indices().data(), c.ccolumn_indices().data(),
-                               c.cvalues().data(), vt);
+    status = mph_matrix_create_hyb(
+        &h, nr, nc, (int64_t)e.nnnz(), (int64_t)e.entries_per_row(), (int64_t)e.cvalues().ncols(),
+        (int64_t)e.cvalues().nrows(),
+        stage_index(st, e.ccolumn_indices().data(), dm_size(e.ccolumn_indices())),
+        stage_values(st, e.cvalues().data(), dm_size(e.cvalues())), (int64_t)c.nnnz(),
+        stage_index(st, c.crow_indices().data(), c.nnnz()),
+        stage_index(st, c.ccolumn_indices().data(), c.nnnz()),
+        stage_values(st, c.cvalues().data(), c.nnnz()), vt);
   } else {
     const auto& d = A.cdia();
     const auto& c = A.ccsr();
-    st = mph_matrix_create_hdc(&h, nr, nc, (int64_t)d.nnnz(), (int64_t)d.cdiagonal_offsets().size(),
-                               (int64_t)d.cvalues().nrows(), d.cdiagonal_offsets().data(),
-                               d.cvalues().data(), (int64_t)c.nnnz(), c.crow_offsets().data(),
-                               c.ccolumn_indices().data(), c.cvalues().data(), vt);
+    status = mph_matrix_create_hdc(
+        &h, nr, nc, (int64_t)d.nnnz(), (int64_t)d.cdiagonal_offsets().size(),
+        (int64_t)d.cvalues().nrows(),
+        stage_index(st, d.cdiagonal_offsets().data(), d.cdiagonal_offsets().size()),
+        stage_values(st, d.cvalues().data(), dm_size(d.cvalues())), (int64_t)c.nnnz(),
+        stage_index(st, c.crow_offsets().data(), c.nrows() + 1),
+        stage_index(st, c.ccolumn_indices().data(), c.nnnz()),
+        stage_values(st, c.cvalues().data(), c.nnnz()), vt);
   }
-  check(st, "Morpheus::multiply");
+  check(status, "Morpheus::multiply");
   // the analysis keeps no copy of any value: products always read the
   // container's value arrays, so values written in place through a View
   // kept from an earlier accessor call are seen by the next multiply (as
   // with the reference's containers, which hold no derived state)
-  st = mph_matrix_set_policy(h, MPH_POLICY_NO_VALUE_COPIES);
-  if (st != MPH_OK) mph_matrix_destroy(h);
-  check(st, "Morpheus::multiply");
+  status = mph_matrix_set_policy(h, MPH_POLICY_NO_VALUE_COPIES);
+  if (status != MPH_OK) mph_matrix_destroy(h);
+  check(status, "Morpheus::multiply");
+  for (auto& f : st.refresh) f();
   return h;
 }
 
@@ -227,19 +361,30 @@ inline mph_matrix_t matrix_handle(const M& A) {
       using DM = device_type_of<M>;
       copy_values(A, *std::static_pointer_cast<DM>(pc.device_copy));
     }
+    if (pc.refresh) pc.refresh();
     return pc.handle;
   }
   pc.reset();
+  auto st = std::make_shared<Stage>();
   if constexpr (M::memory_space::is_device) {
-    pc.handle = create_handle(A);
+    pc.handle = create_handle(A, *st);
   } else {
     using DM = device_type_of<M>;
     auto dev = std::make_shared<DM>();
     dev->allocate(A);
     copy_sparse(A, *dev);
-    pc.handle = create_handle(*dev);
+    pc.handle = create_handle(*dev, *st);
     pc.device_copy = dev;
   }
+  if (!st->refresh.empty()) {
+    pc.refresh = [st]() {
+      for (auto& f : st->refresh) f();
+    };
+    pc.write_back = [st]() {
+      for (auto& f : st->write_back) f();
+    };
+  }
+  pc.stage = st;
   pc.signature = std::move(sig);
   pc.stale = false;
   return pc.handle;
@@ -258,13 +403,21 @@ inline void multiply_concrete(const M& A, const VX& x, VY& y, bool init) {
                                 " matrix, x " + std::to_string(x.size()) + ", y " +
                                 std::to_string(y.size()) + ")");
   mph_matrix_t h = matrix_handle(A);
+  // int vectors travel as FP64 (the handle of an int container computes in FP64)
+  auto run = [&](const V* xd, V* yd) {
+    as_compute_vector(const_cast<V*>(xd), x.size(), true, false, [&](void* xc) {
+      as_compute_vector(yd, y.size(), !init, true, [&](void* yc) {
+        check(mph_multiply(h, xc, yc, init ? 1 : 0, nullptr), "Morpheus::multiply");
+      });
+    });
+  };
   if constexpr (M::memory_space::is_device) {
-    check(mph_multiply(h, x.data(), y.data(), init ? 1 : 0, nullptr), "Morpheus::multiply");
+    run(x.data(), y.data());
   } else {
     DenseVector<V, HIPSpace> xd(x.size()), yd(y.size());
     copy_array(x.const_view(), xd.view(), "x");
     if (!init) copy_array(y.const_view(), yd.view(), "y");
-    check(mph_multiply(h, xd.data(), yd.data(), init ? 1 : 0, nullptr), "Morpheus::multiply");
+    run(xd.data(), yd.data());
     copy_array(yd.const_view(), y.view(), "y");
   }
 }
@@ -281,45 +434,60 @@ inline int64_t i64(size_t v) { return (int64_t)v; }
 template <class S, class D>
 inline void convert_device(const S& src, D& dst);
 
-// COO -> every format (core/src/impl/<Fmt>/Serial/Morpheus_Convert_Impl.hpp)
+// COO -> every format (core/src/impl/<Fmt>/Serial/Morpheus_Convert_Impl.hpp).
+// Indices reach the kernels as int32 (long long containers: narrowed copies
+// in, widened copies out); values move as raw words of their own type.
 template <class S, class D>
 inline void convert_from_coo(const S& src, D& dst) {
   constexpr int vt = value_code<typename S::non_const_value_type>::value;
   const int64_t n = i64(src.nrows()), m = i64(src.ncols()), nnz = i64(src.nnnz());
-  const int* ri = src.crow_indices().data();
-  const int* ci = src.ccolumn_indices().data();
+  Stage st;
+  const int* ri = stage_index(st, src.crow_indices().data(), src.nnnz());
+  const int* ci = stage_index(st, src.ccolumn_indices().data(), src.nnnz());
   const void* v = src.cvalues().data();
+  auto dm_size = [](const auto& d) { return (size_t)d.nrows() * d.ncols(); };
   if constexpr (D::format_id == CSR_FORMAT) {
     dst.resize(n, m, nnz);
-    check(mph_convert_coo_to_csr(n, nnz, ri, ci, v, vt, dst.row_offsets().data(),
-                                 dst.column_indices().data(), dst.values().data(), nullptr),
+    auto ro = idx_out(dst.row_offsets().data(), (size_t)n + 1);
+    auto co = idx_out(dst.column_indices().data(), (size_t)nnz);
+    check(mph_convert_coo_to_csr(n, nnz, ri, ci, v, vt, ro.p, co.p, dst.values().data(), nullptr),
           "Morpheus::convert");
+    ro.done();
+    co.done();
   } else if constexpr (D::format_id == DIA_FORMAT) {
     int64_t nd = 0;
     check(mph_convert_coo_to_dia_size(n, m, nnz, ri, ci, &nd, nullptr), "Morpheus::convert");
     dst.resize(n, m, nnz, nd, 32);
-    check(mph_convert_coo_to_dia(n, m, nnz, ri, ci, v, vt, nd, i64(dst.values().nrows()),
-                                 dst.diagonal_offsets().data(), dst.values().data(), nullptr),
+    auto off = idx_out(dst.diagonal_offsets().data(), (size_t)nd);
+    check(mph_convert_coo_to_dia(n, m, nnz, ri, ci, v, vt, nd, i64(dst.values().nrows()), off.p,
+                                 dst.values().data(), nullptr),
           "Morpheus::convert");
+    off.done();
   } else if constexpr (D::format_id == ELL_FORMAT) {
     int64_t K = 0;
     check(mph_convert_coo_to_ell_size(n, nnz, ri, &K, nullptr), "Morpheus::convert");
     dst.resize(n, m, nnz, K, 32);
+    auto eci = idx_out(dst.column_indices().data(), dm_size(dst.column_indices()));
     check(mph_convert_coo_to_ell(n, nnz, ri, ci, v, vt, K, i64(dst.values().ncols()),
-                                 i64(dst.values().nrows()), dst.column_indices().data(),
-                                 dst.values().data(), nullptr),
+                                 i64(dst.values().nrows()), eci.p, dst.values().data(), nullptr),
           "Morpheus::convert");
+    eci.done();
   } else if constexpr (D::format_id == HYB_FORMAT) {
     int64_t K = 0, enz = 0, cnz = 0;
     check(mph_convert_coo_to_hyb_size(n, nnz, ri, &K, &enz, &cnz, nullptr), "Morpheus::convert");
     dst.resize(n, m, enz, cnz, K, 32);
     auto& e = dst.ell();
     auto& c = dst.coo();
+    auto eci = idx_out(e.column_indices().data(), dm_size(e.column_indices()));
+    auto cri = idx_out(c.row_indices().data(), (size_t)cnz);
+    auto cci = idx_out(c.column_indices().data(), (size_t)cnz);
     check(mph_convert_coo_to_hyb(n, nnz, ri, ci, v, vt, K, i64(e.values().ncols()),
-                                 i64(e.values().nrows()), e.column_indices().data(),
-                                 e.values().data(), c.row_indices().data(),
-                                 c.column_indices().data(), c.values().data(), nullptr),
+                                 i64(e.values().nrows()), eci.p, e.values().data(), cri.p, cci.p,
+                                 c.values().data(), nullptr),
           "Morpheus::convert");
+    eci.done();
+    cri.done();
+    cci.done();
   } else if constexpr (D::format_id == HDC_FORMAT) {
     int64_t nd = 0, dnz = 0, cnz = 0;
     check(mph_convert_coo_to_hdc_size(n, m, nnz, ri, ci, &nd, &dnz, &cnz, nullptr),
@@ -327,11 +495,15 @@ inline void convert_from_coo(const S& src, D& dst) {
     dst.resize(n, m, dnz, cnz, nd, 32);
     auto& d = dst.dia();
     auto& c = dst.csr();
-    check(mph_convert_coo_to_hdc(n, m, nnz, ri, ci, v, vt, nd, i64(d.values().nrows()),
-                                 d.diagonal_offsets().data(), d.values().data(),
-                                 c.row_offsets().data(), c.column_indices().data(),
-                                 c.values().data(), nullptr),
+    auto off = idx_out(d.diagonal_offsets().data(), (size_t)nd);
+    auto cro = idx_out(c.row_offsets().data(), (size_t)n + 1);
+    auto cci = idx_out(c.column_indices().data(), (size_t)cnz);
+    check(mph_convert_coo_to_hdc(n, m, nnz, ri, ci, v, vt, nd, i64(d.values().nrows()), off.p,
+                                 d.values().data(), cro.p, cci.p, c.values().data(), nullptr),
           "Morpheus::convert");
+    off.done();
+    cro.done();
+    cci.done();
   }
   dst.touch();
 }
@@ -342,38 +514,50 @@ inline void convert_to_coo(const S& src, D& dst) {
   constexpr int vt = value_code<typename S::non_const_value_type>::value;
   const int64_t n = i64(src.nrows()), m = i64(src.ncols()), nnz = i64(src.nnnz());
   dst.resize(n, m, nnz);
-  int* ri = dst.row_indices().data();
-  int* ci = dst.column_indices().data();
+  Stage st;
+  auto dm_size = [](const auto& d) { return (size_t)d.nrows() * d.ncols(); };
+  auto ro = idx_out(dst.row_indices().data(), (size_t)nnz);
+  auto co = idx_out(dst.column_indices().data(), (size_t)nnz);
+  int* ri = ro.p;
+  int* ci = co.p;
   void* v = dst.values().data();
-  int st = MPH_OK;
+  int status = MPH_OK;
   if constexpr (S::format_id == CSR_FORMAT) {
-    st = mph_convert_csr_to_coo(n, nnz, src.crow_offsets().data(), src.ccolumn_indices().data(),
-                                src.cvalues().data(), vt, ri, ci, v, nullptr);
+    status = mph_convert_csr_to_coo(n, nnz, stage_index(st, src.crow_offsets().data(), src.nrows() + 1),
+                                    stage_index(st, src.ccolumn_indices().data(), src.nnnz()),
+                                    src.cvalues().data(), vt, ri, ci, v, nullptr);
   } else if constexpr (S::format_id == DIA_FORMAT) {
-    st = mph_convert_dia_to_coo(n, m, nnz, i64(src.cdiagonal_offsets().size()),
-                                i64(src.cvalues().nrows()), src.cdiagonal_offsets().data(),
-                                src.cvalues().data(), vt, ri, ci, v, nullptr);
+    status = mph_convert_dia_to_coo(
+        n, m, nnz, i64(src.cdiagonal_offsets().size()), i64(src.cvalues().nrows()),
+        stage_index(st, src.cdiagonal_offsets().data(), src.cdiagonal_offsets().size()),
+        src.cvalues().data(), vt, ri, ci, v, nullptr);
   } else if constexpr (S::format_id == ELL_FORMAT) {
-    st = mph_convert_ell_to_coo(n, nnz, i64(src.cvalues().ncols()), i64(src.cvalues().nrows()),
-                                src.ccolumn_indices().data(), src.cvalues().data(), vt, ri, ci, v,
-                                nullptr);
+    status = mph_convert_ell_to_coo(
+        n, nnz, i64(src.cvalues().ncols()), i64(src.cvalues().nrows()),
+        stage_index(st, src.ccolumn_indices().data(), dm_size(src.ccolumn_indices())),
+        src.cvalues().data(), vt, ri, ci, v, nullptr);
   } else if constexpr (S::format_id == HYB_FORMAT) {
     const auto& e = src.cell();
     const auto& c = src.ccoo();
-    st = mph_convert_hyb_to_coo(n, nnz, i64(e.cvalues().ncols()), i64(e.cvalues().nrows()),
-                                e.ccolumn_indices().data(), e.cvalues().data(), i64(c.nnnz()),
-                                c.crow_indices().data(), c.ccolumn_indices().data(),
-                                c.cvalues().data(), vt, ri, ci, v, nullptr);
+    status = mph_convert_hyb_to_coo(
+        n, nnz, i64(e.cvalues().ncols()), i64(e.cvalues().nrows()),
+        stage_index(st, e.ccolumn_indices().data(), dm_size(e.ccolumn_indices())),
+        e.cvalues().data(), i64(c.nnnz()), stage_index(st, c.crow_indices().data(), c.nnnz()),
+        stage_index(st, c.ccolumn_indices().data(), c.nnnz()), c.cvalues().data(), vt, ri, ci, v,
+        nullptr);
   } else if constexpr (S::format_id == HDC_FORMAT) {
     const auto& d = src.cdia();
     const auto& c = src.ccsr();
-    st = mph_convert_hdc_to_coo(n, m, nnz, i64(d.cdiagonal_offsets().size()),
-                                i64(d.cvalues().nrows()), d.cdiagonal_offsets().data(),
-                                d.cvalues().data(), i64(c.nnnz()), c.crow_offsets().data(),
-                                c.ccolumn_indices().data(), c.cvalues().data(), vt, ri, ci, v,
-                                nullptr);
+    status = mph_convert_hdc_to_coo(
+        n, m, nnz, i64(d.cdiagonal_offsets().size()), i64(d.cvalues().nrows()),
+        stage_index(st, d.cdiagonal_offsets().data(), d.cdiagonal_offsets().size()),
+        d.cvalues().data(), i64(c.nnnz()), stage_index(st, c.crow_offsets().data(), c.nrows() + 1),
+        stage_index(st, c.ccolumn_indices().data(), c.nnnz()), c.cvalues().data(), vt, ri, ci, v,
+        nullptr);
   }
-  check(st, "Morpheus::convert");
+  check(status, "Morpheus::convert");
+  ro.done();
+  co.done();
   dst.touch();
 }
 
@@ -645,10 +829,13 @@ void sort_by_row_and_column(Matrix& mat) {
   Impl::check_access<ExecSpace, Matrix>();
   Impl::on_device_coo(mat, true, [](auto& d) {
     constexpr int vt = Impl::value_code<typename Matrix::non_const_value_type>::value;
-    Impl::check(mph_coo_sort_by_row_and_column((int64_t)d.nnnz(), d.row_indices().data(),
-                                               d.column_indices().data(), d.values().data(), vt,
-                                               nullptr),
+    auto ri = Impl::idx_out(d.row_indices().data(), d.nnnz(), true);
+    auto ci = Impl::idx_out(d.column_indices().data(), d.nnnz(), true);
+    Impl::check(mph_coo_sort_by_row_and_column((int64_t)d.nnnz(), ri.p, ci.p, d.values().data(),
+                                               vt, nullptr),
                 "Morpheus::sort_by_row_and_column");
+    ri.done();
+    ci.done();
   });
   mat.touch();
 }
@@ -660,8 +847,10 @@ bool is_sorted(Matrix& mat) {
   Impl::check_access<ExecSpace, Matrix>();
   int sorted = 0;
   Impl::on_device_coo(mat, false, [&](auto& d) {
-    Impl::check(mph_coo_is_sorted((int64_t)d.nnnz(), d.crow_indices().data(),
-                                  d.ccolumn_indices().data(), &sorted, nullptr),
+    Impl::Stage st;
+    Impl::check(mph_coo_is_sorted((int64_t)d.nnnz(), Impl::stage_index(st, d.crow_indices().data(), d.nnnz()),
+                                  Impl::stage_index(st, d.ccolumn_indices().data(), d.nnnz()),
+                                  &sorted, nullptr),
                 "Morpheus::is_sorted");
   });
   return sorted != 0;
@@ -948,22 +1137,31 @@ void update_diagonal(Matrix& A, const Vector& diagonal) {
   } else {
     Impl::check_access<ExecSpace, Matrix>();
     auto& dm = const_cast<Vector&>(diagonal);
+    const size_t nd = diagonal.size();
     if constexpr (Matrix::memory_space::is_device) {
       Impl::with_device_vector(dm, true, false, [&](auto* pd) {
-        Impl::check(mph_matrix_update_diagonal(Impl::matrix_handle(A), pd, nullptr),
-                    "Morpheus::update_diagonal");
+        Impl::as_compute_vector(pd, nd, true, false, [&](void* pc) {
+          Impl::check(mph_matrix_update_diagonal(Impl::matrix_handle(A), pc, nullptr),
+                      "Morpheus::update_diagonal");
+        });
       });
+      // int values: the FP64 copy the handle updated, cast back
+      if (A.plan_cache().write_back) A.plan_cache().write_back();
     } else {
       Impl::device_type_of<Matrix> dev;
       dev.allocate(A);
       Impl::copy_sparse(A, dev);
-      mph_matrix_t h = Impl::create_handle(dev);
+      Impl::Stage stage;
+      mph_matrix_t h = Impl::create_handle(dev, stage);
       int st = MPH_OK;
       Impl::with_device_vector(dm, true, false, [&](auto* pd) {
-        st = mph_matrix_update_diagonal(h, pd, nullptr);
+        Impl::as_compute_vector(pd, nd, true, false, [&](void* pc) {
+          st = mph_matrix_update_diagonal(h, pc, nullptr);
+        });
       });
       mph_matrix_destroy(h);
       Impl::check(st, "Morpheus::update_diagonal");
+      for (auto& f : stage.write_back) f();
       Impl::copy_sparse(dev, A);
       A.touch();  // the staged copy of A held the old values
     }
@@ -981,8 +1179,10 @@ void get_diagonal(const Matrix& A, Vector& diagonal) {
   Impl::with_concrete(A, [&](const auto& a) {
     Impl::check_access<ExecSpace, std::decay_t<decltype(a)>>();
     Impl::with_device_vector(diagonal, false, true, [&](auto* pd) {
-      Impl::check(mph_matrix_get_diagonal(Impl::matrix_handle(a), pd, nullptr),
-                  "Morpheus::get_diagonal");
+      Impl::as_compute_vector(pd, diagonal.size(), false, true, [&](void* pc) {
+        Impl::check(mph_matrix_get_diagonal(Impl::matrix_handle(a), pc, nullptr),
+                    "Morpheus::get_diagonal");
+      });
     });
   });
 }

@@ -31,6 +31,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <string>
 #include <type_traits>
@@ -125,18 +126,26 @@ struct ContainerTraits {
                          typename memory_space::default_layout, typename parsed::array_layout>;
   static constexpr bool is_device = memory_space::is_device;
 
-  static_assert(std::is_same_v<non_const_index_type, int>,
-                "the MI355X kernels take int32 indices");
+  // (int64 indices reach the kernels narrowed to int32 -- every index and
+  // size below 2^31 -- and come back widened)
+  static_assert(std::is_same_v<non_const_index_type, int> ||
+                    std::is_same_v<non_const_index_type, long long>,
+                "IndexType must be int or long long");
   static_assert(std::is_arithmetic_v<non_const_value_type>,
                 "ValueType must be an arithmetic type such as int or double");
 };
 
-// Value types the MI355X kernels compute in (multiply / convert take
-// double or float containers).
+// Value types of the C-ABI: double and float containers compute in their
+// own type; int containers are converted as raw words (MPH_I32) and
+// multiplied exactly through FP64 (staged values and vectors).
 template <class V>
 struct value_code {
   static_assert(sizeof(V) == 0,
-                "Morpheus multiply / convert on the MI355X take double or float values");
+                "Morpheus multiply / convert on the MI355X take double, float or int values");
+};
+template <>
+struct value_code<int> {
+  static constexpr int value = MPH_I32;
 };
 template <>
 struct value_code<double> {
@@ -234,6 +243,12 @@ struct PlanCache {
   std::vector<const void*> signature;
   bool stale = true;
   std::shared_ptr<void> device_copy;  // host containers: the staged device matrix
+  // int64-indexed / int-valued containers: the int32 indices and FP64 values
+  // the handle is built over; refresh() recasts the values before every use
+  // of the handle, write_back() casts them into the container after an
+  // operation that writes values
+  std::shared_ptr<void> stage;
+  std::function<void()> refresh, write_back;
   ~PlanCache() {
     if (handle) mph_matrix_destroy(handle);
   }
@@ -241,6 +256,9 @@ struct PlanCache {
     if (handle) mph_matrix_destroy(handle);
     handle = nullptr;
     device_copy.reset();
+    stage.reset();
+    refresh = nullptr;
+    write_back = nullptr;
     stale = true;
   }
 };

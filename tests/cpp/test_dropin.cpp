@@ -681,6 +681,157 @@ static void convert_all() {
 TEST(Convert_Small_AllPairs_HIP) { convert_all<HIPExec>(); }
 TEST(Convert_Small_AllPairs_Serial) { convert_all<Serial>(); }
 
+// ------------------------------------------------------------ type lists
+// The reference's convert / container type lists (core/tests/utils/
+// Utils.hpp:50-60): values {double, int} x indices {long long, int}.  The
+// fixture containers retyped element by element (the fixture's values
+// 1.11 .. 32.32 become 1 .. 32 as int).
+template <class V, class I, class M>
+static auto retype(const M& A) {
+  using Morpheus::HostSpace;
+  if constexpr (M::format_id == Morpheus::COO_FORMAT) {
+    Morpheus::CooMatrix<V, I, HostSpace> B(A.nrows(), A.ncols(), A.nnnz());
+    for (size_t k = 0; k < A.nnnz(); k++) {
+      B.row_indices(k) = (I)A.crow_indices(k);
+      B.column_indices(k) = (I)A.ccolumn_indices(k);
+      B.values(k) = (V)A.cvalues(k);
+    }
+    return B;
+  } else if constexpr (M::format_id == Morpheus::CSR_FORMAT) {
+    Morpheus::CsrMatrix<V, I, HostSpace> B(A.nrows(), A.ncols(), A.nnnz());
+    for (size_t r = 0; r <= A.nrows(); r++) B.row_offsets(r) = (I)A.crow_offsets(r);
+    for (size_t k = 0; k < A.nnnz(); k++) {
+      B.column_indices(k) = (I)A.ccolumn_indices(k);
+      B.values(k) = (V)A.cvalues(k);
+    }
+    return B;
+  } else if constexpr (M::format_id == Morpheus::DIA_FORMAT) {
+    Morpheus::DiaMatrix<V, I, HostSpace> B(A.nrows(), A.ncols(), A.nnnz(), A.ndiags(),
+                                           A.alignment());
+    for (size_t d = 0; d < A.ndiags(); d++) B.diagonal_offsets(d) = (I)A.cdiagonal_offsets(d);
+    for (size_t i = 0; i < A.cvalues().nrows(); i++)
+      for (size_t j = 0; j < A.cvalues().ncols(); j++) B.values(i, j) = (V)A.cvalues(i, j);
+    return B;
+  } else if constexpr (M::format_id == Morpheus::ELL_FORMAT) {
+    Morpheus::EllMatrix<V, I, HostSpace> B(A.nrows(), A.ncols(), A.nnnz(), A.entries_per_row(),
+                                           A.alignment());
+    for (size_t i = 0; i < A.cvalues().nrows(); i++)
+      for (size_t j = 0; j < A.cvalues().ncols(); j++) {
+        B.column_indices(i, j) = (I)A.ccolumn_indices(i, j);
+        B.values(i, j) = (V)A.cvalues(i, j);
+      }
+    return B;
+  } else if constexpr (M::format_id == Morpheus::HYB_FORMAT) {
+    Morpheus::HybMatrix<V, I, HostSpace> B(A.nrows(), A.ncols(), A.cell().nnnz(), A.ccoo().nnnz(),
+                                           A.cell().entries_per_row(), A.alignment());
+    B.ell() = retype<V, I>(A.cell());
+    B.coo() = retype<V, I>(A.ccoo());
+    return B;
+  } else {
+    Morpheus::HdcMatrix<V, I, HostSpace> B(A.nrows(), A.ncols(), A.cdia().nnnz(), A.ccsr().nnnz(),
+                                           A.cdia().ndiags(), A.alignment());
+    B.dia() = retype<V, I>(A.cdia());
+    B.csr() = retype<V, I>(A.ccsr());
+    return B;
+  }
+}
+
+template <class V, class I, int S, int D, class ExecSpace>
+static void convert_pair_typed() {
+  auto src = retype<V, I>(small_of<S>());
+  auto expect = retype<V, I>(small_of<D>());
+  decltype(expect) out;
+  if constexpr (ExecSpace::is_device) {
+    auto sd = Morpheus::create_mirror_container<Dev>(src);
+    Morpheus::copy(src, sd);
+    decltype(Morpheus::create_mirror_container<Dev>(expect)) dd;
+    Morpheus::convert<ExecSpace>(sd, dd);
+    out.allocate(dd);
+    Morpheus::copy(dd, out);
+  } else {
+    Morpheus::convert<ExecSpace>(src, out);
+  }
+  if (!same_container(out, expect)) {
+    std::fprintf(stderr, "  convert %d -> %d (%zu-byte values, %zu-byte indices) differs\n", S,
+                 D, sizeof(V), sizeof(I));
+    EXPECT_TRUE(false);
+  } else {
+    EXPECT_TRUE(true);
+  }
+}
+template <class V, class I, int S, class ExecSpace>
+static void convert_row_typed() {
+  convert_pair_typed<V, I, S, 0, ExecSpace>();
+  convert_pair_typed<V, I, S, 1, ExecSpace>();
+  convert_pair_typed<V, I, S, 2, ExecSpace>();
+  convert_pair_typed<V, I, S, 3, ExecSpace>();
+  convert_pair_typed<V, I, S, 4, ExecSpace>();
+  convert_pair_typed<V, I, S, 5, ExecSpace>();
+}
+template <class V, class I, class ExecSpace>
+static void convert_all_typed() {
+  convert_row_typed<V, I, 0, ExecSpace>();
+  convert_row_typed<V, I, 1, ExecSpace>();
+  convert_row_typed<V, I, 2, ExecSpace>();
+  convert_row_typed<V, I, 3, ExecSpace>();
+  convert_row_typed<V, I, 4, ExecSpace>();
+  convert_row_typed<V, I, 5, ExecSpace>();
+}
+TEST(Convert_Types_Double_LongLong_HIP) { convert_all_typed<double, long long, HIPExec>(); }
+TEST(Convert_Types_Int_LongLong_HIP) { convert_all_typed<int, long long, HIPExec>(); }
+TEST(Convert_Types_Int_Int_HIP) { convert_all_typed<int, int, HIPExec>(); }
+TEST(Convert_Types_Int_LongLong_Serial) { convert_all_typed<int, long long, Serial>(); }
+
+// Multiply over the same type list: int values are exact (the products of
+// the fixture's 1 .. 32 with an integer x in [-8, 8] are small integers).
+template <class V, class I>
+static void multiply_typed() {
+  const auto coo = small_coo();
+  Morpheus::DenseVector<V, I, Host> x(coo.ncols());
+  Vec<Host> xd_ref(coo.ncols());
+  for (size_t j = 0; j < coo.ncols(); j++) {
+    x(j) = (V)((int)(j * 7 % 17) - 8);
+    xd_ref(j) = (double)x(j);
+  }
+  auto coo_t = retype<V, I>(coo);
+  // the reference product of the retyped matrix, in double
+  Coo<Host> coo_d(coo.nrows(), coo.ncols(), coo.nnnz());
+  for (size_t k = 0; k < coo.nnnz(); k++) {
+    coo_d.row_indices(k) = coo.crow_indices(k);
+    coo_d.column_indices(k) = coo.ccolumn_indices(k);
+    coo_d.values(k) = (double)coo_t.cvalues(k);
+  }
+  const Ref ref = dense_ref(coo_d, xd_ref, nullptr);
+  auto check_one = [&](const auto& Ah) {
+    auto Ad = Morpheus::create_mirror_container<Dev>(Ah);
+    Morpheus::copy(Ah, Ad);
+    Morpheus::DenseVector<V, I, Dev> xdev(x.size()), ydev(coo.nrows(), (V)7);
+    Morpheus::copy(x, xdev);
+    Morpheus::multiply<HIPExec>(Ad, xdev, ydev);
+    Morpheus::DenseVector<V, I, Host> y(coo.nrows(), (V)0);
+    Morpheus::copy(ydev, y);
+    Vec<Host> yd(coo.nrows());
+    for (size_t i = 0; i < y.size(); i++) yd(i) = (double)y(i);
+    const bool ok = std::is_same_v<V, int> ? [&] {
+      for (size_t i = 0; i < y.size(); i++)
+        if ((double)y(i) != ref.y[i]) return false;
+      return true;
+    }()
+                                           : within(yd, ref);
+    if (!ok) std::fprintf(stderr, "  multiply format %d differs\n", Ah.format_index());
+    EXPECT_TRUE(ok);
+  };
+  check_one(retype<V, I>(small_coo()));
+  check_one(retype<V, I>(small_csr()));
+  check_one(retype<V, I>(small_dia()));
+  check_one(retype<V, I>(small_ell()));
+  check_one(retype<V, I>(small_hyb()));
+  check_one(retype<V, I>(small_hdc()));
+}
+TEST(Multiply_Types_Double_LongLong) { multiply_typed<double, long long>(); }
+TEST(Multiply_Types_Int_LongLong) { multiply_typed<int, long long>(); }
+TEST(Multiply_Types_Int_Int) { multiply_typed<int, int>(); }
+
 // core/tests/Serial/Test_Convert_Dynamic.hpp: in-place switches
 TEST(Convert_Dynamic_InPlace) {
   for (int s = 0; s < 6; s++) {
