@@ -45,6 +45,9 @@ def parse():
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched SpMM section (512 matrices x 8 RHS)")
     ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--no-batch-plain", action="store_true",
+                    help="skip the plain FP64 forms beside the batch (per-matrix SpMM / SpMV "
+                         "loops and their hipGraphs): counter passes")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB device copy (PMC traffic calibration)")
     ap.add_argument("--no-tune", action="store_true",
@@ -533,7 +536,7 @@ def run_cpu(mats, order, budget):
     return res, oracle.omp_threads()
 
 
-def run_batch(steps):
+def run_batch(steps, plain=True):
     """Batched SpMM: 512 matrices (log-uniform 1k-64k rows; 2D 5-point
     stencils, power-law(1.5) rows with mean 12, banded with bandwidth 5-33),
     k = 8 row-major right-hand sides U[-1, 1]; formats chosen per matrix by
@@ -645,6 +648,10 @@ def run_batch(steps):
         res[f"speedup_{dtype}_vs_batched_f64"] = round(
             res[dtype]["matrhs_per_s"] / res["f64"]["matrhs_per_s"], 3)
     res["tolerance"] = {"f32": 1e-5, "bf16": 1e-2}
+    if not plain:
+        for D in D64:
+            D.close()
+        return res
     # plain FP64 GPU forms: one SpMM per matrix, and one SpMV per matrix and RHS
     Y64 = [torch.empty((D.nrows, K), dtype=torch.float64, device="cuda") for D in D64]
     plain = {}
@@ -943,7 +950,7 @@ def main():
     if args.only:
         sec = args.only
         if sec == "batch":
-            res = run_batch(args.batch_steps)
+            res = run_batch(args.batch_steps, not args.no_batch_plain)
         elif sec.startswith("large"):
             res = run_large(autotune=False,
                             matrices=("S27", "Z12") if sec == "large" else (sec.split("-")[1],))
@@ -1071,7 +1078,7 @@ def main():
 
     cg = run_cg(mats, 100)
     p2 = run_p2()
-    batch = None if args.no_batch else run_batch(args.batch_steps)
+    batch = None if args.no_batch else run_batch(args.batch_steps, not args.no_batch_plain)
     sweep = None if (args.no_sweep or only) else run_sweep(z15, s7, 20)
     large = None if (args.no_large or only) else run_large()
 

@@ -1555,6 +1555,38 @@ namespace {
 // the kernel skips, a row longer than the chunk that the pass does not take
 // keeps its position chunks, and the other rows form short blocks with the
 // usual limits; the records are checked before upload.
+// the longest row of offsets ro (a workgroup maximum, then one atomic)
+__global__ __launch_bounds__(256) void max_row_len_kernel(const int32_t* __restrict__ ro,
+                                                          int64_t nrows, int32_t* __restrict__ out) {
+  __shared__ int32_t s_max;
+  if (threadIdx.x == 0) s_max = 0;
+  __syncthreads();
+  int32_t m = 0;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows;
+       r += (int64_t)gridDim.x * blockDim.x)
+    m = max(m, ro[r + 1] - ro[r]);
+  atomicMax(&s_max, m);
+  __syncthreads();
+  if (threadIdx.x == 0) atomicMax(out, s_max);
+}
+
+int max_row_len(const int32_t* d_ro, int64_t nrows, hipStream_t s, int32_t* out) {
+  int32_t* d = nullptr;
+  MPH_HIP_CHECK(hipMalloc(&d, sizeof(int32_t)));
+  const int32_t zero = 0;
+  hipError_t e = hipMemcpyAsync(d, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(max_row_len_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(nrows, 256), 2048)),
+                       dim3(256), 0, s, d_ro, nrows, d);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d, sizeof(int32_t), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(d);
+  MPH_HIP_CHECK(e);
+  return MPH_OK;
+}
+
 int split_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
   RowBlockPlan& p = A->rb;
   const int64_t nrows = A->nrows;
@@ -1567,6 +1599,13 @@ int split_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
                         (A->format == MPH_HDC && A->ndiag > 0);
   const bool split = prefixed && (xt_on || nrows >= kSplitMinRows || p.prefix_apart);
   if ((!xt_on && !split) || nrows == 0 || A->ncols == 0) return MPH_OK;
+  if (xt_on && !split) {
+    // nothing to tile: the plain row-block plan stands (no host pass over
+    // the row offsets)
+    int32_t longest = 0;
+    MPH_CHECK_STATUS(max_row_len(d_ro, nrows, s, &longest));
+    if (longest <= kXTileMinRowLen) return MPH_OK;
+  }
   const int32_t* d_ci = p.ci;
   const void* d_val = p.val;
 
