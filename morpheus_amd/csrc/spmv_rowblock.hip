@@ -771,10 +771,28 @@ __global__ __launch_bounds__(256) void xtile_combine_kernel(RBArgs<V> a, XTArgs 
   }
 }
 
+// Analysis: unsorted[i] flags a long row whose columns are not sorted (one
+// workgroup per row, its threads striding over the row's adjacent pairs).
+__global__ __launch_bounds__(256) void xtile_sorted_kernel(const int32_t* __restrict__ ro,
+                                                           const int32_t* __restrict__ ci,
+                                                           const int32_t* __restrict__ rows,
+                                                           int32_t* __restrict__ unsorted) {
+  __shared__ int32_t s_bad;
+  if (threadIdx.x == 0) s_bad = 0;
+  __syncthreads();
+  const int32_t r = rows[blockIdx.x];
+  const int32_t b = ro[r], e = ro[r + 1];
+  int32_t bad = 0;
+  for (int32_t j = b + 1 + (int32_t)threadIdx.x; j < e; j += (int32_t)blockDim.x)
+    bad |= ci[j - 1] > ci[j] ? 1 : 0;
+  if (bad) s_bad = 1;
+  __syncthreads();
+  if (threadIdx.x == 0) unsorted[blockIdx.x] = s_bad;
+}
+
 // Analysis: per long row i and tile q = 0..ntiles, the first item whose
 // column is >= q * tile (lower bound in the sorted columns), stored
-// tile-major at out[q * nl + i]; unsorted[i] flags rows whose columns are
-// not sorted.
+// tile-major at out[q * nl + i].
 __global__ void xtile_bounds_kernel(const int32_t* __restrict__ ro,
                                     const int32_t* __restrict__ ci,
                                     const int32_t* __restrict__ rows, int32_t nl,
@@ -788,13 +806,6 @@ __global__ void xtile_bounds_kernel(const int32_t* __restrict__ ro,
   const int32_t b = ro[r], e = ro[r + 1];
   if (q == 0) {
     out[t] = b;
-    int32_t bad = 0;
-    for (int32_t j = b + 1; j < e; j++)
-      if (ci[j - 1] > ci[j]) {
-        bad = 1;
-        break;
-      }
-    unsorted[i] = bad;
     return;
   }
   if (q == ntiles) {
@@ -1066,14 +1077,14 @@ __global__ void hyb_merge_count_kernel(const int32_t* __restrict__ eci, int64_t 
   cnt[r] = c;
 }
 
-// HYB merged copy, fill: one thread per row writes its ELL entries in slot
-// order, then its COO entries in storage order (the order of the Serial HYB
-// product, an ELL pass then a COO pass), into [mro[r], mro[r + 1]).
+// HYB merged copy, fill: row r's ELL entries in slot order, then its COO
+// entries in storage order (the order of the Serial HYB product, an ELL pass
+// then a COO pass), into [mro[r], mro[r + 1]).  The ELL part: one thread per
+// row; the COO part: one thread per entry (entry j of row r lands at
+// mro[r + 1] - tro[r + 1] + j, the row's range ending with its COO entries).
 template <typename T>
 __global__ void hyb_merge_fill_kernel(const int32_t* __restrict__ eci, const T* __restrict__ ev,
                                       int64_t pitch, int64_t loop, int64_t nrows,
-                                      const int32_t* __restrict__ tro,
-                                      const int32_t* __restrict__ cci, const T* __restrict__ cv,
                                       const int32_t* __restrict__ mro, int32_t* __restrict__ mci,
                                       T* __restrict__ mv) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1087,11 +1098,20 @@ __global__ void hyb_merge_fill_kernel(const int32_t* __restrict__ eci, const T* 
       o++;
     }
   }
-  for (int32_t j = tro[r]; j < tro[r + 1]; j++) {
-    mci[o] = cci[j];
-    mv[o] = cv[j];
-    o++;
-  }
+}
+
+template <typename T>
+__global__ void hyb_merge_tail_kernel(const int32_t* __restrict__ cri, int64_t nnz,
+                                      const int32_t* __restrict__ tro,
+                                      const int32_t* __restrict__ cci, const T* __restrict__ cv,
+                                      const int32_t* __restrict__ mro, int32_t* __restrict__ mci,
+                                      T* __restrict__ mv) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nnz) return;
+  const int32_t r = cri[j];
+  const int64_t o = (int64_t)mro[r + 1] - tro[r + 1] + j;
+  mci[o] = cci[j];
+  mv[o] = cv[j];
 }
 
 __global__ void rows_unsorted_kernel(const int32_t* __restrict__ ri, int64_t nnz,
@@ -1265,6 +1285,8 @@ int xtile_build(mph_matrix_impl* A, const std::vector<int32_t>& ro, const int32_
             hipSuccess)
       st = MPH_ERR_HIP;
     if (st == MPH_OK) {
+      hipLaunchKernelGGL(xtile_sorted_kernel, dim3((unsigned)nl), dim3(256), 0, s, d_ro, d_ci,
+                         d_rows, d_uns);
       hipLaunchKernelGGL(xtile_bounds_kernel, dim3((unsigned)ceil_div((int64_t)bnd.size(), 256)),
                          dim3(256), 0, s, d_ro, d_ci, d_rows, nl, ntiles, (int64_t)tile, d_bnd,
                          d_uns);
@@ -1837,24 +1859,19 @@ int hyb_merge(mph_matrix_impl* A, hipStream_t s) {
     set_error("hipMalloc failed for the HYB merged copy");
     return fail(MPH_ERR_OUT_OF_MEMORY);
   }
+  auto fill = [&](auto tag) {
+    using T = decltype(tag);
+    hipLaunchKernelGGL(hyb_merge_fill_kernel<T>, dim3(g), dim3(256), 0, s, A->ell_ci,
+                       static_cast<const T*>(A->ell_v), A->ell_pitch, loop, nrows, mro, p.m_ci,
+                       static_cast<T*>(p.m_val));
+    hipLaunchKernelGGL(hyb_merge_tail_kernel<T>, dim3((unsigned)ceil_div(A->coo_nnz, 256)),
+                       dim3(256), 0, s, A->coo_ri, A->coo_nnz, p.tail_ro, A->coo_ci,
+                       static_cast<const T*>(A->coo_v), mro, p.m_ci, static_cast<T*>(p.m_val));
+  };
   switch (vs) {
-    case 8:
-      hipLaunchKernelGGL(hyb_merge_fill_kernel<uint64_t>, dim3(g), dim3(256), 0, s, A->ell_ci,
-                         static_cast<const uint64_t*>(A->ell_v), A->ell_pitch, loop, nrows,
-                         p.tail_ro, A->coo_ci, static_cast<const uint64_t*>(A->coo_v), mro, p.m_ci,
-                         static_cast<uint64_t*>(p.m_val));
-      break;
-    case 4:
-      hipLaunchKernelGGL(hyb_merge_fill_kernel<uint32_t>, dim3(g), dim3(256), 0, s, A->ell_ci,
-                         static_cast<const uint32_t*>(A->ell_v), A->ell_pitch, loop, nrows,
-                         p.tail_ro, A->coo_ci, static_cast<const uint32_t*>(A->coo_v), mro, p.m_ci,
-                         static_cast<uint32_t*>(p.m_val));
-      break;
-    default:
-      hipLaunchKernelGGL(hyb_merge_fill_kernel<uint16_t>, dim3(g), dim3(256), 0, s, A->ell_ci,
-                         static_cast<const uint16_t*>(A->ell_v), A->ell_pitch, loop, nrows,
-                         p.tail_ro, A->coo_ci, static_cast<const uint16_t*>(A->coo_v), mro, p.m_ci,
-                         static_cast<uint16_t*>(p.m_val));
+    case 8: fill(uint64_t{}); break;
+    case 4: fill(uint32_t{}); break;
+    default: fill(uint16_t{}); break;
   }
   if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
     set_error("HYB merge fill failed");
