@@ -45,6 +45,7 @@ void release_plans(mph_matrix_impl* A) {
   dfree(p.skrec);
   if (p.smap) dfree(p.smap);
   if (p.sro) dfree(p.sro);
+  dfree(p.ell_cnt);
   xtile_release(p.xt);
   p = RowBlockPlan{};
   CooPlan& c = A->coo;
@@ -282,7 +283,9 @@ int plan_bytes(mph_matrix_t A, int init, int64_t* out) {
         break;
       }
       int64_t pre = 0;
-      if (hyb) {
+      if (hyb && p.ell_cnt) {
+        pre = p.ell_visited * (si + sv) + A->nrows;  // the slots up to each row's count
+      } else if (hyb) {
         pre = A->ell_loop * A->nrows * (si + sv);
       } else {
         pre = dia_part();

@@ -118,6 +118,11 @@ struct RowBlockPlan {
   bool split = false;            // HYB / HDC: ELL / DIA pass, then the list
   int32_t* smap = nullptr;       // split: list row -> matrix row
   int32_t* sro = nullptr;        // split: item offsets of the list rows (+1)
+  // split HYB whose ELL slots are mostly padding: per row, 1 + its last slot
+  // holding a column (0 for a row without any); the ELL pass then loads no
+  // slot beyond it, so lines holding only padding are never fetched
+  uint8_t* ell_cnt = nullptr;
+  int64_t ell_visited = 0;       // the sum of ell_cnt
 };
 
 // Analysis of a row-sorted COO matrix for the segmented reduction.

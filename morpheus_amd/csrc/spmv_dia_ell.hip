@@ -145,6 +145,9 @@ struct EllArgs {
   typename vtraits<V>::vec* __restrict__ y;
   int32_t ld;
   int init;
+  // ell_kernel only: per row, 1 + its last slot holding a column (nullptr:
+  // every row visits `loop` slots)
+  const uint8_t* __restrict__ cnt;
 };
 
 template <typename V, int KR, int RPT>
@@ -158,11 +161,30 @@ __global__ __launch_bounds__(256) void ell_kernel(EllArgs<V> a) {
   for (int u = 0; u < RPT; u++)
 #pragma unroll
     for (int q = 0; q < KR; q++) t[u][q] = 0;
+  // slots each row visits, and the wave's loop bound (uniform)
+  int32_t lim[RPT];
+  int32_t wl = a.loop;
+  if (a.cnt) {
+    int32_t m = 0;
+#pragma unroll
+    for (int u = 0; u < RPT; u++) {
+      const int64_t row = row0 + u * 256;
+      lim[u] = row < a.nrows ? (int32_t)a.cnt[row] : 0;
+      m = max(m, lim[u]);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o));
+    wl = __builtin_amdgcn_readfirstlane(m);
+  } else {
+#pragma unroll
+    for (int u = 0; u < RPT; u++) lim[u] = a.loop;
+  }
   // batches of 8 slots: indices first, then values and x entries
-  for (int n0 = 0; n0 < a.loop; n0 += 8) {
-    // index and value loads are unconditional (slots outside the matrix read
-    // slot 0 and are discarded): a load under a branch is waited on where
-    // the branch joins, which left one index load in flight at a time
+  for (int n0 = 0; n0 < wl; n0 += 8) {
+    // index and value loads are unconditional (slots outside the matrix, or
+    // beyond the row's count, read slot 0 and are discarded): a load under a
+    // branch is waited on where the branch joins, which left one index load
+    // in flight at a time
     int64_t c[RPT][8];
     int64_t slot[RPT][8];
 #pragma unroll
@@ -171,7 +193,7 @@ __global__ __launch_bounds__(256) void ell_kernel(EllArgs<V> a) {
       for (int u = 0; u < RPT; u++) {
         const int64_t row = row0 + u * 256;
         const int n = n0 + w;
-        const bool in = n < a.loop && row < a.nrows;
+        const bool in = n < lim[u] && row < a.nrows;
         slot[u][w] = in ? row + a.pitch * n : 0;
         const int32_t ci = ld_stream(a.ci + slot[u][w]);
         c[u][w] = in ? (int64_t)ci : -1;
@@ -516,6 +538,7 @@ int launch_ell_k(mph_matrix_impl* A, const void* x, void* y, int ld, int init,
   a.y = static_cast<typename vtraits<V>::vec*>(y);
   a.ld = ld;
   a.init = init;
+  a.cnt = A->rb.ell_cnt;
   const int64_t nb = ceil_div(A->nrows, 256 * RPT);
   if (nb == 0) return MPH_OK;
   hipLaunchKernelGGL((ell_kernel<V, KR, RPT>), dim3((unsigned)nb), dim3(256), 0,
@@ -647,6 +670,7 @@ template <bool DIA>
 bool pair_ok(const mph_matrix_impl* A, int k) {
   if (k != 1 || A->tuning.rows_per_thread != 0) return false;
   if (A->vt != MPH_F32 && A->vt != MPH_BF16) return false;
+  if (!DIA && A->rb.ell_cnt) return false;  // the counted one-row kernel
   const int64_t pitch = DIA ? A->dia_pitch : A->ell_pitch;
   const uintptr_t v = reinterpret_cast<uintptr_t>(DIA ? A->dia_v : A->ell_v);
   const size_t pair_bytes = 2 * value_size(A->vt);

@@ -1609,6 +1609,48 @@ int max_row_len(const int32_t* d_ro, int64_t nrows, hipStream_t s, int32_t* out)
   return MPH_OK;
 }
 
+// per row, 1 + its last ELL slot holding a column (0 without any)
+__global__ void ell_count_kernel(const int32_t* __restrict__ eci, int64_t pitch, int64_t loop,
+                                 int64_t nrows, uint8_t* __restrict__ cnt) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  int32_t c = 0;
+  for (int64_t n = 0; n < loop; n++)
+    if (eci[r + pitch * n] >= 0) c = (int32_t)n + 1;
+  cnt[r] = (uint8_t)c;
+}
+
+// A split HYB's ELL pass visits each row's slots up to its last filled one
+// when that at least halves the slots visited (p.ell_cnt); otherwise every
+// row visits ell_loop slots.
+int ell_counts(mph_matrix_impl* A, hipStream_t s) {
+  RowBlockPlan& p = A->rb;
+  const int64_t nrows = A->nrows, loop = A->ell_loop;
+  if (loop <= 0 || loop > 255 || nrows == 0) return MPH_OK;
+  uint8_t* d = nullptr;
+  MPH_HIP_CHECK(hipMalloc(&d, (size_t)nrows));
+  hipLaunchKernelGGL(ell_count_kernel, dim3((unsigned)ceil_div(nrows, 256)), dim3(256), 0, s,
+                     A->ell_ci, A->ell_pitch, loop, nrows, d);
+  std::vector<uint8_t> h(nrows);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d, (size_t)nrows, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    dfree(d);
+    set_error("ELL slot count analysis failed");
+    return MPH_ERR_HIP;
+  }
+  int64_t visited = 0;
+  for (int64_t r = 0; r < nrows; r++) visited += h[r];
+  if (2 * visited > nrows * loop) {
+    dfree(d);
+    return MPH_OK;
+  }
+  p.ell_cnt = d;
+  p.ell_visited = visited;
+  return MPH_OK;
+}
+
 int split_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
   RowBlockPlan& p = A->rb;
   const int64_t nrows = A->nrows;
@@ -1777,6 +1819,7 @@ int split_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
   p.split = split;
   p.smap = d_map;
   p.sro = d_lro;
+  if (split && A->format == MPH_HYB) MPH_CHECK_STATUS(ell_counts(A, s));
   return MPH_OK;
 }
 
