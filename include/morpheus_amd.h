@@ -165,7 +165,15 @@ int mph_matrix_plan_bytes(mph_matrix_t A, int init, int64_t* bytes);
  *   rows_per_thread : DIA / ELL rows per thread (1 or 2, strided by 256 rows);
  *                  0 = one row in FP64, and in FP32 / BF16 SpMV two adjacent
  *                  rows whose values (and ELL column indices) are loaded
- *                  as pairs, when the pitch is even
+ *                  as pairs, when the pitch is even; with 0, DIA SpMV (and
+ *                  HDC's DIA pass) stages the x entries of each cluster of
+ *                  nearby diagonals (offsets within 512, two or more
+ *                  diagonals; at most 64 diagonals and 8 such clusters) in
+ *                  LDS once per workgroup; the row-pair kernels do so when
+ *                  the clusters average 8 diagonals or more.  The
+ *                  environment variable MPH_DIA_STAGE, read at analysis,
+ *                  sets 0 = no staging, 1 = the one-row kernels only.
+ *                  The results do not depend on it.
  *   xtile        : x-tiled long rows of CSR / COO / HYB / HDC SpMV (1 = off,
  *                  2 = on; 0 = on when x exceeds 2 MiB).  Rows longer than
  *                  512 entries with sorted columns are summed per 32 KiB tile

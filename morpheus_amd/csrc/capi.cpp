@@ -54,6 +54,7 @@ void release_plans(mph_matrix_impl* A) {
   if (c.chunks) dfree(c.chunks);
   xtile_release(c.xt);
   c = CooPlan{};
+  A->dia_stage = DiaStage{};
   A->analysed = false;
 }
 
@@ -125,10 +126,14 @@ int analyse(mph_matrix_t A, hipStream_t s) {
       st = coo_analyse(A, s);
       break;
     case MPH_CSR:
-    case MPH_HDC:
       st = rowblock_analyse(A, s);
       break;
+    case MPH_HDC:
+      st = dia_analyse(A, s);
+      if (st == MPH_OK) st = rowblock_analyse(A, s);
+      break;
     case MPH_DIA:
+      st = dia_analyse(A, s);
       break;
     case MPH_ELL:
     case MPH_HYB: {

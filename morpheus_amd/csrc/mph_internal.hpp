@@ -125,6 +125,32 @@ struct RowBlockPlan {
   int64_t ell_visited = 0;       // the sum of ell_cnt
 };
 
+// DIA SpMV with x staged in LDS: the diagonals, in storage order, cut into
+// clusters of consecutive diagonals whose offsets lie within kDiaClusterSpan
+// of each other.  A workgroup of 256 rows stages, for each cluster of two or
+// more diagonals, the x entries its rows read through that cluster
+// (256 + span of them) once; a one-diagonal cluster reads x directly.
+constexpr int kDiaMaxClusters = 8;
+constexpr int kDiaClusterSpan = 512;
+constexpr int kDiaStageMax = 2048;  // x entries staged per workgroup
+constexpr int kDiaStageMaxDiags = 64;
+constexpr int kDiaPairMinDiags = 8;  // row-pair kernels: mean diagonals per staged cluster
+constexpr int32_t kDiaDirect = INT32_MIN;
+struct DiaStage {
+  int32_t n = 0;                    // staged clusters (0: the plain DIA kernel)
+  int32_t omin[kDiaMaxClusters];    // smallest offset of the cluster
+  int32_t base[kDiaMaxClusters];    // first LDS entry of its window
+  int32_t len[kDiaMaxClusters];     // entries of its window (256 + span)
+  // per diagonal: the LDS entry holding x[r0 + 0] for its cluster's window
+  // (base - omin), or kDiaDirect for a diagonal read from x directly
+  int32_t lb[kDiaStageMaxDiags];
+  // the same for the row-pair kernel's workgroups of 512 rows (windows of
+  // 512 + span entries, cluster c's at base + 256 c), when they fit in
+  // 2 kDiaStageMax entries (pair != 0)
+  int32_t pair = 0;
+  int32_t lb2[kDiaStageMaxDiags];
+};
+
 // Analysis of a row-sorted COO matrix for the segmented reduction.
 // A COO wave chunk: entries [s, e).
 struct CooChunk {
@@ -164,6 +190,7 @@ struct mph_matrix_impl {
   int64_t ndiag = 0, dia_pitch = 0, dia_nnz = 0;
   const int32_t* dia_off = nullptr;
   const void* dia_v = nullptr;
+  DiaStage dia_stage;  // from the analysis
   // ELL / HYB-ELL
   int64_t ell_K = 0, ell_width = 0, ell_pitch = 0, ell_nnz = 0;
   int64_t ell_loop = 0;  // slots visited per row (K, or width if needed)
@@ -186,6 +213,8 @@ int rowblock_multiply(mph_matrix_impl* A, int k, const void* x, void* y,
 int coo_analyse(mph_matrix_impl* A, hipStream_t s);
 int coo_multiply(mph_matrix_impl* A, int k, const void* x, void* y, int init,
                  hipStream_t s);
+// Clusters A's diagonals for the LDS-staged DIA SpMV (A->dia_stage).
+int dia_analyse(mph_matrix_impl* A, hipStream_t s);
 int dia_multiply(mph_matrix_impl* A, int k, const void* x, void* y, int init,
                  hipStream_t s);
 int ell_multiply(mph_matrix_impl* A, int k, const void* x, void* y, int init,
