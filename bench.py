@@ -71,8 +71,12 @@ def product_kernels(fmt, tuning):
     launch order, with the launch shape `tuning` picked: Z15's long rows take
     the x-tiled pass; row-sorted COO runs the row-block plan unless a
     segmented shape (coo_items) was picked; Z15's HYB (ELL slots about 10 %
-    filled) runs over its merged CSR copy; S7's HDC has an empty CSR part."""
+    filled) runs over its merged CSR copy; S7's HDC has an empty CSR part;
+    S7's DIA stages its in-plane diagonals' x entries in LDS (the default
+    launch shape) or takes the plain kernel (a rows_per_thread shape)."""
     rpt = 2 if tuning.get("rows_per_thread") == 2 else 1
+    dia = (["dia_staged_kernel<double>"] if not tuning.get("rows_per_thread")
+           else [f"dia_kernel<double, 1, {rpt}>"])
     items = tuning.get("coo_items") or 0
     xt = ["xtile_kernel<double, false>", "xtile_combine_kernel<double, 0, false>"]
     rows = ["rowblock_spmv_kernel<double, 0, false>", *xt]
@@ -82,8 +86,8 @@ def product_kernels(fmt, tuning):
             "coo_zero_kernel<double>", *xt, f"coo_segmented_kernel<double, {items}, false>",
             f"coo_fixup_kernel<double, {items}, false>"],
         "HYB": rows,
-        "HDC": [f"dia_kernel<double, 1, {rpt}>"],
-        "DIA": [f"dia_kernel<double, 1, {rpt}>"],
+        "HDC": dia,
+        "DIA": dia,
         "ELL": [f"ell_kernel<double, 1, {rpt}>"],
     }[fmt]
 
@@ -803,8 +807,9 @@ def run_one_shot(z15, reps=20):
 
 def run_no_copies(mats, vecs, order, reps=20):
     """The products of the Morpheus drop-in containers: the same matrices
-    through handles with MPH_POLICY_NO_VALUE_COPIES (no x-tiled copy, no HYB
-    merged copy: every product reads the caller's value arrays), for the
+    through handles with MPH_POLICY_NO_VALUE_COPIES (the x-tiled pass reads
+    the values in place, no HYB merged copy: every product reads the caller's
+    value arrays), for the
     formats whose default plan keeps a copy on these matrices; mean of `reps`
     back-to-back products, and the plan's bytes."""
     import numpy as np

@@ -138,12 +138,14 @@ int mph_matrix_analyse(mph_matrix_t A, mph_stream_t stream);
 
 /* Analysis policy (no reference counterpart: reference containers hold no
  * derived state, core/src/Morpheus_CsrMatrix.hpp:148-163).
- * MPH_POLICY_NO_VALUE_COPIES: the analysis keeps no copy of any value (no
- * x-tiled pass, no HYB merged copy: a large HYB runs its ELL part as a pass
- * of its own, each row stopping at its last filled slot when the slots are
- * mostly padding, then adds its COO tail); every product reads the caller's
- * value arrays, so values written in place are always seen.  The Morpheus
- * drop-in containers set it.  Changing the policy releases the analysis. */
+ * MPH_POLICY_NO_VALUE_COPIES: the analysis keeps no copy of any value: the
+ * x-tiled pass copies the long rows' columns only and reads their values in
+ * place, and a large HYB keeps no merged copy (it runs its ELL part as a
+ * pass of its own, each row stopping at its last filled slot when the slots
+ * are mostly padding, then adds its COO tail); every product reads the
+ * caller's value arrays, so values written in place are always seen.  The
+ * Morpheus drop-in containers set it.  Changing the policy releases the
+ * analysis. */
 #define MPH_POLICY_NO_VALUE_COPIES 1
 int mph_matrix_set_policy(mph_matrix_t A, int flags);
 int mph_matrix_format(mph_matrix_t A);

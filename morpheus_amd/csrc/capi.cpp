@@ -220,6 +220,8 @@ int64_t csr_plan_bytes(const mph_matrix_impl* A, const RowBlockPlan& p, int64_t 
     // tile-major copy (16-bit column + value per slot), the (tile, row)
     // partials written and read back, the tiled rows' ids
     b += tiled * (2 + sv) + 2 * (int64_t)p.xt.ntiles * p.xt.rows * acc + (int64_t)p.xt.rows * si;
+    // a direct plan reads the values in place and each run's position
+    if (p.xt.rpos) b += (int64_t)p.xt.groups * p.xt.ntiles * (p.xt.gr + 1) * si;
   }
   if (p.split) b += list_rows * si;  // the list's row map
   return b;

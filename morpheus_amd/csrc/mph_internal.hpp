@@ -72,6 +72,11 @@ struct XTilePlan {
   uint16_t* col = nullptr;       // tile-major copy: column - tile * tile width
   void* val = nullptr;           // tile-major copy: values
   void* partial = nullptr;       // [ntiles * rows] (pairs in FP32), tile-major
+  // direct plan (no copy of any value): the position in the caller's arrays
+  // of each run's first item (lro's shape), and the caller's value array the
+  // pass reads instead of `val`
+  int32_t* rpos = nullptr;
+  const void* src = nullptr;
 };
 
 // Analysis of a CSR-shaped item set (CSR itself, the CSR part of HDC, the
@@ -253,10 +258,11 @@ uint8_t block_lanes(int nr, const int32_t* ro_begin);
 // the candidate items would be).
 // tiled[r] != 0 marks the rows taken.  xtile_multiply runs the pass of a
 // plan without DIA / ELL parts (COO) and writes y of the tiled rows (x and y
-// read with element stride ld).
+// read with element stride ld).  direct: the plan copies the columns only;
+// the pass reads the values from d_val at each run's position.
 int xtile_build(mph_matrix_impl* A, const std::vector<int32_t>& ro, const int32_t* d_ro,
                 const int32_t* d_ci, const void* d_val, std::vector<int32_t> lrows,
-                XTilePlan& xp, std::vector<char>& tiled, hipStream_t s);
+                XTilePlan& xp, std::vector<char>& tiled, hipStream_t s, bool direct = false);
 void xtile_release(XTilePlan& xp);
 int xtile_multiply(mph_matrix_impl* A, const XTilePlan& xp, const void* x, void* y, int init,
                    int64_t ld, hipStream_t s);
@@ -267,8 +273,9 @@ int default_chunk(int vt);
 // scratch (spmv_coo.hip), looked up on first use per device
 int noalloc_scratch(void** out);
 
-// the handle's x-tiling mode (tuning.xtile: 0 auto, 1 off, 2 on); off when
-// its policy forbids value copies
+// the x-tiling mode of the segmented COO pass (tuning.xtile: 0 auto, 1 off,
+// 2 on); off when the policy forbids value copies (that pass has no direct
+// plan; the row-block plans read the values in place instead)
 inline int xtile_mode(const mph_matrix_impl* A) {
   return (A->policy & MPH_POLICY_NO_VALUE_COPIES) ? 1 : A->tuning.xtile;
 }

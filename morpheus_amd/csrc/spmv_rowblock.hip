@@ -402,9 +402,11 @@ __global__ __launch_bounds__(kRowBlockThreads) void rowblock_spmv_kernel(
 // LDS of xtile_kernel: x tile, a group's run offsets and its sub-block
 // records (a group has at most kXTileMaxGroupRows rows, so
 // at most as many sub-blocks, plus a sentinel).
-inline size_t xtile_lds_bytes(size_t vec, size_t /*acc*/, int32_t tile) {
+// (A direct plan keeps its sub-block records in memory and the runs'
+// positions in LDS instead.)
+inline size_t xtile_lds_bytes(size_t vec, size_t /*acc*/, int32_t tile, bool direct = false) {
   return vec * (size_t)tile + sizeof(int32_t) * (kXTileMaxGroupRows + 2) +
-         sizeof(XTSub) * (kXTileMaxGroupRows + 1);
+         (direct ? sizeof(int32_t) : sizeof(XTSub)) * (kXTileMaxGroupRows + 1);
 }
 
 struct XTArgs {
@@ -415,15 +417,16 @@ struct XTArgs {
   const int32_t* __restrict__ sub_start;
   const uint8_t* __restrict__ head;
   const uint16_t* __restrict__ col;
-  const void* __restrict__ val;
+  const void* __restrict__ val;  // the copy's values, or (direct) the caller's
   void* __restrict__ partial;
+  const int32_t* __restrict__ rpos;  // direct plans: position of each run's first item
 };
 
 // Items of one sub-block as loaded by a wave: lane l holds the kBatch
 // consecutive items l*kBatch .. l*kBatch + kBatch - 1.
 inline XTArgs xt_args(const XTilePlan& p) {
   return XTArgs{p.tile, p.ntiles, p.groups, p.gr, p.rows, p.row, p.lro,
-                p.sub, p.sub_start, p.head, p.col, p.val, p.partial};
+                p.sub, p.sub_start, p.head, p.col, p.rpos ? p.src : p.val, p.partial, p.rpos};
 }
 
 // A sub-block record is the same for every lane of the wave that reads it:
@@ -679,6 +682,195 @@ __global__ __launch_bounds__(kXTileThreads) void xtile_kernel(RBArgs<V> a, XTArg
   }
 }
 
+// The x-tiled pass of a direct plan (no copy of any value): the columns come
+// from the column copy (stored in item order), the values from the caller's
+// array.  Run r of a sub-block holds its block items ro[r] .. ro[r + 1] - 1,
+// the first at position rp[r] of the caller's arrays (a run is a contiguous
+// piece of its row).  Lane l takes items l, l + 64, ..., l + 448 of the
+// sub-block, so that each load instruction reads 64 consecutive items --
+// consecutive positions inside a run -- instead of items eight apart; an
+// item's run comes from a binary search over the run offsets in LDS.  The
+// runs are then reduced chunk by chunk (items 64 c .. 64 c + 63): a
+// segmented scan across the wave, a run's total written by the lane holding
+// its last item, and a run that continues into the next chunk carried to
+// that chunk's first lane.  LDS: x tile | run offsets | run positions (the
+// sub-block records are read through scalar loads).
+template <typename V>
+struct XDLoad {
+  uint32_t c[kBatch];
+  int32_t r[kBatch];
+  typename XTHeld<V>::T v[kBatch];
+};
+
+// chunk c (items 64 c + lane) of a direct plan's sub-block: column, run
+// and value
+template <typename V>
+__device__ __forceinline__ void xtile_direct_load1(const XTArgs& t, const V* val,
+                                                   const int32_t* ro, const int32_t* rp,
+                                                   const XTSub& s0, int32_t nr, int32_t n,
+                                                   int lane, int c, XDLoad<V>& d) {
+  const int32_t j = c * kWave + lane;
+  const int32_t E = s0.e0 + j;
+  // the last run r < nr with ro[r] <= E (nr <= 64 runs)
+  int32_t r = 0;
+#pragma unroll
+  for (int step = kWave / 2; step > 0; step >>= 1)
+    if (r + step < nr && ro[r + step] <= E) r += step;
+  d.r[c] = r;
+  const int32_t pos = j < n ? rp[r] + (E - ro[r]) : 0;
+  d.c[c] = (uint32_t)ld_stream(t.col + (int64_t)s0.p0 + (j < n ? j : 0));
+  d.v[c] = XTHeld<V>::hold(ld_stream(val + pos));
+}
+
+template <typename S>
+__device__ __forceinline__ S readlane_sum(const S& v, int l) {
+  auto rl = [&](auto x) {
+    using T = decltype(x);
+    if constexpr (sizeof(T) == 8) {
+      const uint64_t u = __builtin_bit_cast(uint64_t, x);
+      const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+      const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+      return __builtin_bit_cast(T, ((uint64_t)hi << 32) | lo);
+    } else {
+      return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), l));
+    }
+  };
+  return S{rl(v.s), rl(v.c)};
+}
+
+template <typename V, bool COMP>
+__global__ __launch_bounds__(kXTileThreads) void xtile_direct_kernel(RBArgs<V> a, XTArgs t) {
+  using Acc = typename vtraits<V>::acc;
+  using Vec = typename vtraits<V>::vec;
+  using S = XSum<Acc, COMP>;
+  constexpr int kWaves = kXTileThreads / kWave;
+  extern __shared__ __align__(16) unsigned char smem[];
+  Vec* xs = reinterpret_cast<Vec*>(smem);
+  int32_t* s_lro = reinterpret_cast<int32_t*>(xs + t.tile);
+  int32_t* s_rpos = s_lro + kXTileMaxGroupRows + 2;
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const uint32_t id = blockIdx.x;
+  const uint32_t k = id / kNumXcd;
+  const int32_t g = (int32_t)(k % (uint32_t)t.groups);
+  const int32_t tile = (int32_t)((k / (uint32_t)t.groups) * kNumXcd + id % kNumXcd);
+  if (tile >= t.ntiles) return;  // whole workgroup: no barrier reached
+  const int64_t c0 = (int64_t)tile * t.tile;
+  const int32_t nx = (int32_t)min<int64_t>(t.tile, a.ncols - c0);
+  const int64_t blk = (int64_t)g * t.ntiles + tile;
+  const int32_t* lro = t.lro + blk * (t.gr + 1);
+  const int32_t* rpo = t.rpos + blk * (t.gr + 1);
+  const int32_t gstart = g * t.gr;
+  const int32_t sb0 = t.sub_start[blk];
+  const int32_t nsub = t.sub_start[blk + 1] - 1 - sb0;  // sub-blocks (a sentinel follows)
+  const V* val = static_cast<const V*>(t.val);
+  Acc* part = static_cast<Acc*>(t.partial);
+  {
+    constexpr int kPer = 8;  // x loads in flight per thread
+    for (int32_t j0 = threadIdx.x; j0 < nx; j0 += kXTileThreads * kPer) {
+      Vec xv[kPer];
+#pragma unroll
+      for (int q = 0; q < kPer; q++) {
+        const int32_t j = j0 + q * kXTileThreads;
+        xv[q] = j < nx ? a.x[(c0 + j) * a.ld] : (Vec)0;
+      }
+#pragma unroll
+      for (int q = 0; q < kPer; q++) {
+        const int32_t j = j0 + q * kXTileThreads;
+        if (j < nx) xs[j] = xv[q];
+      }
+    }
+  }
+  for (int32_t j = threadIdx.x; j <= t.gr; j += kXTileThreads) {
+    s_lro[j] = lro[j];
+    s_rpos[j] = rpo[j];
+  }
+  __syncthreads();
+
+  // (a wave-uniform sub-block index: the records come through scalar loads)
+  auto rec = [&](int32_t q) { return uniform_sub(t.sub[__builtin_amdgcn_readfirstlane(sb0 + q)]); };
+  XTSub s0{0, 0}, s1{0, 0};
+  XDLoad<V> cur;
+  if (wave < nsub) {
+    s0 = rec(wave);
+    s1 = rec(wave + 1);
+#pragma unroll
+    for (int c = 0; c < kBatch; c++)
+      xtile_direct_load1<V>(t, val, s_lro + (s0.i0 - gstart), s_rpos + (s0.i0 - gstart), s0,
+                            s1.i0 - s0.i0, s1.e0 - s0.e0, lane, c, cur);
+  }
+  for (int32_t q = wave; q < nsub; q += kWaves) {
+    const int32_t nr = s1.i0 - s0.i0, n = s1.e0 - s0.e0;
+    const int32_t i0 = s0.i0;
+    const int32_t* ro = s_lro + (i0 - gstart);
+    auto put = [&](int32_t r, const S& v) {
+      const int64_t o = (int64_t)tile * t.rows + i0 + r;
+      if constexpr (COMP) {
+        part[2 * o] = v.s;
+        part[2 * o + 1] = v.c;
+      } else {
+        part[o] = v.s;
+      }
+    };
+    S carry = S::zero();
+    int32_t carry_run = -1;
+#pragma unroll
+    for (int c = 0; c < kBatch; c++) {
+      if (c * kWave >= n) break;  // uniform
+      const int32_t j = c * kWave + lane;
+      const bool valid = j < n;
+      const int32_t r = cur.r[c];
+      // (the clamp keeps a corrupted column inside the LDS tile)
+      const Acc xv = (Acc)xs[min(cur.c[c], (uint32_t)(nx - 1))];
+      S acc = S::zero();
+      if (valid) {
+        if constexpr (COMP)
+          acc.add_prod(XTHeld<V>::use(cur.v[c]), xv);
+        else
+          acc.add(XTHeld<V>::use(cur.v[c]) * xv);
+      }
+      if (lane == 0 && valid && r == carry_run) {
+        S tmp = carry;
+        tmp.merge(acc);
+        acc = tmp;
+      }
+      const int32_t r_prev = dpp_i32<kDppWaveShr1>(r);
+      int flag = (lane == 0 || !valid || r_prev != r) ? 1 : 0;
+      const int rl = lane & 15;
+      xscan_step<kDppRowShr1, 0xf>(acc, flag, rl >= 1);
+      xscan_step<kDppRowShr2, 0xf>(acc, flag, rl >= 2);
+      xscan_step<kDppRowShr4, 0xf>(acc, flag, rl >= 4);
+      xscan_step<kDppRowShr8, 0xf>(acc, flag, rl >= 8);
+      xscan_step<kDppRowBcast15, 0xa>(acc, flag, (lane & 16) != 0);
+      xscan_step<kDppRowBcast31, 0xc>(acc, flag, lane >= 32);
+      // the run of the next chunk's first item (uniform; -1 past the items)
+      const int32_t next_run = (c + 1 < kBatch && (c + 1) * kWave < n)
+                                   ? __builtin_amdgcn_readlane(cur.r[c + 1 < kBatch ? c + 1 : c], 0)
+                                   : -1;
+      const int32_t r_next = dpp_i32<kDppWaveShl1>(r);
+      const bool last = valid && (lane == kWave - 1 ? next_run != r : (j + 1 >= n || r_next != r));
+      if (last) put(r, acc);
+      carry = readlane_sum(acc, kWave - 1);
+      carry_run = (c * kWave + kWave - 1 < n && next_run == __builtin_amdgcn_readlane(r, kWave - 1))
+                      ? next_run
+                      : -1;
+    }
+    // runs without items in this tile
+    if (lane < nr && ro[lane] == ro[lane + 1]) put(lane, S::zero());
+    // the next sub-block's items, requested once this one is reduced (its
+    // loads and this one's products are never live together; requested per
+    // chunk between the reductions, they made the pass slower: 76 -> 85 us
+    // on Z15)
+    if (q + kWaves < nsub) {
+      s0 = rec(q + kWaves);
+      s1 = rec(q + kWaves + 1);
+#pragma unroll
+      for (int c = 0; c < kBatch; c++)
+        xtile_direct_load1<V>(t, val, s_lro + (s0.i0 - gstart), s_rpos + (s0.i0 - gstart), s0,
+                              s1.i0 - s0.i0, s1.e0 - s0.e0, lane, c, cur);
+    }
+  }
+}
+
 // Analysis: the tile-major copy.  Thread (tile q, long row i) copies the
 // row's items [bnd(q, i), bnd(q + 1, i)) to its run of the (group, tile)
 // block, placing item j of a sub-block at slot p0 + (j % 8) * S + j / 8.
@@ -688,7 +880,7 @@ __global__ void xtile_copy_kernel(const int32_t* __restrict__ bnd, int32_t nl, i
                                   const int32_t* __restrict__ run_sub,
                                   const XTSub* __restrict__ sub, const int32_t* __restrict__ ci,
                                   const V* __restrict__ val, uint16_t* __restrict__ col_out,
-                                  V* __restrict__ val_out) {
+                                  V* __restrict__ val_out, int direct) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)nl * ntiles) return;
   const int64_t q = t / nl, i = t % nl;
@@ -701,9 +893,11 @@ __global__ void xtile_copy_kernel(const int32_t* __restrict__ bnd, int32_t nl, i
   const int32_t first = lro[w] - s0.e0;  // the run's first item within the sub-block
   for (int32_t j = b; j < e; j++) {
     const int32_t L = first + (j - b);
-    const int64_t slot = (int64_t)s0.p0 + (int64_t)(L % 8) * S + L / 8;
+    // a direct plan keeps the columns in item order (its pass gives lane l
+    // the items l, l + 64, ...), the copy plan transposed
+    const int64_t slot = (int64_t)s0.p0 + (direct ? (int64_t)L : (int64_t)(L % 8) * S + L / 8);
     col_out[slot] = (uint16_t)(ci[j] - q * tile);
-    val_out[slot] = val[j];
+    if (val_out) val_out[slot] = val[j];
   }
 }
 
@@ -1235,7 +1429,8 @@ void launch_xtile_copy(const int32_t* bnd, int32_t nl, int32_t ntiles, int32_t g
   const int64_t n = (int64_t)nl * ntiles;
   hipLaunchKernelGGL((xtile_copy_kernel<V>), dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s,
                      bnd, nl, ntiles, gr, tile, lro, run_sub, sub, ci,
-                     static_cast<const V*>(val), col_out, static_cast<V*>(val_out));
+                     static_cast<const V*>(val), col_out, static_cast<V*>(val_out),
+                     val_out ? 0 : 1);
 }
 
 }  // namespace
@@ -1245,7 +1440,7 @@ void launch_xtile_copy(const int32_t* bnd, int32_t nl, int32_t ntiles, int32_t g
 // then the tile-major copy is made on the device.
 int xtile_build(mph_matrix_impl* A, const std::vector<int32_t>& ro, const int32_t* d_ro,
                 const int32_t* d_ci, const void* d_val, std::vector<int32_t> lrows,
-                XTilePlan& xp, std::vector<char>& tiled, hipStream_t s) {
+                XTilePlan& xp, std::vector<char>& tiled, hipStream_t s, bool direct) {
   const int64_t nrows = A->nrows;
   const size_t vec_bytes = A->vt == MPH_F64 ? 8 : 4;
   tiled.assign(nrows, 0);
@@ -1378,6 +1573,8 @@ int xtile_build(mph_matrix_impl* A, const std::vector<int32_t>& ro, const int32_
   // run offsets, sub-blocks and the tiled rows' bounds (compacted)
   std::vector<int32_t> lro((size_t)nblk * (gr + 1));
   std::vector<int32_t> run_sub((size_t)nblk * (gr + 1), 0);
+  // direct plans: the position of each run's first item (0 past the group)
+  std::vector<int32_t> rpos(direct ? lro.size() : 0, 0);
   std::vector<XTSub> subs;
   std::vector<int32_t> sub_start(nblk + 1);
   std::vector<int32_t> tb((size_t)nt * (ntiles + 1));
@@ -1404,6 +1601,7 @@ int xtile_build(mph_matrix_impl* A, const std::vector<int32_t>& ro, const int32_
         }
         L[j - j0] = (int32_t)total;
         RS[j - j0] = (int32_t)subs.size() - 1;
+        if (direct) rpos[(size_t)k * (gr + 1) + (j - j0)] = B(q, trow[j]);
         total += len;
         cur_rows++;
         cur_items += len;
@@ -1457,6 +1655,7 @@ int xtile_build(mph_matrix_impl* A, const std::vector<int32_t>& ro, const int32_
   const size_t vsz = A->vt == MPH_F64 ? 8 : A->vt == MPH_F32 ? 4 : 2;
   const size_t acc = A->vt == MPH_F64 ? 8 : 4, pair = A->vt == MPH_F64 ? 1 : 2;
   int32_t *d_row = nullptr, *d_lro = nullptr, *d_ss = nullptr, *d_tb = nullptr, *d_rs = nullptr;
+  int32_t* d_rpos = nullptr;
   XTSub* d_sub = nullptr;
   uint8_t* d_head = nullptr;
   uint16_t* d_col = nullptr;
@@ -1472,6 +1671,7 @@ int xtile_build(mph_matrix_impl* A, const std::vector<int32_t>& ro, const int32_
     dfree(d_col);
     dfree(d_cv);
     dfree(d_part);
+    dfree(d_rpos);
   };
   if (hipMalloc(&d_row, sizeof(int32_t) * nt) != hipSuccess ||
       hipMalloc(&d_lro, sizeof(int32_t) * lro.size()) != hipSuccess ||
@@ -1481,7 +1681,8 @@ int xtile_build(mph_matrix_impl* A, const std::vector<int32_t>& ro, const int32_
       hipMalloc(&d_sub, sizeof(XTSub) * subs.size()) != hipSuccess ||
       hipMalloc(&d_head, head.size()) != hipSuccess ||
       hipMalloc(&d_col, sizeof(uint16_t) * std::max<int64_t>(slots, 1)) != hipSuccess ||
-      hipMalloc(&d_cv, vsz * std::max<int64_t>(slots, 1)) != hipSuccess ||
+      (!direct && hipMalloc(&d_cv, vsz * std::max<int64_t>(slots, 1)) != hipSuccess) ||
+      (direct && hipMalloc(&d_rpos, sizeof(int32_t) * rpos.size()) != hipSuccess) ||
       hipMalloc(&d_part, acc * pair * (size_t)ntiles * nt) != hipSuccess) {
     release();
     set_error("hipMalloc failed in the x-tile analysis");
@@ -1493,7 +1694,9 @@ int xtile_build(mph_matrix_impl* A, const std::vector<int32_t>& ro, const int32_
             hipMemcpyAsync(d_tb, tb.data(), sizeof(int32_t) * tb.size(), hipMemcpyHostToDevice, s) == hipSuccess &&
             hipMemcpyAsync(d_rs, run_sub.data(), sizeof(int32_t) * run_sub.size(), hipMemcpyHostToDevice, s) == hipSuccess &&
             hipMemcpyAsync(d_sub, subs.data(), sizeof(XTSub) * subs.size(), hipMemcpyHostToDevice, s) == hipSuccess &&
-            hipMemcpyAsync(d_head, head.data(), head.size(), hipMemcpyHostToDevice, s) == hipSuccess;
+            hipMemcpyAsync(d_head, head.data(), head.size(), hipMemcpyHostToDevice, s) == hipSuccess &&
+            (!direct || hipMemcpyAsync(d_rpos, rpos.data(), sizeof(int32_t) * rpos.size(),
+                                       hipMemcpyHostToDevice, s) == hipSuccess);
   if (ok) {
     if (A->vt == MPH_F64)
       launch_xtile_copy<double>(d_tb, nt, ntiles, gr, tile, d_lro, d_rs, d_sub, d_ci, d_val, d_col,
@@ -1518,16 +1721,19 @@ int xtile_build(mph_matrix_impl* A, const std::vector<int32_t>& ro, const int32_
   // with a halved tile launches with less)
   {
     hipError_t e = hipSuccess;
-    const size_t xlds = xtile_lds_bytes(vec_bytes, acc, tile_max);
+    const size_t xlds = xtile_lds_bytes(vec_bytes, acc, tile_max, direct);
+    auto attr = [&](const void* f) {
+      return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)xlds);
+    };
     if (A->vt == MPH_F64)
-      e = hipFuncSetAttribute(reinterpret_cast<const void*>(&xtile_kernel<double, false>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)xlds);
+      e = direct ? attr(reinterpret_cast<const void*>(&xtile_direct_kernel<double, false>))
+                 : attr(reinterpret_cast<const void*>(&xtile_kernel<double, false>));
     else if (A->vt == MPH_F32)
-      e = hipFuncSetAttribute(reinterpret_cast<const void*>(&xtile_kernel<float, true>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)xlds);
+      e = direct ? attr(reinterpret_cast<const void*>(&xtile_direct_kernel<float, true>))
+                 : attr(reinterpret_cast<const void*>(&xtile_kernel<float, true>));
     else
-      e = hipFuncSetAttribute(reinterpret_cast<const void*>(&xtile_kernel<bf16, true>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)xlds);
+      e = direct ? attr(reinterpret_cast<const void*>(&xtile_direct_kernel<bf16, true>))
+                 : attr(reinterpret_cast<const void*>(&xtile_kernel<bf16, true>));
     if (e != hipSuccess) {
       release();
       set_error(std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
@@ -1542,6 +1748,8 @@ int xtile_build(mph_matrix_impl* A, const std::vector<int32_t>& ro, const int32_
   xp.col = d_col;
   xp.val = d_cv;
   xp.partial = d_part;
+  xp.rpos = d_rpos;
+  xp.src = direct ? d_val : nullptr;
   xp.tile = tile;
   xp.ntiles = ntiles;
   xp.groups = ngroups;
@@ -1560,6 +1768,7 @@ void xtile_release(XTilePlan& xp) {
   dfree(xp.col);
   dfree(xp.val);
   dfree(xp.partial);
+  dfree(xp.rpos);
   xp = XTilePlan{};
 }
 
@@ -1655,7 +1864,10 @@ int split_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
   RowBlockPlan& p = A->rb;
   const int64_t nrows = A->nrows;
   const size_t vec_bytes = A->vt == MPH_F64 ? 8 : 4;
-  const int mode = xtile_mode(A);
+  // (with MPH_POLICY_NO_VALUE_COPIES the x-tiled pass reads the values in
+  // place: a direct plan)
+  const int mode = A->tuning.xtile;
+  const bool direct = (A->policy & MPH_POLICY_NO_VALUE_COPIES) != 0;
   const bool xt_on = mode == 2 || (mode == 0 && (int64_t)A->ncols * (int64_t)vec_bytes >= kXTileMinXBytes);
   // (an HDC without diagonals / a HYB without ELL slots is its CSR-shaped
   // part: no prefix pass)
@@ -1682,7 +1894,8 @@ int split_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
     std::vector<int32_t> lrows;
     for (int64_t r = 0; r < nrows; r++)
       if ((int64_t)ro[r + 1] - ro[r] > kXTileMinRowLen) lrows.push_back((int32_t)r);
-    if (!lrows.empty()) MPH_CHECK_STATUS(xtile_build(A, ro, d_ro, d_ci, d_val, lrows, p.xt, tiled, s));
+    if (!lrows.empty())
+      MPH_CHECK_STATUS(xtile_build(A, ro, d_ro, d_ci, d_val, lrows, p.xt, tiled, s, direct));
   }
   if (p.xt.rows == 0 && !split) return MPH_OK;
 
@@ -2084,9 +2297,14 @@ int launch_split_spmv(mph_matrix_impl* A, const void* x, void* y, int init, hipS
   if (p.xt.rows > 0) {
     XTArgs t = xt_args(p.xt);
     const int64_t tiles8 = ceil_div((int64_t)p.xt.ntiles, (int64_t)kNumXcd) * kNumXcd;
-    const size_t xlds = xtile_lds_bytes(sizeof(typename vtraits<V>::vec), sizeof(Acc), p.xt.tile);
-    hipLaunchKernelGGL((xtile_kernel<V, comp>), dim3((unsigned)(tiles8 * p.xt.groups)),
-                       dim3(kXTileThreads), xlds, s, a, t);
+    const bool direct = p.xt.rpos != nullptr;
+    const size_t xlds = xtile_lds_bytes(sizeof(typename vtraits<V>::vec), sizeof(Acc), p.xt.tile, direct);
+    if (direct)
+      hipLaunchKernelGGL((xtile_direct_kernel<V, comp>), dim3((unsigned)(tiles8 * p.xt.groups)),
+                         dim3(kXTileThreads), xlds, s, a, t);
+    else
+      hipLaunchKernelGGL((xtile_kernel<V, comp>), dim3((unsigned)(tiles8 * p.xt.groups)),
+                         dim3(kXTileThreads), xlds, s, a, t);
     hipLaunchKernelGGL((xtile_combine_kernel<V, PRE, comp>),
                        dim3((unsigned)ceil_div(p.xt.rows, kXCombRows)), dim3(256), 0, s, a, t);
   }
@@ -2282,9 +2500,14 @@ int xtile_multiply_t(mph_matrix_impl* A, const XTilePlan& xp, const void* x, voi
   a.init = init;
   XTArgs t = xt_args(xp);
   const int64_t tiles8 = ceil_div((int64_t)xp.ntiles, (int64_t)kNumXcd) * kNumXcd;
-  const size_t xlds = xtile_lds_bytes(sizeof(typename vtraits<V>::vec), sizeof(Acc), xp.tile);
-  hipLaunchKernelGGL((xtile_kernel<V, comp>), dim3((unsigned)(tiles8 * xp.groups)),
-                     dim3(kXTileThreads), xlds, s, a, t);
+  const bool direct = xp.rpos != nullptr;
+  const size_t xlds = xtile_lds_bytes(sizeof(typename vtraits<V>::vec), sizeof(Acc), xp.tile, direct);
+  if (direct)
+    hipLaunchKernelGGL((xtile_direct_kernel<V, comp>), dim3((unsigned)(tiles8 * xp.groups)),
+                       dim3(kXTileThreads), xlds, s, a, t);
+  else
+    hipLaunchKernelGGL((xtile_kernel<V, comp>), dim3((unsigned)(tiles8 * xp.groups)),
+                       dim3(kXTileThreads), xlds, s, a, t);
   hipLaunchKernelGGL((xtile_combine_kernel<V, PRE_NONE, comp>),
                      dim3((unsigned)ceil_div(xp.rows, kXCombRows)), dim3(256), 0, s, a, t);
   MPH_HIP_CHECK(hipGetLastError());

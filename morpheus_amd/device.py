@@ -103,8 +103,8 @@ class DeviceMatrix:
 
     def set_policy(self, flags: int):
         """mph_matrix_set_policy: capi.POLICY_NO_VALUE_COPIES keeps every
-        product on the caller's value arrays (no x-tiled copy, no HYB merged
-        copy)."""
+        product on the caller's value arrays (the x-tiled pass reads them in
+        place; no HYB merged copy)."""
         capi.call("mph_matrix_set_policy", self.handle, int(flags))
 
     def get_tuning(self):

@@ -3,8 +3,8 @@
 512 matrices (log-uniform 1k-64k rows; a third each of 2D 5-point stencils,
 power-law(1.5) rows with mean 12 and banded matrices of bandwidth 5-33; 7.6M
 rows, 89M entries in all), k = 8 row-major right-hand sides U[-1, 1], one
-persistent launch (direct and replayed from a hipGraph), in FP64, FP32 and
-BF16 storage.  Two format assignments: the autotuner's pick per matrix, and
+launch per kind of member over the batch's tile list (direct and replayed
+from a hipGraph), in FP64, FP32 and BF16 storage.  Two format assignments: the autotuner's pick per matrix, and
 every admissible format in turn (so that all six appear).  Two columns of
 every matrix's Y are checked against the oracle: FP64 per row within
 4 n_row u sum |a x|; FP32 / BF16 as max_i |y_i - y_ref_i| / sum_j |a_ij x_j|

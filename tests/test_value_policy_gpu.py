@@ -5,7 +5,7 @@ Reference containers hold no derived state
 current values.  Through the C-ABI a handle's analysis may keep copies of
 values (the x-tiled long rows' tile-major copy, a HYB's merged CSR copy);
 with the policy MPH_POLICY_NO_VALUE_COPIES (which the C++ drop-in sets) it
-keeps none, so values written in place are seen by the next product without
+keeps none (the x-tiled pass then reads the values in place), so values written in place are seen by the next product without
 a new analysis.  With the default policy the same product is right after
 mph_matrix_analyse.  Each product is checked against the Serial oracle on
 the rewritten matrix with the FP64 per-row bound.
@@ -79,9 +79,9 @@ def test_values_written_in_place(gpu, case, policy):
 
 @pytest.mark.parametrize("case", ["CSR x-tiled", "HYB merged"])
 def test_policy_reads_no_copy(gpu, case):
-    """The no-copies policy gives the x-tiled / merged plans up (the plan's
-    bytes drop by the copies it no longer reads) and stays within the bound;
-    switching it back restores the default plan."""
+    """The no-copies policy replaces the x-tiled copy by the direct pass and
+    gives the merged HYB plan up, and stays within the bound; switching it
+    back restores the default plan."""
     import torch
     from morpheus_amd import device
     H = _wide_csr() if case == "CSR x-tiled" else holey_hyb(70000, 0.2, 3)
