@@ -464,7 +464,10 @@ int mph_i32_stats(const int32_t* v, int64_t n, double mean, int32_t threshold,
  * and MPH_BATCH_CSR_ROWS_PER_GROUP (4 by default, or 1: rows a lane group
  * of a DIA / ELL resp. CSR-shaped row tile takes), MPH_BATCH_STAGE (1 by
  * default: narrow DIA members stage X rows in LDS), MPH_BATCH_WAVE_ROWS (1
- * by default: rows per wave of a one-wave tile). */
+ * by default: rows per wave of a one-wave tile), MPH_BATCH_SORT_ROWS (1 by
+ * default: the short rows of CSR-shaped members go to tiles sorted by row
+ * length), MPH_BATCH_SUB_MAX (128 by default: rows of 33 up to this many
+ * entries take four rows per wave; 0: one row per wave). */
 typedef struct mph_batch_s* mph_batch_t;
 int mph_batch_create(mph_batch_t* out, int32_t nmat, const mph_matrix_t* mats,
                      int32_t k, const void* const* X, void* const* Y,

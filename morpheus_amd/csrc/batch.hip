@@ -27,9 +27,13 @@
 //    costs of a tile (its record, the store of Y) are spread over R times as
 //    many rows.  Each row still accumulates its entries in order into one
 //    sum per column, whatever R.
-//  * CSR-shaped rows of kShortRowMax + 1 .. kGroupRowMax entries: list tiles,
-//    one row per lane group; of kGroupRowMax + 1 .. kWaveRowMax entries: one
-//    wave per row; longer rows: the whole workgroup.
+//  * The short rows of a CSR-shaped member without a DIA / ELL part go to row
+//    tiles over a row list sorted by item count, so the rows of a wave have
+//    about the same length.  CSR-shaped rows of kShortRowMax + 1 ..
+//    kGroupRowMax entries: list tiles, one row per lane group; of
+//    kGroupRowMax + 1 .. kSubRowMax entries (members without a DIA / ELL
+//    part): kSubRows rows per wave; up to kWaveRowMax entries: one wave per
+//    row; longer rows: the whole workgroup.
 //  * DIA members whose offsets span at most kStageSpan stage the X rows of a
 //    tile in LDS (each is read by every diagonal).  FP32 / BF16
 //    rows of more than kCompensateLen entries are summed as Dot2 pairs by a
@@ -43,7 +47,15 @@
 
 namespace mph {
 
-enum { TILE_ROWS = 0, TILE_WAVE = 1, TILE_WHOLE = 2, TILE_ROWS_STAGED = 3, TILE_LIST = 4 };
+enum {
+  TILE_ROWS = 0,
+  TILE_WAVE = 1,
+  TILE_WHOLE = 2,
+  TILE_ROWS_STAGED = 3,
+  TILE_LIST = 4,
+  TILE_SORTED = 5,
+  TILE_SUB = 6
+};
 enum { BPRE_NONE = 0, BPRE_DIA = 1, BPRE_ELL = 2 };
 
 // A member: an optional DIA / ELL part (pre) added to every row, and an
@@ -83,6 +95,8 @@ constexpr int kGroupRowMax = 32;   // rows a lane group takes in a row tile
 constexpr int kShortRowMax = 8;
 constexpr int kWaveRowMax = 512;   // rows one wave takes
 constexpr int kWaveRows = 1;       // rows per wave of a one-wave tile
+constexpr int kSubRows = 4;        // rows per wave of a sub-wave tile
+constexpr int kSubRowMax = 128;    // rows of kGroupRowMax + 1 .. this: sub-wave tiles
 // DIA row tiles whose offsets span at most kStageSpan (banded matrices)
 // stage the X rows they read in LDS: each X row is then fetched once per
 // tile instead of once per diagonal.  (Staging only the X rows of a group of
@@ -260,9 +274,10 @@ __device__ void tile_rows(const BatchTile& t, int init, const int32_t* __restric
   for (int i = 0; i < R; i++) {
     int32_t r;
     bool valid;
-    if constexpr (LIST) {  // lane group g: entry r0 + g of the row list (r1 entries)
-      valid = g < t.r1;
-      r = rows[t.r0 + (valid ? g : 0)];
+    if constexpr (LIST) {  // lane group g: entries r0 + g + RS i of the row list (r1 entries)
+      const int32_t e = g + L::RS * i;
+      valid = e < t.r1;
+      r = rows[t.r0 + (valid ? e : 0)];
     } else {
       r = t.r0 + g + L::RS * i;
       valid = r < t.r1;
@@ -465,6 +480,43 @@ __device__ void tile_wave(const BatchTile& t, const int32_t* __restrict__ rows, 
   }
 }
 
+// kSubRows rows per wave (a row list): each row's 64 / kSubRows lanes, G =
+// 64 / (kSubRows KL) entry groups, stride over it.  For rows of a few dozen
+// items a whole wave per row leaves most of its lanes' batch slots empty and
+// takes four times the tiles.
+template <typename V, int KR, int PRE>
+__device__ void tile_sub(const BatchTile& t, const int32_t* __restrict__ rows, int init) {
+  using L = Lay<V, KR>;
+  using Acc = typename L::Acc;
+  constexpr int S = (kWave / L::KL < kSubRows) ? kWave / L::KL : kSubRows;
+  constexpr int LW = kWave / S;  // lanes per row
+  constexpr int G = LW / L::KL;  // entry groups per row
+  const BatchMat& m = t.m;
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int sub = lane / LW, sl = lane % LW;
+  const int g = sl / L::KL, l = sl % L::KL;
+  const int i = wave * S + sub;
+  const bool valid = i < t.r1;
+  const int32_t r = rows[t.r0 + (valid ? i : 0)];
+  int64_t st, len;
+  csr_extent(m, r, st, len);
+  if (!valid) len = 0;
+  const Gp<typename L::Vec> x = gp<typename L::Vec>(m.x);
+  Acc acc[1][L::Q];
+#pragma unroll
+  for (int q = 0; q < L::Q; q++) acc[0][q] = 0;
+  row_entries<V, PRE>(m, r, st, len, g, G, [&](V(&v)[kB], const int32_t(&c)[kB], uint32_t ok) {
+    fma_batch<V, KR, 1>(acc, v, c, ok, x, l);
+  });
+  // (xor distances below LW stay inside the row's lanes)
+#pragma unroll
+  for (int q = 0; q < L::Q; q++)
+    for (int s = LW >> 1; s >= L::KL; s >>= 1) acc[0][q] += shfl_xor(acc[0][q], s);
+  if (valid && g == 0)
+    store_piece<typename L::Vec, L::Q, Acc>(
+        gm<typename L::Vec>(m.y) + (int64_t)r * KR + l * L::Q, acc[0], init);
+}
+
 // one row per workgroup: its G = 256 / KL entry groups stride over the row
 template <typename V, int KR, int PRE>
 __device__ void tile_whole(const BatchTile& t, int init, typename Lay<V, KR>::Acc* s_red) {
@@ -538,6 +590,10 @@ __global__ __launch_bounds__(kBT) __attribute__((amdgpu_waves_per_eu(5))) void b
     case tile_code(TILE_LIST, BPRE_NONE, true):
       tile_rows<V, KR, 1, BPRE_NONE, true, false, true>(t, init, rows);
       break;
+    case tile_code(TILE_SORTED, BPRE_NONE, true):
+      tile_rows<V, KR, RC, BPRE_NONE, true, false, true>(t, init, rows);
+      break;
+    case tile_code(TILE_SUB, BPRE_NONE, true): tile_sub<V, KR, BPRE_NONE>(t, rows, init); break;
     case tile_code(TILE_LIST, BPRE_DIA, true):
       tile_rows<V, KR, 1, BPRE_DIA, true, false, true>(t, init, rows);
       break;
@@ -730,6 +786,17 @@ struct TileSet {
   std::vector<int32_t> rows;  // row ids of one-wave tiles (r0 indexes this list)
 };
 
+// MPH_BATCH_SORT_ROWS=0: short rows of CSR-shaped members in row order
+bool getenv_sorted() {
+  const char* env = std::getenv("MPH_BATCH_SORT_ROWS");
+  return !env || std::atoi(env) != 0;
+}
+// MPH_BATCH_SUB_MAX: the longest row of the sub-wave tiles (0: none)
+int64_t getenv_sub_max() {
+  const char* env = std::getenv("MPH_BATCH_SUB_MAX");
+  return env ? std::max(0, std::atoi(env)) : kSubRowMax;
+}
+
 void tile_member(TileSet& ts, const BatchMat& m, const std::vector<int32_t>* ro, int vt, int k,
                  const BatchShape& sh, int wave_rows, bool stage_ok) {
   const int R = ro ? sh.rc : sh.r;
@@ -761,6 +828,14 @@ void tile_member(TileSet& ts, const BatchMat& m, const std::vector<int32_t>* ro,
     ts.rows.insert(ts.rows.end(), wave.begin(), wave.end());
     wave.clear();
   };
+  std::vector<int32_t> subl;
+  auto flush_sub = [&]() {
+    if (subl.empty()) return;
+    ts.tiles.push_back(tile(TILE_SUB, (int64_t)ts.rows.size(), (int64_t)subl.size()));
+    ts.rows.insert(ts.rows.end(), subl.begin(), subl.end());
+    subl.clear();
+  };
+  const int64_t sub_max = P == 0 ? getenv_sub_max() : 0;
   std::vector<int32_t> list;
   auto flush_list = [&]() {
     if (list.empty()) return;
@@ -769,6 +844,27 @@ void tile_member(TileSet& ts, const BatchMat& m, const std::vector<int32_t>* ro,
     list.clear();
   };
   const int64_t row_max = R > 1 ? kShortRowMax : kGroupRowMax;
+  // a member without a DIA / ELL part, R rows per lane group: its short rows
+  // (at most row_max items) sorted by item count into row-list tiles, so a
+  // wave's rows have about the same length and its batches end with them
+  // (in row order a wave of power-law rows mostly runs to its one longest)
+  const bool sorted = R > 1 && P == 0 && getenv_sorted();
+  if (sorted) {
+    std::vector<std::vector<int32_t>> by_len((size_t)row_max + 1);
+    for (int64_t r = 0; r < m.nrows; r++) {
+      const int64_t items = (int64_t)(*ro)[r + 1] - (*ro)[r];
+      if (items <= row_max) by_len[(size_t)items].push_back((int32_t)r);
+    }
+    std::vector<int32_t> order;
+    order.reserve((size_t)m.nrows);
+    for (const auto& b : by_len) order.insert(order.end(), b.begin(), b.end());
+    const int64_t per = R * rs;
+    for (size_t i = 0; i < order.size(); i += (size_t)per) {
+      const int64_t cnt = std::min<int64_t>(per, (int64_t)(order.size() - i));
+      ts.tiles.push_back(tile(TILE_SORTED, (int64_t)ts.rows.size(), cnt));
+      ts.rows.insert(ts.rows.end(), order.begin() + (int64_t)i, order.begin() + (int64_t)i + cnt);
+    }
+  }
   for (int64_t r0 = 0; r0 < m.nrows; r0 += R * rs) {
     const int64_t r1 = std::min(m.nrows, r0 + R * rs);
     for (int64_t r = r0; r < r1; r++) {
@@ -783,16 +879,20 @@ void tile_member(TileSet& ts, const BatchMat& m, const std::vector<int32_t>* ro,
         ts.comp.push_back(tile(TILE_WHOLE, r, r + 1));
       } else if (len > kWaveRowMax) {
         ts.tiles.push_back(tile(TILE_WHOLE, r, r + 1));
+      } else if (len <= sub_max) {
+        subl.push_back((int32_t)r);
+        if ((int)subl.size() == kSubRows * kBT / kWave) flush_sub();
       } else {
         wave.push_back((int32_t)r);
         if ((int)wave.size() == wave_rows * kBT / kWave) flush_wave();
       }
     }
     // the block's tile covers every row of [r0, r1); its lane groups leave
-    // the longer rows to their own tiles
-    ts.tiles.push_back(tile(TILE_ROWS, r0, r1));
+    // the longer rows to their own tiles (sorted: the short rows have theirs)
+    if (!sorted) ts.tiles.push_back(tile(TILE_ROWS, r0, r1));
   }
   flush_wave();
+  flush_sub();
   flush_list();
 }
 
