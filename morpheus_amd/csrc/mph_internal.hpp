@@ -38,11 +38,15 @@ constexpr int kXTileBytes = 32 << 10;          // x tile in LDS
 constexpr int kXTileCap = 512;                 // items of a (one-wave) sub-block
 constexpr int kXTileRows = 64;                 // rows of a sub-block
 constexpr int kXTileMaxGroupRows = 1024;       // rows of a group (run offsets in LDS)
-// SpMM with x-tiled rows: the x-tiled pass runs once per column for k up to
+// SpMM with x-tiled rows: the x-tiled rows take the x-tiled pass for k up to
 // kXTileSpmmMaxK, and for k up to kXTileSpmmLongRunMaxK when the tiled rows
-// average at least kXTileSpmmLongRun items per (row, tile) run
+// average at least kXTileSpmmLongRun items per (row, tile) run -- one pass
+// per column (a plan without the value copy), or the k-wide pass over
+// groups of 4 / 2 columns (the Kw limits)
 constexpr int kXTileSpmmMaxK = 2;
 constexpr int kXTileSpmmLongRunMaxK = 4;
+constexpr int kXTileSpmmKwMaxK = 4;
+constexpr int kXTileSpmmKwLongRunMaxK = 8;
 constexpr int64_t kXTileSpmmLongRun = 32;
 constexpr int64_t kXTileMaxBounds = 1 << 26;  // entries of the per-row tile-bound table (rows x (tiles + 1))
 constexpr int64_t kXTileMinXBytes = 2 << 20;  // auto: x larger than this
@@ -72,6 +76,10 @@ struct XTilePlan {
   uint16_t* col = nullptr;       // tile-major copy: column - tile * tile width
   void* val = nullptr;           // tile-major copy: values
   void* partial = nullptr;       // [ntiles * rows] (pairs in FP32), tile-major
+  // the k-wide pass's partials, [ntiles * rows * partial_k_cols] (pairs in
+  // FP32), allocated by its first launch
+  void* partial_k = nullptr;
+  int32_t partial_k_cols = 0;
   // direct plan (no copy of any value): the position in the caller's arrays
   // of each run's first item (lro's shape), and the caller's value array the
   // pass reads instead of `val`
@@ -266,6 +274,13 @@ int xtile_build(mph_matrix_impl* A, const std::vector<int32_t>& ro, const int32_
 void xtile_release(XTilePlan& xp);
 int xtile_multiply(mph_matrix_impl* A, const XTilePlan& xp, const void* x, void* y, int init,
                    int64_t ld, hipStream_t s);
+// The same pass for kr = 2 or 4 columns at once (a plan with the value copy):
+// the x tile's kr columns staged together, each item read once; column j
+// equals xtile_multiply's result over column j bit for bit.
+int xtile_multiply_k(mph_matrix_impl* A, XTilePlan& xp, int kr, const void* x, void* y, int init,
+                     int64_t ld, hipStream_t s);
+// the widest such pass (4, 2; 1: none) whose LDS tile the device allows
+int xtile_k_cols(const mph_matrix_impl* A, const XTilePlan& xp);
 // HYB / COO: row offsets of a row-sorted COO entry list (ro has nrows + 1 entries)
 int coo_row_offsets(const int32_t* ri, int64_t nnz, int64_t nrows, int32_t* ro, hipStream_t s);
 int default_chunk(int vt);

@@ -30,6 +30,7 @@
 //  * FP32 accumulation (FP32 / BF16 storage) uses TwoSum-compensated sums on
 //    rows longer than kCompensateLen items.
 #include <algorithm>
+#include <cstdlib>
 
 #include "mph_internal.hpp"
 
@@ -682,6 +683,208 @@ __global__ __launch_bounds__(kXTileThreads) void xtile_kernel(RBArgs<V> a, XTArg
   }
 }
 
+// The x-tiled pass for KR right-hand sides at once (X, Y row-major with
+// leading dimension a.ld, the group's first column at a.x / a.y): the
+// workgroup stages the KR columns of its x tile in LDS (entry (c, j) at
+// c * KR + j) and each item's value meets its KR entries.  Every column is
+// reduced exactly as xtile_kernel reduces its one column -- the same runs,
+// products, lane-serial sums and scan -- so column j equals the
+// single-column pass over X[:, j] bit for bit, while the copy's items are
+// read once for the KR columns.  Partials per (tile, row, column) in partk.
+template <int CTRL, int ROWS, typename S, int KR>
+__device__ __forceinline__ void xscan_step_k(S (&acc)[KR], int& flag, bool valid) {
+  S o[KR];
+#pragma unroll
+  for (int j = 0; j < KR; j++) o[j] = acc[j].template mv<CTRL, ROWS>();
+  const int of = dpp_i32<CTRL, ROWS>(flag);
+  if (valid && !flag) {
+#pragma unroll
+    for (int j = 0; j < KR; j++) acc[j].merge(o[j]);
+  }
+  if (valid) flag |= of;
+}
+
+inline size_t xtile_k_lds_bytes(size_t vec, int32_t tile, int kr) {
+  return vec * (size_t)tile * (size_t)kr + sizeof(int32_t) * (kXTileMaxGroupRows + 2) +
+         sizeof(XTSub) * (kXTileMaxGroupRows + 1);
+}
+
+template <typename V, bool COMP, int KR>
+__global__ __launch_bounds__(kXTileThreads) void xtile_kernel_k(RBArgs<V> a, XTArgs t, void* partk) {
+  using Acc = typename vtraits<V>::acc;
+  using Vec = typename vtraits<V>::vec;
+  using S = XSum<Acc, COMP>;
+  constexpr int kWaves = kXTileThreads / kWave;
+  extern __shared__ __align__(16) unsigned char smem[];
+  // LDS: x tile (KR columns) | run offsets of the group | sub-block records
+  Vec* xs = reinterpret_cast<Vec*>(smem);
+  int32_t* s_lro = reinterpret_cast<int32_t*>(xs + (size_t)t.tile * KR);
+  XTSub* s_sub = reinterpret_cast<XTSub*>(s_lro + kXTileMaxGroupRows + 2);
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const uint32_t id = blockIdx.x;
+  const uint32_t k = id / kNumXcd;
+  const int32_t g = (int32_t)(k % (uint32_t)t.groups);
+  const int32_t tile = (int32_t)((k / (uint32_t)t.groups) * kNumXcd + id % kNumXcd);
+  if (tile >= t.ntiles) return;  // whole workgroup: no barrier reached
+  const int64_t c0 = (int64_t)tile * t.tile;
+  const int32_t nx = (int32_t)min<int64_t>(t.tile, a.ncols - c0);
+  const int64_t blk = (int64_t)g * t.ntiles + tile;
+  const int32_t* lro = t.lro + blk * (t.gr + 1);
+  const int32_t gstart = g * t.gr;
+  const int32_t sb0 = t.sub_start[blk];
+  const int32_t nsub = t.sub_start[blk + 1] - 1 - sb0;
+  const V* cval = static_cast<const V*>(t.val);
+  Acc* part = static_cast<Acc*>(partk);
+
+  XTSub s0{0, 0}, s1{0, 0};
+  XTLoad<V> cur;
+  if (wave < nsub) {
+    s0 = uniform_sub(t.sub[sb0 + wave]);
+    s1 = uniform_sub(t.sub[sb0 + wave + 1]);
+    xtile_load<V>(t, cval, s0, s1.e0 - s0.e0, sb0 + wave, lane, cur);
+  }
+  {
+    constexpr int kPer = 8;
+    const int32_t ne = nx * KR;
+    for (int32_t j0 = threadIdx.x; j0 < ne; j0 += kXTileThreads * kPer) {
+      Vec xv[kPer];
+#pragma unroll
+      for (int q = 0; q < kPer; q++) {
+        const int32_t e = j0 + q * kXTileThreads;
+        xv[q] = e < ne ? a.x[(c0 + e / KR) * a.ld + e % KR] : (Vec)0;
+      }
+#pragma unroll
+      for (int q = 0; q < kPer; q++) {
+        const int32_t e = j0 + q * kXTileThreads;
+        if (e < ne) xs[e] = xv[q];
+      }
+    }
+  }
+  for (int32_t j = threadIdx.x; j <= t.gr; j += kXTileThreads) s_lro[j] = lro[j];
+  for (int32_t j = threadIdx.x; j <= nsub; j += kXTileThreads) s_sub[j] = t.sub[sb0 + j];
+  __syncthreads();
+
+  for (int32_t q = wave; q < nsub; q += kWaves) {
+    const int32_t nr = s1.i0 - s0.i0, n = s1.e0 - s0.e0;
+    const int32_t i0 = s0.i0, e0 = s0.e0;
+    const int32_t* ro = s_lro + (i0 - gstart);
+    // FP64: the products; FP32 / BF16: value and x kept apart for Dot2
+    Acc pa[kBatch], px[kBatch][KR];
+#pragma unroll
+    for (int b = 0; b < kBatch; b++) {
+      const uint32_t cc = min(cur.c[b], (uint32_t)(nx - 1));
+      const Acc av = XTHeld<V>::use(cur.v[b]);
+#pragma unroll
+      for (int j = 0; j < KR; j++) {
+        const Acc xv = (Acc)xs[cc * KR + j];
+        if constexpr (COMP) {
+          px[b][j] = xv;
+        } else {
+          px[b][j] = av * xv;
+        }
+      }
+      pa[b] = av;
+    }
+    const int32_t hd = (int32_t)min(cur.h, (uint32_t)(nr - 1));
+    auto put = [&](int32_t r, const S (&v)[KR]) {
+      const int64_t o = ((int64_t)tile * t.rows + i0 + r) * KR;
+#pragma unroll
+      for (int j = 0; j < KR; j++) {
+        if constexpr (COMP) {
+          part[2 * (o + j)] = v[j].s;
+          part[2 * (o + j) + 1] = v[j].c;
+        } else {
+          part[o + j] = v[j].s;
+        }
+      }
+    };
+    const int32_t jb = lane * kBatch;
+    const int cnt = (int)max(0, min(kBatch, n - jb));
+    const bool active = cnt > 0;
+    const int32_t r = active ? hd : 0;
+    const int32_t h = r;
+    int32_t cur_r = r;
+    S head[KR], run[KR];
+#pragma unroll
+    for (int j = 0; j < KR; j++) {
+      head[j] = S::zero();
+      run[j] = S::zero();
+    }
+    bool single = true;
+    const int32_t base = e0 + jb;
+    int32_t next_rel = active ? ro[cur_r + 1] - base : 0;
+#pragma unroll
+    for (int b = 0; b < kBatch; b++) {
+      if (b < cnt) {
+        if (b >= next_rel) {
+          if (single) {
+#pragma unroll
+            for (int j = 0; j < KR; j++) head[j] = run[j];
+            single = false;
+          } else {
+            put(cur_r, run);
+          }
+          do {
+            cur_r++;
+            next_rel = ro[cur_r + 1] - base;
+          } while (b >= next_rel);
+#pragma unroll
+          for (int j = 0; j < KR; j++) run[j] = S::zero();
+        }
+#pragma unroll
+        for (int j = 0; j < KR; j++) {
+          if constexpr (COMP)
+            run[j].add_prod(pa[b], px[b][j]);
+          else
+            run[j].add(px[b][j]);
+        }
+      }
+    }
+    const int32_t tl = cur_r;
+    if (q + kWaves < nsub) {
+      s0 = uniform_sub(s_sub[q + kWaves]);
+      s1 = uniform_sub(s_sub[q + kWaves + 1]);
+      xtile_load<V>(t, cval, s0, s1.e0 - s0.e0, sb0 + q + kWaves, lane, cur);
+    }
+    const int32_t t_prev = dpp_i32<kDppWaveShr1>(tl);
+    const bool cont = active && lane > 0 && single && h == t_prev;
+    S acc[KR];
+#pragma unroll
+    for (int j = 0; j < KR; j++) acc[j] = active ? run[j] : S::zero();
+    int flag = cont ? 0 : 1;
+    const int rl = lane & 15;
+    xscan_step_k<kDppRowShr1, 0xf>(acc, flag, rl >= 1);
+    xscan_step_k<kDppRowShr2, 0xf>(acc, flag, rl >= 2);
+    xscan_step_k<kDppRowShr4, 0xf>(acc, flag, rl >= 4);
+    xscan_step_k<kDppRowShr8, 0xf>(acc, flag, rl >= 8);
+    xscan_step_k<kDppRowBcast15, 0xa>(acc, flag, (lane & 16) != 0);
+    xscan_step_k<kDppRowBcast31, 0xc>(acc, flag, lane >= 32);
+    S prev_acc[KR];
+#pragma unroll
+    for (int j = 0; j < KR; j++) prev_acc[j] = acc[j].template mv<kDppWaveShr1, 0xf>();
+    const int32_t h_next = dpp_i32<kDppWaveShl1>(h);
+    const int last_lane = (n - 1) / kBatch;
+    if (active) {
+      if (!single) {
+        S hv[KR];
+#pragma unroll
+        for (int j = 0; j < KR; j++) {
+          hv[j] = head[j];
+          if (lane > 0 && h == t_prev) hv[j].merge(prev_acc[j]);
+        }
+        put(h, hv);
+      }
+      if (lane == last_lane || h_next != tl) put(tl, acc);
+    }
+    if (lane < nr && ro[lane] == ro[lane + 1]) {
+      S z[KR];
+#pragma unroll
+      for (int j = 0; j < KR; j++) z[j] = S::zero();
+      put(lane, z);
+    }
+  }
+}
+
 // The x-tiled pass of a direct plan (no copy of any value): the columns come
 // from the column copy (stored in item order), the values from the caller's
 // array.  Run r of a sub-block holds its block items ro[r] .. ro[r + 1] - 1,
@@ -962,6 +1165,79 @@ __global__ __launch_bounds__(256) void xtile_combine_kernel(RBArgs<V> a, XTArgs 
     Acc tot = 0;
     for (int q = 0; q < kXCombRanges; q++) tot += s_sum[q][ri];
     store_y(a.y, r * a.ld, tot + pre[0], a.init);
+  }
+}
+
+// The combine of the k-wide pass: xtile_combine_kernel (no DIA / ELL part)
+// for each of the KR columns, in the same order, into y[r * ld + j].
+template <typename V, bool COMP, int KR>
+__global__ __launch_bounds__(256) void xtile_combine_k_kernel(RBArgs<V> a, XTArgs t,
+                                                              const void* partk) {
+  using Acc = typename vtraits<V>::acc;
+  __shared__ Acc s_sum[KR][kXCombRanges][kXCombRows];
+  __shared__ Acc s_err[KR][kXCombRanges][kXCombRows];
+  const int ri = threadIdx.x % kXCombRows, rg = threadIdx.x / kXCombRows;
+  const int64_t i = (int64_t)blockIdx.x * kXCombRows + ri;
+  const Acc* part = static_cast<const Acc*>(partk);
+  const int32_t per = (t.ntiles + kXCombRanges - 1) / kXCombRanges;
+  const int32_t q0 = rg * per, q1 = min(t.ntiles, q0 + per);
+  csum<Acc> cs[KR];
+  Acc s[KR];
+#pragma unroll
+  for (int j = 0; j < KR; j++) {
+    cs[j] = csum<Acc>{0, 0};
+    s[j] = 0;
+  }
+  if (i < t.rows) {
+    for (int32_t qb = q0; qb < q1; qb += kBatch) {
+      Acc ps[kBatch][KR], pc[kBatch][KR];
+#pragma unroll
+      for (int b = 0; b < kBatch; b++) {
+        const int64_t o = ((int64_t)(qb + b) * t.rows + i) * KR;
+#pragma unroll
+        for (int j = 0; j < KR; j++) {
+          if constexpr (COMP) {
+            ps[b][j] = qb + b < q1 ? part[2 * (o + j)] : (Acc)0;
+            pc[b][j] = qb + b < q1 ? part[2 * (o + j) + 1] : (Acc)0;
+          } else {
+            ps[b][j] = qb + b < q1 ? part[o + j] : (Acc)0;
+            pc[b][j] = 0;
+          }
+        }
+      }
+#pragma unroll
+      for (int b = 0; b < kBatch; b++) {
+        if (qb + b < q1) {
+#pragma unroll
+          for (int j = 0; j < KR; j++) {
+            if constexpr (COMP)
+              cs[j].combine(ps[b][j], pc[b][j]);
+            else
+              s[j] += ps[b][j];
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < KR; j++) {
+    s_sum[j][rg][ri] = COMP ? cs[j].s : s[j];
+    s_err[j][rg][ri] = COMP ? cs[j].c : (Acc)0;
+  }
+  __syncthreads();
+  if (rg != 0 || i >= t.rows) return;
+  const int64_t r = t.row[i];
+#pragma unroll
+  for (int j = 0; j < KR; j++) {
+    if constexpr (COMP) {
+      csum<Acc> tot{0, 0};
+      for (int q = 0; q < kXCombRanges; q++) tot.combine(s_sum[j][q][ri], s_err[j][q][ri]);
+      store_y_cs(a.y, r * a.ld + j, tot, a.init);
+    } else {
+      Acc tot = 0;
+      for (int q = 0; q < kXCombRanges; q++) tot += s_sum[j][q][ri];
+      store_y(a.y, r * a.ld + j, tot + (Acc)0, a.init);
+    }
   }
 }
 
@@ -1768,6 +2044,7 @@ void xtile_release(XTilePlan& xp) {
   dfree(xp.col);
   dfree(xp.val);
   dfree(xp.partial);
+  dfree(xp.partial_k);
   dfree(xp.rpos);
   xp = XTilePlan{};
 }
@@ -2388,20 +2665,30 @@ template <typename V, int PRE>
 int launch_spmm(mph_matrix_impl* A, int k, const void* x, void* y, int init,
                 hipStream_t s) {
   using Vec = typename vtraits<V>::vec;
-  // A CSR-shaped matrix with x-tiled rows and few right-hand sides: the
-  // x-tiled pass runs once per column (x and y with stride k), reading the
-  // tile-major copy k times (10 B per item in FP64) instead of gathering a
-  // random k-wide row of X per item, which costs a 64-128 B line from
-  // beyond L2 once X (k times x) outgrows the caches; the other rows take
-  // the k-wide kernel over the split plan.  Measured: on a 2M-row
-  // power-law(1.2) matrix (runs of about 43 items) k = 2 / 4 run 2.1 / 1.4x
-  // faster and k = 8 1.2x slower; on a 1M-row power-law(1.5) matrix (runs of
-  // about 13) k = 2 1.1x faster and k = 4 1.1x slower.
+  // A CSR-shaped matrix with x-tiled rows and few right-hand sides: its
+  // x-tiled rows take the x-tiled pass (x and y with stride k) -- the k-wide
+  // pass over groups of 4 / 2 columns, which reads the tile-major copy once
+  // per group, or (a plan without the value copy) one pass per column --
+  // instead of gathering a random k-wide row of X per item, which costs a
+  // 64-128 B line from beyond L2 once X (k times x) outgrows the caches;
+  // the other rows take the k-wide kernel over the split plan.  Measured
+  // against those gathers: on a 2M-row power-law(1.2) matrix (runs of about
+  // 43 items) the k-wide pass runs k = 2 / 4 / 8 1.1 / 1.5 / 1.4x faster and
+  // k = 16 1.04x slower, one pass per column k = 2 / 4 2.1 / 1.4x faster and
+  // k = 8 1.2x slower; on a 1M-row power-law(1.5) matrix (runs of about 13)
+  // the k-wide pass k = 4 1.08x faster and k = 8 1.2x slower, one pass per
+  // column k = 2 1.1x faster and k = 4 1.1x slower.
   const XTilePlan& xp = A->rb.xt;
   const bool long_runs =
       xp.rows > 0 && xp.items >= kXTileSpmmLongRun * (int64_t)xp.rows * xp.ntiles;
-  const bool xt_cols = PRE == PRE_NONE && A->rb.sblocks && !A->rb.split && xp.rows > 0 &&
-                       (k <= kXTileSpmmMaxK || (long_runs && k <= kXTileSpmmLongRunMaxK));
+  const char* kw_env = std::getenv("MPH_XTILE_K");
+  const int kw = xp.rows == 0 || xp.rpos || (kw_env && std::atoi(kw_env) == 0)
+                     ? 1
+                     : xtile_k_cols(A, xp);
+  const int max_k = kw > 1 ? (long_runs ? kXTileSpmmKwLongRunMaxK : kXTileSpmmKwMaxK)
+                           : (long_runs ? kXTileSpmmLongRunMaxK : kXTileSpmmMaxK);
+  const bool xt_cols =
+      PRE == PRE_NONE && A->rb.sblocks && !A->rb.split && xp.rows > 0 && k <= max_k;
   // split k into column groups of 16, 8, 4, 2, 1 sharing the leading dim k
   int done = 0;
   while (done < k) {
@@ -2432,10 +2719,23 @@ int launch_spmm(mph_matrix_impl* A, int k, const void* x, void* y, int init,
     }
     if (st != MPH_OK) return st;
   }
-  if (xt_cols)
-    for (int j = 0; j < k; j++)
+  if (xt_cols) {
+    // the x-tiled rows: groups of 4 / 2 columns through the k-wide pass (a
+    // plan with the value copy; MPH_XTILE_K=0 keeps one pass per column),
+    // the rest one column at a time
+    int j = 0;
+    if (kw > 1) {
+      for (; kw >= 4 && j + 4 <= k; j += 4)
+        MPH_CHECK_STATUS(xtile_multiply_k(A, A->rb.xt, 4, static_cast<const Vec*>(x) + j,
+                                          static_cast<Vec*>(y) + j, init, k, s));
+      for (; j + 2 <= k; j += 2)
+        MPH_CHECK_STATUS(xtile_multiply_k(A, A->rb.xt, 2, static_cast<const Vec*>(x) + j,
+                                          static_cast<Vec*>(y) + j, init, k, s));
+    }
+    for (; j < k; j++)
       MPH_CHECK_STATUS(xtile_multiply(A, A->rb.xt, static_cast<const Vec*>(x) + j,
                                       static_cast<Vec*>(y) + j, init, k, s));
+  }
   return MPH_OK;
 }
 
@@ -2514,7 +2814,93 @@ int xtile_multiply_t(mph_matrix_impl* A, const XTilePlan& xp, const void* x, voi
   return MPH_OK;
 }
 
+// The k-wide x-tiled pass over columns [0, KR) of X / Y (leading dimension
+// ld): its partials buffer and LDS attribute are set up by the first call.
+template <typename V, int KR>
+int xtile_multiply_k_t(mph_matrix_impl* A, XTilePlan& xp, const void* x, void* y, int init,
+                       int64_t ld, hipStream_t s) {
+  using Acc = typename vtraits<V>::acc;
+  using Vec = typename vtraits<V>::vec;
+  constexpr bool comp = std::is_same<Acc, float>::value;
+  const size_t xlds = xtile_k_lds_bytes(sizeof(Vec), xp.tile, KR);
+  if (xp.partial_k_cols < KR) {
+    void* d = nullptr;
+    const size_t bytes = sizeof(Acc) * (comp ? 2 : 1) * (size_t)xp.ntiles * (size_t)xp.rows * KR;
+    MPH_HIP_CHECK(hipMalloc(&d, bytes));
+    const hipError_t e = hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&xtile_kernel_k<V, comp, KR>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)xlds);
+    if (e != hipSuccess) {
+      dfree(d);
+      set_error(std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
+      return MPH_ERR_HIP;
+    }
+    dfree(xp.partial_k);
+    xp.partial_k = d;
+    xp.partial_k_cols = KR;
+  } else {
+    const hipError_t e = hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&xtile_kernel_k<V, comp, KR>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)xlds);
+    if (e != hipSuccess) {
+      set_error(std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
+      return MPH_ERR_HIP;
+    }
+  }
+  RBArgs<V> a{};
+  a.nrows = A->nrows;
+  a.ncols = A->ncols;
+  a.x = static_cast<const Vec*>(x);
+  a.y = static_cast<Vec*>(y);
+  a.ld = (int32_t)ld;
+  a.init = init;
+  XTArgs t = xt_args(xp);
+  const int64_t tiles8 = ceil_div((int64_t)xp.ntiles, (int64_t)kNumXcd) * kNumXcd;
+  hipLaunchKernelGGL((xtile_kernel_k<V, comp, KR>), dim3((unsigned)(tiles8 * xp.groups)),
+                     dim3(kXTileThreads), xlds, s, a, t, xp.partial_k);
+  hipLaunchKernelGGL((xtile_combine_k_kernel<V, comp, KR>),
+                     dim3((unsigned)ceil_div(xp.rows, kXCombRows)), dim3(256), 0, s, a, t,
+                     (const void*)xp.partial_k);
+  MPH_HIP_CHECK(hipGetLastError());
+  return MPH_OK;
+}
+
 }  // namespace
+
+// The widest k-wide pass (4, 2, or 1: none) whose LDS tile fits the device's
+// per-workgroup limit.
+int xtile_k_cols(const mph_matrix_impl* A, const XTilePlan& xp) {
+  int dev = 0, lim = 0, optin = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 1;
+  if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+    lim = 0;
+  if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess)
+    optin = 0;
+  const size_t cap = (size_t)std::max(lim, optin);
+  const size_t vec = vector_size(A->vt);
+  if (xtile_k_lds_bytes(vec, xp.tile, 4) <= cap) return 4;
+  if (xtile_k_lds_bytes(vec, xp.tile, 2) <= cap) return 2;
+  return 1;
+}
+
+// KR = 2 or 4 columns of a copy plan's x-tiled pass (the caller checks both).
+int xtile_multiply_k(mph_matrix_impl* A, XTilePlan& xp, int kr, const void* x, void* y, int init,
+                     int64_t ld, hipStream_t s) {
+  if (xp.rows == 0) return MPH_OK;
+  if (xp.rpos || (kr != 2 && kr != 4)) return runtime_error("k-wide x-tiled pass: unsupported plan");
+  switch (A->vt) {
+    case MPH_F64:
+      return kr == 4 ? xtile_multiply_k_t<double, 4>(A, xp, x, y, init, ld, s)
+                     : xtile_multiply_k_t<double, 2>(A, xp, x, y, init, ld, s);
+    case MPH_F32:
+      return kr == 4 ? xtile_multiply_k_t<float, 4>(A, xp, x, y, init, ld, s)
+                     : xtile_multiply_k_t<float, 2>(A, xp, x, y, init, ld, s);
+    case MPH_BF16:
+      return kr == 4 ? xtile_multiply_k_t<bf16, 4>(A, xp, x, y, init, ld, s)
+                     : xtile_multiply_k_t<bf16, 2>(A, xp, x, y, init, ld, s);
+  }
+  return invalid("unknown value type");
+}
 
 int xtile_multiply(mph_matrix_impl* A, const XTilePlan& xp, const void* x, void* y, int init,
                    int64_t ld, hipStream_t s) {
