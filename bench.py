@@ -59,7 +59,8 @@ def parse():
                     help="skip the format x precision sweep (FP64 / FP32 / BF16 on S7, Z15, B9)")
     ap.add_argument("--only", default="",
                     help="run one section alone and print its JSON (profiling): batch, "
-                         "large (FP64 formats on S27 and Z12), large-S27, large-Z12, p2")
+                         "large (FP64 formats on S27 and Z12), large-S27, large-Z12, p2, "
+                         "spmm (FP64 CSR SpMM on Z12, k = 1..16)")
     ap.add_argument("--formats", default="",
                     help="comma-separated subset of the step's formats (profiling one "
                          "format at a time; the JSON line then covers that subset)")
@@ -392,6 +393,56 @@ def run_large_autotuned(A, mname, ks=(1, 4, 16), reps=5):
             f"k{k}": round(res[dtype][f"k{k}"]["matrhs_per_s"] / res["f64"][f"k{k}"]["matrhs_per_s"], 3)
             for k in ks}
     res["tolerance"] = {"f32": 1e-5, "bf16": 1e-2}
+    return res
+
+
+def run_spmm_z12(ks=(1, 2, 4, 8, 16), reps=10):
+    """FP64 CSR SpMM on Z12 (2M rows, power-law(1.2), 57.5M nnz; its long
+    rows x-tiled) for k right-hand sides (row-major X / Y, U[-1, 1]): median
+    of `reps` products back to back, the algorithmic bytes and fraction of
+    8 TB/s, and column 0 against the per-row FP64 bound of the oracle.  With
+    the value copy the x-tiled rows take the k-wide x-tiled pass (groups of
+    4 / 2 columns) up to k = 8, the position chunks beyond."""
+    import numpy as np
+    import torch
+    import oracle  # the oracle: FP64 reference products and row scales
+    from morpheus_amd import device
+    from morpheus_amd import generators as gen
+    A = gen.Z12()
+    stream = torch.cuda.current_stream()
+    rng = np.random.default_rng(43)
+    D = device.to_device(A, "f64")
+    D.analyse()
+    nrow = np.diff(A.row_offsets)
+    res = {"matrix": "Z12", "nnz": int(A.nnz)}
+    for k in ks:
+        X = rng.uniform(-1.0, 1.0, (A.ncols, k))
+        x0 = np.ascontiguousarray(X[:, 0])
+        Xt = torch.from_numpy(X if k > 1 else x0).cuda()
+        Y = torch.empty((A.nrows, k) if k > 1 else (A.nrows,), dtype=torch.float64, device="cuda")
+        run = (lambda: D.multiply(Xt, Y, True, stream)) if k == 1 else \
+            (lambda: D.multiply_multi(Xt, Y, True, stream))
+        run()
+        torch.cuda.synchronize()
+        y0 = (Y if k == 1 else Y[:, 0]).cpu().numpy()
+        ts = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            run()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        ms = float(np.median(ts))
+        nbytes = D.algorithmic_bytes(k)
+        ref = oracle.spmv(A, x0)
+        bound = 4.0 * np.maximum(nrow, 1) * 2.0 ** -53 * oracle.absrow(A, x0)
+        res[f"k{k}"] = {"ms": round(ms, 4), "bytes_alg": int(nbytes),
+                        "hbm_frac": round(data_gbs(nbytes, ms) / HBM_PEAK_GBS, 4),
+                        "within_fp64_bound_col0": bool(np.all(np.abs(y0 - ref) <= bound))}
+        log(f"spmm Z12 k={k}: {ms * 1e3:.0f} us ({res[f'k{k}']['hbm_frac']:.2f} of 8 TB/s)")
+        del Xt, Y
+    D.close()
     return res
 
 
@@ -961,6 +1012,8 @@ def main():
                             matrices=("S27", "Z12") if sec == "large" else (sec.split("-")[1],))
         elif sec == "p2":
             res = run_p2()
+        elif sec == "spmm":
+            res = run_spmm_z12()
         else:
             sys.exit(f"unknown section {sec}")
         if args.calibrate_copy:

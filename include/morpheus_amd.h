@@ -182,8 +182,12 @@ int mph_matrix_plan_bytes(mph_matrix_t A, int init, int64_t* bytes);
  *                  of x staged in LDS, so their x reads are LDS reads instead
  *                  of scattered cache-line fetches (the tile is halved, down
  *                  to an eighth, when most of those rows are too dense for
- *                  it; SpMM runs the pass once per column for k <= 2, or
- *                  k <= 4 when the rows average 32+ items per tile).  On
+ *                  it; SpMM runs the pass over groups of 4 / 2 columns of
+ *                  the x tile for k <= 4, or k <= 8 when the rows average
+ *                  32+ items per tile, or, without the value copy or with
+ *                  the environment variable MPH_XTILE_K=0, once per column
+ *                  for k <= 2 / 4; where both take the x-tiled rows this
+ *                  way, each column comes out the same bit for bit).  On
  *                  (x-tiling) or from
  *                  65,536 rows, HYB / HDC run their ELL / DIA part as its own
  *                  pass and the COO / CSR part over the rows that have one;
