@@ -183,7 +183,7 @@ int mph_matrix_plan_bytes(mph_matrix_t A, int init, int64_t* bytes);
  *                  of scattered cache-line fetches (the tile is halved, down
  *                  to an eighth, when most of those rows are too dense for
  *                  it; SpMM runs the pass over groups of 4 / 2 columns of
- *                  the x tile for k <= 4, or k <= 8 when the rows average
+ *                  the x tile for k <= 4, or k <= 16 when the rows average
  *                  32+ items per tile, or, without the value copy or with
  *                  the environment variable MPH_XTILE_K=0, once per column
  *                  for k <= 2 / 4; where both take the x-tiled rows this

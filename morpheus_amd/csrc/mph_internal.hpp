@@ -46,7 +46,7 @@ constexpr int kXTileMaxGroupRows = 1024;       // rows of a group (run offsets i
 constexpr int kXTileSpmmMaxK = 2;
 constexpr int kXTileSpmmLongRunMaxK = 4;
 constexpr int kXTileSpmmKwMaxK = 4;
-constexpr int kXTileSpmmKwLongRunMaxK = 8;
+constexpr int kXTileSpmmKwLongRunMaxK = 16;
 constexpr int64_t kXTileSpmmLongRun = 32;
 constexpr int64_t kXTileMaxBounds = 1 << 26;  // entries of the per-row tile-bound table (rows x (tiles + 1))
 constexpr int64_t kXTileMinXBytes = 2 << 20;  // auto: x larger than this

@@ -704,17 +704,21 @@ __device__ __forceinline__ void xscan_step_k(S (&acc)[KR], int& flag, bool valid
   if (valid) flag |= of;
 }
 
+// threads of a k-wide workgroup: its LDS leaves one workgroup per CU, so it
+// takes 16 waves (4 per SIMD; the kernel fits their 128 registers)
+constexpr int kXTileKThreads = 1024;
+
 inline size_t xtile_k_lds_bytes(size_t vec, int32_t tile, int kr) {
   return vec * (size_t)tile * (size_t)kr + sizeof(int32_t) * (kXTileMaxGroupRows + 2) +
          sizeof(XTSub) * (kXTileMaxGroupRows + 1);
 }
 
-template <typename V, bool COMP, int KR>
-__global__ __launch_bounds__(kXTileThreads) void xtile_kernel_k(RBArgs<V> a, XTArgs t, void* partk) {
+template <typename V, bool COMP, int KR, int NT>
+__global__ __launch_bounds__(NT) void xtile_kernel_k(RBArgs<V> a, XTArgs t, void* partk) {
   using Acc = typename vtraits<V>::acc;
   using Vec = typename vtraits<V>::vec;
   using S = XSum<Acc, COMP>;
-  constexpr int kWaves = kXTileThreads / kWave;
+  constexpr int kWaves = NT / kWave;
   extern __shared__ __align__(16) unsigned char smem[];
   // LDS: x tile (KR columns) | run offsets of the group | sub-block records
   Vec* xs = reinterpret_cast<Vec*>(smem);
@@ -746,22 +750,22 @@ __global__ __launch_bounds__(kXTileThreads) void xtile_kernel_k(RBArgs<V> a, XTA
   {
     constexpr int kPer = 8;
     const int32_t ne = nx * KR;
-    for (int32_t j0 = threadIdx.x; j0 < ne; j0 += kXTileThreads * kPer) {
+    for (int32_t j0 = threadIdx.x; j0 < ne; j0 += NT * kPer) {
       Vec xv[kPer];
 #pragma unroll
       for (int q = 0; q < kPer; q++) {
-        const int32_t e = j0 + q * kXTileThreads;
+        const int32_t e = j0 + q * NT;
         xv[q] = e < ne ? a.x[(c0 + e / KR) * a.ld + e % KR] : (Vec)0;
       }
 #pragma unroll
       for (int q = 0; q < kPer; q++) {
-        const int32_t e = j0 + q * kXTileThreads;
+        const int32_t e = j0 + q * NT;
         if (e < ne) xs[e] = xv[q];
       }
     }
   }
-  for (int32_t j = threadIdx.x; j <= t.gr; j += kXTileThreads) s_lro[j] = lro[j];
-  for (int32_t j = threadIdx.x; j <= nsub; j += kXTileThreads) s_sub[j] = t.sub[sb0 + j];
+  for (int32_t j = threadIdx.x; j <= t.gr; j += NT) s_lro[j] = lro[j];
+  for (int32_t j = threadIdx.x; j <= nsub; j += NT) s_sub[j] = t.sub[sb0 + j];
   __syncthreads();
 
   for (int32_t q = wave; q < nsub; q += kWaves) {
@@ -2673,11 +2677,11 @@ int launch_spmm(mph_matrix_impl* A, int k, const void* x, void* y, int init,
   // 64-128 B line from beyond L2 once X (k times x) outgrows the caches;
   // the other rows take the k-wide kernel over the split plan.  Measured
   // against those gathers: on a 2M-row power-law(1.2) matrix (runs of about
-  // 43 items) the k-wide pass runs k = 2 / 4 / 8 1.1 / 1.5 / 1.4x faster and
-  // k = 16 1.04x slower, one pass per column k = 2 / 4 2.1 / 1.4x faster and
-  // k = 8 1.2x slower; on a 1M-row power-law(1.5) matrix (runs of about 13)
-  // the k-wide pass k = 4 1.08x faster and k = 8 1.2x slower, one pass per
-  // column k = 2 1.1x faster and k = 4 1.1x slower.
+  // 43 items) the k-wide pass runs k = 4 / 8 / 16 1.65 / 1.6 / 1.08x faster,
+  // one pass per column k = 2 / 4 2.1 / 1.4x faster and k = 8 1.2x slower;
+  // on a 1M-row power-law(1.5) matrix (runs of about 13) the k-wide pass
+  // k = 4 1.17x faster and k = 8 1.17x slower, one pass per column k = 2
+  // 1.1x faster and k = 4 1.1x slower.
   const XTilePlan& xp = A->rb.xt;
   const bool long_runs =
       xp.rows > 0 && xp.items >= kXTileSpmmLongRun * (int64_t)xp.rows * xp.ntiles;
@@ -2828,7 +2832,7 @@ int xtile_multiply_k_t(mph_matrix_impl* A, XTilePlan& xp, const void* x, void* y
     const size_t bytes = sizeof(Acc) * (comp ? 2 : 1) * (size_t)xp.ntiles * (size_t)xp.rows * KR;
     MPH_HIP_CHECK(hipMalloc(&d, bytes));
     const hipError_t e = hipFuncSetAttribute(
-        reinterpret_cast<const void*>(&xtile_kernel_k<V, comp, KR>),
+        reinterpret_cast<const void*>(&xtile_kernel_k<V, comp, KR, kXTileKThreads>),
         hipFuncAttributeMaxDynamicSharedMemorySize, (int)xlds);
     if (e != hipSuccess) {
       dfree(d);
@@ -2840,7 +2844,7 @@ int xtile_multiply_k_t(mph_matrix_impl* A, XTilePlan& xp, const void* x, void* y
     xp.partial_k_cols = KR;
   } else {
     const hipError_t e = hipFuncSetAttribute(
-        reinterpret_cast<const void*>(&xtile_kernel_k<V, comp, KR>),
+        reinterpret_cast<const void*>(&xtile_kernel_k<V, comp, KR, kXTileKThreads>),
         hipFuncAttributeMaxDynamicSharedMemorySize, (int)xlds);
     if (e != hipSuccess) {
       set_error(std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
@@ -2856,8 +2860,8 @@ int xtile_multiply_k_t(mph_matrix_impl* A, XTilePlan& xp, const void* x, void* y
   a.init = init;
   XTArgs t = xt_args(xp);
   const int64_t tiles8 = ceil_div((int64_t)xp.ntiles, (int64_t)kNumXcd) * kNumXcd;
-  hipLaunchKernelGGL((xtile_kernel_k<V, comp, KR>), dim3((unsigned)(tiles8 * xp.groups)),
-                     dim3(kXTileThreads), xlds, s, a, t, xp.partial_k);
+  hipLaunchKernelGGL((xtile_kernel_k<V, comp, KR, kXTileKThreads>), dim3((unsigned)(tiles8 * xp.groups)),
+                     dim3(kXTileKThreads), xlds, s, a, t, xp.partial_k);
   hipLaunchKernelGGL((xtile_combine_k_kernel<V, comp, KR>),
                      dim3((unsigned)ceil_div(xp.rows, kXCombRows)), dim3(256), 0, s, a, t,
                      (const void*)xp.partial_k);

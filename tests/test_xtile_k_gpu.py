@@ -10,11 +10,11 @@ columns take the single-column pass.  Each column is reduced exactly as the
 single-column pass reduces it, so the product must equal, bit for bit, the
 same product with MPH_XTILE_K=0 (one pass per column) wherever that also
 takes the x-tiled route (k <= 4 for runs of 32 items or more per tile, k <=
-2 otherwise; beyond, the k-wide pass serves k <= 8 / 4 and the comparison is
-with the oracle alone), and stay within the FP64 per-row bound of the
+2 otherwise; beyond, the k-wide pass serves k <= 16 / 4 and the comparison
+is with the oracle alone), and stay within the FP64 per-row bound of the
 oracle.  Cases: long rows whose runs per tile average hundreds of items and
-short runs, k = 2, 3, 4 and 8, accumulation, FP32 and BF16, a plan without
-the value copy (the per-column pass), COO and HYB.
+short runs, k = 2, 3, 4, 8 and 16, accumulation, FP32 and BF16, a plan
+without the value copy (the per-column pass), COO and HYB.
 """
 import numpy as np
 import pytest
@@ -83,6 +83,7 @@ def test_device_lds_limit(gpu):
 
 @pytest.mark.parametrize("mat,k,same_route", [("long runs", 2, True), ("long runs", 3, True),
                                               ("long runs", 4, True), ("long runs", 8, False),
+                                              ("long runs", 16, False),
                                               ("short runs", 2, True), ("short runs", 4, False)])
 @pytest.mark.parametrize("init", [True, False])
 def test_kwide_equals_columns_fp64(gpu, monkeypatch, mat, k, same_route, init):
