@@ -1403,7 +1403,12 @@ __global__ __launch_bounds__(kRowBlockThreads) void rowblock_spmm_kernel(
   const int64_t row1 = a.blocks[b + 1].row;
   const int nr = (int)(row1 - row0);
   const int n = (int)(a.ro[row1] - a.ro[row0]);
-  int L = choose_lanes(n, nr, a.lanes_fixed);
+  // lanes per row: the analysis' choice for the block (from its longest rows
+  // per wave, as the SpMV pass uses it), else from the block's mean length
+  // (which gives one lane to a 500-entry row among one-entry rows)
+  int L = a.lanes_fixed > 0 ? a.lanes_fixed
+          : a.krec         ? a.krec[4 * (int64_t)b + 3]
+                           : choose_lanes(n, nr, 0);
   if constexpr (PRE != PRE_NONE) {
     // keep at least a few lanes per row for the prefix part
     if (a.lanes_fixed == 0 && L < 4) L = 4;
