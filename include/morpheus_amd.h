@@ -159,7 +159,9 @@ int mph_matrix_plan_bytes(mph_matrix_t A, int init, int64_t* bytes);
 
 /* Launch-shape knobs the autotuner searches (0 = library default).
  *   rows_chunk   : nnz budget of a CSR row block (short rows) / long-row chunk
- *   lanes        : fixed lanes per row for CSR row blocks (1..64, 0 = adaptive)
+ *   lanes        : fixed lanes per row for CSR row blocks (1..64; 0 = per
+ *                  block, chosen by the analysis from the longest rows of
+ *                  each wave, for SpMV and SpMM alike)
  *   coo_items    : entries per lane of the COO segmented reduction (4, 8, 16);
  *                  0 = a row-sorted COO matrix runs the CSR row-block plan
  *                  over row offsets derived from its row indices (rows_chunk
