@@ -220,7 +220,12 @@ int mph_multiply(mph_matrix_t A, const void* x, void* y, int init,
 
 /* k right-hand sides (SpMM): X is ncols x k and Y is nrows x k, row-major
  * (element (i, j) at [i*k + j]).  No reference counterpart: the reference
- * multiply takes one DenseVector (core/src/Morpheus_Multiply.hpp:57-78). */
+ * multiply takes one DenseVector (core/src/Morpheus_Multiply.hpp:57-78).
+ * Asynchronous on `stream`.  On a handle with x-tiled long rows, the first
+ * product with k >= 2 allocates the partials of the k-wide x-tiled pass
+ * (kept by the handle until its analysis is released); later products
+ * allocate nothing, so a product to be captured into a hipGraph follows one
+ * eager call with the same k. */
 int mph_multiply_multi(mph_matrix_t A, int32_t k, const void* X, void* Y,
                        int init, mph_stream_t stream);
 
