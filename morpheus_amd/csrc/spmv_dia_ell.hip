@@ -361,6 +361,86 @@ __global__ __launch_bounds__(256) void dia_spmm_kernel(DiaArgs<V> a) {
   *yp = out;
 }
 
+// DIA SpMM with R consecutive rows per lane group: for each run of
+// diagonals whose offsets step by one (o, o + 1, ..., at most kDiaRunMax),
+// the group loads the R + m - 1 X pieces the run covers once and slides
+// over them, instead of one X piece per row and diagonal (the 27-point
+// stencil: 10 X loads for 8 rows x 3 diagonals instead of 24).  Each row
+// still sums its diagonals in storage order, as dia_spmm_kernel does, with
+// the same skips, so the two give identical results.  Loads are
+// unconditional (columns outside x read column 0, a row past the matrix its
+// last row) and the products selected.
+constexpr int kDiaRunMax = 3;
+
+template <typename V, int KL, int R>
+__global__ __launch_bounds__(256) void dia_spmm_runs_kernel(DiaArgs<V> a) {
+  using Acc = typename vtraits<V>::acc;
+  using Vec = typename vtraits<V>::vec;
+  using P = Piece<Vec>;
+  constexpr int Q = P::Q;
+  constexpr int W = R + kDiaRunMax - 1;
+  const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t row0 = ((int64_t)b * (256 / KL) + threadIdx.x / KL) * R;
+  const int l = threadIdx.x % KL;
+  if (row0 >= a.nrows) return;  // whole lane groups: no cross-group exchange below
+  const int64_t last = a.nrows - 1;
+  Acc t[R][Q];
+#pragma unroll
+  for (int i = 0; i < R; i++)
+#pragma unroll
+    for (int q = 0; q < Q; q++) t[i][q] = 0;
+  for (int d = 0; d < a.ndiag;) {
+    const int64_t o = a.off[d];
+    int m = 1;
+    while (m < kDiaRunMax && d + m < a.ndiag && (int64_t)a.off[d + m] == o + m) m++;
+    typename P::T xw[W];
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+      const int64_t col = row0 + o + i;
+      const bool in = i < R + m - 1 && col >= 0 && col < a.ncols;
+      xw[i] = *reinterpret_cast<const typename P::T*>(a.x + (in ? col : 0) * a.ld + l * Q);
+    }
+#pragma unroll
+    for (int j = 0; j < kDiaRunMax; j++) {
+      if (j < m) {
+        V v[R];
+#pragma unroll
+        for (int i = 0; i < R; i++) v[i] = ld_stream(a.val + a.pitch * (d + j) + min(row0 + i, last));
+        hold_raw(v);
+#pragma unroll
+        for (int i = 0; i < R; i++) {
+          const int64_t col = row0 + i + o + j;
+          const bool ok = row0 + i < a.nrows && col >= 0 && col < a.ncols;
+          const Acc vw = vtraits<V>::widen(v[i]);
+#pragma unroll
+          for (int q = 0; q < Q; q++) {
+            const Acc f = t[i][q] + vw * (Acc)xw[i + j][q];
+            t[i][q] = ok ? f : t[i][q];
+          }
+        }
+      }
+    }
+    d += m;
+  }
+#pragma unroll
+  for (int i = 0; i < R; i++) {
+    const int64_t row = row0 + i;
+    if (row < a.nrows) {
+      typename P::T* yp = reinterpret_cast<typename P::T*>(a.y + row * a.ld + l * Q);
+      typename P::T out;
+      if (a.init) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) out[q] = (Vec)t[i][q];
+      } else {
+        const typename P::T old = *yp;
+#pragma unroll
+        for (int q = 0; q < Q; q++) out[q] = (Vec)((Acc)old[q] + t[i][q]);
+      }
+      *yp = out;
+    }
+  }
+}
+
 template <typename V, int KL>
 __global__ __launch_bounds__(256) void ell_spmm_kernel(EllArgs<V> a) {
   using Acc = typename vtraits<V>::acc;
@@ -688,7 +768,22 @@ int launch_spmm_lanes(mph_matrix_impl* A, const void* x, void* y, int ld, int in
     a.y = static_cast<typename vtraits<V>::vec*>(y);
     a.ld = ld;
     a.init = init;
-    hipLaunchKernelGGL((dia_spmm_kernel<V, KL>), dim3((unsigned)nb), dim3(256), 0, s, a);
+    // the run form where it was measured faster: FP64 with 8 lanes per row
+    // (16 columns per group; S27 k = 16 1107 -> 964 us), not for fewer lanes
+    // (k = 4: 314 -> 685 us, each lane loading its 8 rows' values one by
+    // one) nor in FP32 (k = 16: 412 -> 510 us); MPH_DIA_SPMM_RUNS=0 / 1
+    // forces it off / on
+    const char* env = std::getenv("MPH_DIA_SPMM_RUNS");
+    const int mode = env ? std::atoi(env) : -1;
+    const bool runs = A->ncols > 0 &&
+                      (mode > 0 || (mode < 0 && std::is_same<V, double>::value && KL == 8));
+    if (runs) {
+      constexpr int R = 8;
+      const int64_t nbr = ceil_div(A->nrows, (int64_t)(256 / KL) * R);
+      hipLaunchKernelGGL((dia_spmm_runs_kernel<V, KL, R>), dim3((unsigned)nbr), dim3(256), 0, s, a);
+    } else {
+      hipLaunchKernelGGL((dia_spmm_kernel<V, KL>), dim3((unsigned)nb), dim3(256), 0, s, a);
+    }
   } else {
     EllArgs<V> a{};
     a.nrows = A->nrows;
