@@ -369,7 +369,8 @@ __global__ __launch_bounds__(256) void dia_spmm_kernel(DiaArgs<V> a) {
 // still sums its diagonals in storage order, as dia_spmm_kernel does, with
 // the same skips, so the two give identical results.  Loads are
 // unconditional (columns outside x read column 0, a row past the matrix its
-// last row) and the products selected.
+// last row) and the products selected; the workgroup loads the run's values
+// for its rows into LDS together, so no lane group leaves before the end.
 constexpr int kDiaRunMax = 3;
 
 template <typename V, int KL, int R>
@@ -379,10 +380,16 @@ __global__ __launch_bounds__(256) void dia_spmm_runs_kernel(DiaArgs<V> a) {
   using P = Piece<Vec>;
   constexpr int Q = P::Q;
   constexpr int W = R + kDiaRunMax - 1;
+  constexpr int kRows = 256 / KL * R;  // rows of the workgroup
+  // the run's values for the workgroup's rows, loaded by all its threads in
+  // one coalesced pass per diagonal (each lane then reads its R rows'
+  // values from here instead of R loads of its own)
+  __shared__ V s_val[kDiaRunMax][kRows];
   const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
-  const int64_t row0 = ((int64_t)b * (256 / KL) + threadIdx.x / KL) * R;
+  const int64_t wr0 = (int64_t)b * kRows;
+  const int gl = threadIdx.x / KL;
+  const int64_t row0 = wr0 + (int64_t)gl * R;
   const int l = threadIdx.x % KL;
-  if (row0 >= a.nrows) return;  // whole lane groups: no cross-group exchange below
   const int64_t last = a.nrows - 1;
   Acc t[R][Q];
 #pragma unroll
@@ -393,6 +400,7 @@ __global__ __launch_bounds__(256) void dia_spmm_runs_kernel(DiaArgs<V> a) {
     const int64_t o = a.off[d];
     int m = 1;
     while (m < kDiaRunMax && d + m < a.ndiag && (int64_t)a.off[d + m] == o + m) m++;
+    // the X rows first: the value stores below wait for every older load
     typename P::T xw[W];
 #pragma unroll
     for (int i = 0; i < W; i++) {
@@ -400,18 +408,19 @@ __global__ __launch_bounds__(256) void dia_spmm_runs_kernel(DiaArgs<V> a) {
       const bool in = i < R + m - 1 && col >= 0 && col < a.ncols;
       xw[i] = *reinterpret_cast<const typename P::T*>(a.x + (in ? col : 0) * a.ld + l * Q);
     }
+    __syncthreads();  // the previous run's values are read
+    for (int j = 0; j < m; j++)
+      for (int e = threadIdx.x; e < kRows; e += 256)
+        s_val[j][e] = ld_stream(a.val + a.pitch * (d + j) + min(wr0 + e, last));
+    __syncthreads();
 #pragma unroll
     for (int j = 0; j < kDiaRunMax; j++) {
       if (j < m) {
-        V v[R];
-#pragma unroll
-        for (int i = 0; i < R; i++) v[i] = ld_stream(a.val + a.pitch * (d + j) + min(row0 + i, last));
-        hold_raw(v);
 #pragma unroll
         for (int i = 0; i < R; i++) {
           const int64_t col = row0 + i + o + j;
           const bool ok = row0 + i < a.nrows && col >= 0 && col < a.ncols;
-          const Acc vw = vtraits<V>::widen(v[i]);
+          const Acc vw = vtraits<V>::widen(s_val[j][gl * R + i]);
 #pragma unroll
           for (int q = 0; q < Q; q++) {
             const Acc f = t[i][q] + vw * (Acc)xw[i + j][q];
@@ -768,15 +777,15 @@ int launch_spmm_lanes(mph_matrix_impl* A, const void* x, void* y, int ld, int in
     a.y = static_cast<typename vtraits<V>::vec*>(y);
     a.ld = ld;
     a.init = init;
-    // the run form where it was measured faster: FP64 with 8 lanes per row
-    // (16 columns per group; S27 k = 16 1107 -> 964 us), not for fewer lanes
-    // (k = 4: 314 -> 685 us, each lane loading its 8 rows' values one by
-    // one) nor in FP32 (k = 16: 412 -> 510 us); MPH_DIA_SPMM_RUNS=0 / 1
-    // forces it off / on
+    // the run form where it was measured faster: FP64 with 4 or more lanes
+    // per row (S27 k = 8 586 -> 498 us, k = 16 1108 -> 651 us), not with 1 or
+    // 2 (k = 2 213 -> 478 us, k = 4 315 -> 435 us) nor in FP32 (k = 4 124 ->
+    // 341 us, k = 16 421 -> 451 us); MPH_DIA_SPMM_RUNS=0 / 1 forces it off /
+    // on
     const char* env = std::getenv("MPH_DIA_SPMM_RUNS");
     const int mode = env ? std::atoi(env) : -1;
     const bool runs = A->ncols > 0 &&
-                      (mode > 0 || (mode < 0 && std::is_same<V, double>::value && KL == 8));
+                      (mode > 0 || (mode < 0 && std::is_same<V, double>::value && KL >= 4));
     if (runs) {
       constexpr int R = 8;
       const int64_t nbr = ceil_div(A->nrows, (int64_t)(256 / KL) * R);

@@ -6,9 +6,10 @@ row over the diagonals in storage order, entries whose column falls outside
 the matrix skipped), per right-hand side.
 
 The lane-group DIA SpMM kernel takes one row per lane group; the run form
-(by default in FP64 with 8 lanes per row; forced on / off here with
+(by default in FP64 with 4 or more lanes per row; forced on / off here with
 MPH_DIA_SPMM_RUNS=1 / 0) takes 8 consecutive rows and, for each run of
-diagonals whose offsets step by one, loads the X rows the run covers once.
+diagonals whose offsets step by one, loads the X rows the run covers once
+(the run's values through LDS).
 Each row sums its diagonals in the same order with the same skips, so the
 two must agree bit for bit.  Cases: 27- and 5-point stencils, a band,
 unsorted and repeated offsets, offsets past the matrix edge, rectangular
