@@ -177,12 +177,13 @@ int mph_matrix_plan_bytes(mph_matrix_t A, int init, int64_t* bytes);
  *                  the clusters average 8 diagonals or more.  The
  *                  environment variable MPH_DIA_STAGE, read at analysis,
  *                  sets 0 = no staging, 1 = the one-row kernels only.
- *                  The results do not depend on it.  FP64 DIA SpMM with 8
- *                  or more columns per lane group takes 8 consecutive rows
- *                  per group and loads, per run of diagonals with
- *                  consecutive offsets, the X rows the run covers once and
- *                  the run's values through LDS (MPH_DIA_SPMM_RUNS=0 / 1
- *                  forces that off / on; the results are the same bits).
+ *                  The results do not depend on it.  DIA SpMM with 4 or
+ *                  more lanes per row (FP64 from 8 columns per pass, FP32
+ *                  from 16) takes 4 consecutive rows per lane group and
+ *                  loads, per run of diagonals with consecutive offsets, the
+ *                  X rows the run covers once and the run's values through
+ *                  LDS (MPH_DIA_SPMM_RUNS=0 / 1 forces that off / on; the
+ *                  results are the same bits).
  *   xtile        : x-tiled long rows of CSR / COO / HYB / HDC SpMV (1 = off,
  *                  2 = on; 0 = on when x exceeds 2 MiB).  Rows longer than
  *                  512 entries with sorted columns are summed per 32 KiB tile

@@ -361,11 +361,11 @@ __global__ __launch_bounds__(256) void dia_spmm_kernel(DiaArgs<V> a) {
   *yp = out;
 }
 
-// DIA SpMM with R consecutive rows per lane group: for each run of
+// DIA SpMM with R consecutive rows per lane group (R = 4): for each run of
 // diagonals whose offsets step by one (o, o + 1, ..., at most kDiaRunMax),
 // the group loads the R + m - 1 X pieces the run covers once and slides
 // over them, instead of one X piece per row and diagonal (the 27-point
-// stencil: 10 X loads for 8 rows x 3 diagonals instead of 24).  Each row
+// stencil: 6 X loads for 4 rows x 3 diagonals instead of 12).  Each row
 // still sums its diagonals in storage order, as dia_spmm_kernel does, with
 // the same skips, so the two give identical results.  Loads are
 // unconditional (columns outside x read column 0, a row past the matrix its
@@ -777,17 +777,18 @@ int launch_spmm_lanes(mph_matrix_impl* A, const void* x, void* y, int ld, int in
     a.y = static_cast<typename vtraits<V>::vec*>(y);
     a.ld = ld;
     a.init = init;
-    // the run form where it was measured faster: FP64 with 4 or more lanes
-    // per row (S27 k = 8 586 -> 498 us, k = 16 1108 -> 651 us), not with 1 or
-    // 2 (k = 2 213 -> 478 us, k = 4 315 -> 435 us) nor in FP32 (k = 4 124 ->
-    // 341 us, k = 16 421 -> 451 us); MPH_DIA_SPMM_RUNS=0 / 1 forces it off /
-    // on
+    // the run form (4 rows per lane group) where it was measured faster: 4
+    // or more lanes per row in FP64 (S27 k = 8 587 -> 423 us, k = 16 1108 ->
+    // 630 us) and FP32 (k = 16 415 -> 374 us); not with fewer lanes (FP64
+    // k = 2 213 -> 255 us, k = 4 even; FP32 k = 4 124 -> 188 us, k = 8 209 ->
+    // 258 us) nor in BF16 (not measured); MPH_DIA_SPMM_RUNS=0 / 1 forces it
+    // off / on
     const char* env = std::getenv("MPH_DIA_SPMM_RUNS");
     const int mode = env ? std::atoi(env) : -1;
-    const bool runs = A->ncols > 0 &&
-                      (mode > 0 || (mode < 0 && std::is_same<V, double>::value && KL >= 4));
+    constexpr bool wide = KL >= 4 && !std::is_same<V, bf16>::value;
+    const bool runs = A->ncols > 0 && (mode > 0 || (mode < 0 && wide));
     if (runs) {
-      constexpr int R = 8;
+      constexpr int R = 4;
       const int64_t nbr = ceil_div(A->nrows, (int64_t)(256 / KL) * R);
       hipLaunchKernelGGL((dia_spmm_runs_kernel<V, KL, R>), dim3((unsigned)nbr), dim3(256), 0, s, a);
     } else {

@@ -6,8 +6,8 @@ row over the diagonals in storage order, entries whose column falls outside
 the matrix skipped), per right-hand side.
 
 The lane-group DIA SpMM kernel takes one row per lane group; the run form
-(by default in FP64 with 4 or more lanes per row; forced on / off here with
-MPH_DIA_SPMM_RUNS=1 / 0) takes 8 consecutive rows and, for each run of
+(by default in FP64 / FP32 with 4 or more lanes per row; forced on / off
+here with MPH_DIA_SPMM_RUNS=1 / 0) takes 4 consecutive rows and, for each run of
 diagonals whose offsets step by one, loads the X rows the run covers once
 (the run's values through LDS).
 Each row sums its diagonals in the same order with the same skips, so the
