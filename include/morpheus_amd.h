@@ -178,8 +178,8 @@ int mph_matrix_plan_bytes(mph_matrix_t A, int init, int64_t* bytes);
  *                  environment variable MPH_DIA_STAGE, read at analysis,
  *                  sets 0 = no staging, 1 = the one-row kernels only.
  *                  The results do not depend on it.  DIA SpMM with 4 or
- *                  more lanes per row (FP64 from 8 columns per pass, FP32
- *                  from 16) takes 4 consecutive rows per lane group and
+ *                  more lanes per row (FP64 from 8 columns per pass, FP32 /
+ *                  BF16 from 16) takes 4 consecutive rows per lane group and
  *                  loads, per run of diagonals with consecutive offsets, the
  *                  X rows the run covers once and the run's values through
  *                  LDS (MPH_DIA_SPMM_RUNS=0 / 1 forces that off / on; the

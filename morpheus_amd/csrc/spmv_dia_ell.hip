@@ -778,14 +778,14 @@ int launch_spmm_lanes(mph_matrix_impl* A, const void* x, void* y, int ld, int in
     a.ld = ld;
     a.init = init;
     // the run form (4 rows per lane group) where it was measured faster: 4
-    // or more lanes per row in FP64 (S27 k = 8 587 -> 423 us, k = 16 1108 ->
-    // 630 us) and FP32 (k = 16 415 -> 374 us); not with fewer lanes (FP64
-    // k = 2 213 -> 255 us, k = 4 even; FP32 k = 4 124 -> 188 us, k = 8 209 ->
-    // 258 us) nor in BF16 (not measured); MPH_DIA_SPMM_RUNS=0 / 1 forces it
-    // off / on
+    // or more lanes per row (S27 FP64 k = 8 587 -> 423 us, k = 16 1108 ->
+    // 630 us; FP32 k = 16 415 -> 374 us; BF16 k = 16 383 -> 347 us); not with
+    // fewer lanes (FP64 k = 2 213 -> 255 us, k = 4 even; FP32 k = 4 124 ->
+    // 188 us, k = 8 209 -> 258 us; BF16 k = 8 209 -> 254 us);
+    // MPH_DIA_SPMM_RUNS=0 / 1 forces it off / on
     const char* env = std::getenv("MPH_DIA_SPMM_RUNS");
     const int mode = env ? std::atoi(env) : -1;
-    constexpr bool wide = KL >= 4 && !std::is_same<V, bf16>::value;
+    constexpr bool wide = KL >= 4;
     const bool runs = A->ncols > 0 && (mode > 0 || (mode < 0 && wide));
     if (runs) {
       constexpr int R = 4;

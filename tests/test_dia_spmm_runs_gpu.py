@@ -6,11 +6,11 @@ row over the diagonals in storage order, entries whose column falls outside
 the matrix skipped), per right-hand side.
 
 The lane-group DIA SpMM kernel takes one row per lane group; the run form
-(by default in FP64 / FP32 with 4 or more lanes per row; forced on / off
-here with MPH_DIA_SPMM_RUNS=1 / 0) takes 4 consecutive rows and, for each run of
+(by default with 4 or more lanes per row; forced on / off here with
+MPH_DIA_SPMM_RUNS=1 / 0) takes 4 consecutive rows and, for each run of
 diagonals whose offsets step by one, loads the X rows the run covers once
-(the run's values through LDS).
-Each row sums its diagonals in the same order with the same skips, so the
+(the run's values through LDS).  Each row sums its diagonals in the same
+order with the same skips, so the
 two must agree bit for bit.  Cases: 27- and 5-point stencils, a band,
 unsorted and repeated offsets, offsets past the matrix edge, rectangular
 shapes, rows not a multiple of the group's rows, k = 4, 8, 16 and 32,
@@ -91,7 +91,8 @@ def test_runs_equal_groups_fp64(gpu, monkeypatch, name, k, init):
 
 
 @pytest.mark.parametrize("name", ["S27", "unsorted, repeated", "rectangular tall"])
-@pytest.mark.parametrize("dtype,k", [("f64", 8), ("f64", 32), ("f32", 8), ("f32", 16), ("bf16", 8)])
+@pytest.mark.parametrize("dtype,k", [("f64", 8), ("f64", 32), ("f32", 8), ("f32", 16), ("bf16", 8),
+                                     ("bf16", 16)])
 def test_runs_equal_groups_widths(gpu, monkeypatch, name, dtype, k):
     D = _cases()[name]
     rng = np.random.default_rng(72)
