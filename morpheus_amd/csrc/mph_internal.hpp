@@ -54,6 +54,9 @@ constexpr int64_t kXTileMinRowLen = 512;       // rows longer than this are cand
 // HYB / HDC with at least this many rows run the ELL / DIA part as its own
 // coalesced pass and the tail / CSR part over the rows that have one.
 constexpr int64_t kSplitMinRows = 1 << 16;
+// split FP64 HDC: CSR rows of at most this many entries are added by the DIA
+// pass's threads
+constexpr int64_t kHdcTailMax = 8;
 // HYB from kSplitMinRows rows (or with x-tiling on) whose ELL slots are
 // filled below kHybMergeMaxFill, or with more than 1 / kHybMergeTailRows of
 // its rows holding COO entries, runs over a merged CSR copy
@@ -129,6 +132,10 @@ struct RowBlockPlan {
   int32_t nsblocks = 0;
   int32_t nchunked = 0;          // long rows left to position chunks
   bool split = false;            // HYB / HDC: ELL / DIA pass, then the list
+  // split FP64 HDC whose CSR rows all hold at most kHdcTailMax entries (none
+  // x-tiled or chunked): the DIA pass adds each row's CSR entries itself
+  // (hdc_tail_multiply), so no list pass read-modify-writes scattered y
+  bool hdc_tail = false;
   int32_t* smap = nullptr;       // split: list row -> matrix row
   int32_t* sro = nullptr;        // split: item offsets of the list rows (+1)
   // split HYB whose ELL slots are mostly padding: per row, 1 + its last slot
@@ -228,6 +235,9 @@ int coo_multiply(mph_matrix_impl* A, int k, const void* x, void* y, int init,
                  hipStream_t s);
 // Clusters A's diagonals for the LDS-staged DIA SpMV (A->dia_stage).
 int dia_analyse(mph_matrix_impl* A, hipStream_t s);
+// HDC with hdc_tail set, k = 1: the staged DIA pass with each row's CSR
+// entries added after its diagonals
+int hdc_tail_multiply(mph_matrix_impl* A, const void* x, void* y, int init, hipStream_t s);
 int dia_multiply(mph_matrix_impl* A, int k, const void* x, void* y, int init,
                  hipStream_t s);
 int ell_multiply(mph_matrix_impl* A, int k, const void* x, void* y, int init,

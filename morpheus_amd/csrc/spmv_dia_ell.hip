@@ -34,6 +34,11 @@ struct DiaArgs {
   typename vtraits<V>::vec* __restrict__ y;
   int32_t ld;
   int init;
+  // dia_staged_kernel<V, true>: HDC's CSR part, added per row after the
+  // diagonals
+  const int32_t* __restrict__ tro;
+  const int32_t* __restrict__ tci;
+  const V* __restrict__ tval;
 };
 
 // t[u][q] += v[u][w] * x[c[u][w]][q] over a batch of 8 diagonals / slots,
@@ -144,7 +149,7 @@ __global__ __launch_bounds__(256) void dia_kernel(DiaArgs<V> a) {
 // entries from there instead of a global load (on the 27-point stencil the
 // x loads per row drop from 27 to about 7).  Diagonals are summed in storage
 // order, as dia_kernel sums them: the two kernels give identical results.
-template <typename V>
+template <typename V, bool TAIL = false>
 __global__ __launch_bounds__(256) void dia_staged_kernel(DiaArgs<V> a, DiaStage cl) {
   using Acc = typename vtraits<V>::acc;
   using Vec = typename vtraits<V>::vec;
@@ -181,6 +186,15 @@ __global__ __launch_bounds__(256) void dia_staged_kernel(DiaArgs<V> a, DiaStage 
       } else {
         if (ok[w]) t += vtraits<V>::widen(v[w]) * (Acc)a.x[col[w]];
       }
+    }
+  }
+  if constexpr (TAIL) {
+    // HDC: the row's CSR entries (a few at most), in order after the
+    // diagonals -- the reference's DIA product followed by its CSR product
+    // accumulating onto it
+    if (row < a.nrows) {
+      const int32_t s0 = a.tro[row], s1 = a.tro[row + 1];
+      for (int32_t j = s0; j < s1; j++) t += vtraits<V>::widen(a.tval[j]) * (Acc)a.x[a.tci[j]];
     }
   }
   if (row < a.nrows) a.y[row] = a.init ? (Vec)t : (Vec)((Acc)a.y[row] + t);
@@ -1003,6 +1017,31 @@ int dia_analyse(mph_matrix_impl* A, hipStream_t s) {
 int dia_multiply(mph_matrix_impl* A, int k, const void* x, void* y, int init,
                  hipStream_t s) {
   return dispatch<true>(A, k, x, y, init, s);
+}
+
+int hdc_tail_multiply(mph_matrix_impl* A, const void* x, void* y, int init, hipStream_t s) {
+  if (A->format != MPH_HDC || A->vt != MPH_F64 || A->dia_stage.n == 0)
+    return runtime_error("HDC tail pass: unsupported plan");
+  DiaArgs<double> a{};
+  a.nrows = A->nrows;
+  a.ncols = A->ncols;
+  a.ndiag = (int32_t)A->ndiag;
+  a.pitch = A->dia_pitch;
+  a.off = A->dia_off;
+  a.val = static_cast<const double*>(A->dia_v);
+  a.x = static_cast<const double*>(x);
+  a.y = static_cast<double*>(y);
+  a.ld = 1;
+  a.init = init;
+  a.tro = A->csr_ro;
+  a.tci = A->csr_ci;
+  a.tval = static_cast<const double*>(A->csr_v);
+  const int64_t nb = ceil_div(A->nrows, 256);
+  if (nb == 0) return MPH_OK;
+  hipLaunchKernelGGL((dia_staged_kernel<double, true>), dim3((unsigned)nb), dim3(256), 0, s, a,
+                     A->dia_stage);
+  MPH_HIP_CHECK(hipGetLastError());
+  return MPH_OK;
 }
 
 int ell_multiply(mph_matrix_impl* A, int k, const void* x, void* y, int init,

@@ -2316,6 +2316,15 @@ int split_analyse(mph_matrix_impl* A, const int32_t* d_ro, hipStream_t s) {
   p.nsblocks = (int32_t)sb.size() - 1;
   p.nchunked = nchunked;
   p.split = split;
+  {
+    // FP64 HDC whose CSR rows are all short, none tiled or chunked
+    // (MPH_HDC_TAIL=0 keeps the list pass)
+    const char* env = std::getenv("MPH_HDC_TAIL");
+    bool ok = split && A->format == MPH_HDC && A->vt == MPH_F64 && nchunked == 0 &&
+              p.xt.rows == 0 && !(env && std::atoi(env) == 0);
+    for (int64_t i = 0; ok && i < m; i++) ok = len(i) <= kHdcTailMax;
+    p.hdc_tail = ok;
+  }
   p.smap = d_map;
   p.sro = d_lro;
   if (split && A->format == MPH_HYB) MPH_CHECK_STATUS(ell_counts(A, s));
@@ -2608,6 +2617,8 @@ int launch_spmv(mph_matrix_impl* A, const void* x, void* y, int init,
   constexpr bool comp = std::is_same<Acc, float>::value;
   if (A->rb.sblocks) {
     if (!A->rb.split) return launch_split_spmv<V, PRE_NONE>(A, x, y, init, s);
+    if (A->rb.hdc_tail && A->dia_stage.n > 0 && A->tuning.rows_per_thread == 0)
+      return hdc_tail_multiply(A, x, y, init, s);
     // HYB / HDC: the ELL / DIA part of every row in one coalesced pass that
     // writes y (zeros when that part is empty; nothing to do then when
     // accumulating), then the tail / CSR part of the rows that have one is
