@@ -203,7 +203,11 @@ int mph_matrix_plan_bytes(mph_matrix_t A, int init, int64_t* bytes);
  *                  in which more than an eighth of the rows have COO
  *                  entries, runs instead as CSR over a merged copy made by
  *                  the analysis (per row its ELL entries, then its COO
- *                  entries); an HDC without diagonals runs as its CSR part. */
+ *                  entries); an FP64 HDC whose CSR rows hold at most 8
+ *                  entries each (none x-tiled) runs one DIA pass that adds
+ *                  each row's CSR entries after its diagonals
+ *                  (MPH_HDC_TAIL=0 keeps the two passes); an HDC without
+ *                  diagonals runs as its CSR part. */
 typedef struct mph_tuning {
   int32_t rows_chunk;
   int32_t lanes;
