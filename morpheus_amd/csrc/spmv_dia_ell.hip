@@ -16,6 +16,10 @@
 // beyond entries_per_row hold only the -1 padding.  FP32 / BF16 SpMV takes
 // two adjacent rows per thread (row-pair kernels below); FP32 / BF16 rows of
 // more than kCompensateLen diagonals / slots take a Dot2-compensated kernel.
+// SpMM gives each row a lane group (16 bytes of its columns per lane); with
+// 4 or more lanes per row DIA gives each group 4 consecutive rows and loads
+// the X rows of each run of consecutive diagonals once.  A split FP64 HDC
+// whose CSR rows are short adds them in the staged DIA pass.
 #include <algorithm>
 #include <cstdlib>
 
